@@ -1,0 +1,111 @@
+/*
+ * qcs_ext.h -- extension entry points of the MI355X build.
+ *
+ * None of these functions has a counterpart in the reference header (include/qcs.h of
+ * RQS-SVG); they are additive, change nothing declared in qcs.h, and follow the same
+ * convention: return 1 on success, 0 on failure with the message sent to the callback set
+ * by qcs_set_exception_callback.  Both libqcs.so (HIP) and the CPU oracle library export
+ * them.
+ *
+ * Definitions used below:
+ *   - "logical order": amplitude index bit q is logical qubit q (the save_statevector
+ *     order of the reference, src/qcs.cu:2515-2536); amplitudes are interleaved
+ *     (re, im) doubles.
+ *   - "physical order": the order in device memory; physical bit q holds logical qubit
+ *     perm_p2l[q].
+ *   - p(i) = fma(re_i, re_i, im_i * im_i), computed in FP64.
+ *   - canonical sum of a list of 2^m values: split into tiles of 4096 (the last one padded
+ *     with zeros), sum each tile by the stride-halving tree
+ *         for (s = 2048; s >= 1; s /= 2) for (i < s) x[i] = x[i] + x[i + s];
+ *     and apply the same rule to the list of tile sums until one value is left.
+ *   - probability prefix: the 64-ary right-nested prefix of p over physical order:
+ *     blocks of 64 consecutive entries are folded left to right, the list of block totals
+ *     is treated the same way, and the inclusive prefix at index i is
+ *         E_L (+) (E_{L-1} (+) ( ... (+) (E_1 (+) p(i))))
+ *     where E_k is the left-to-right sum of the totals that precede i's level-k block
+ *     inside its level-(k+1) block.  It is non-decreasing, and its last entry is the
+ *     "probability total".
+ *   - uniforms: u_j = (splitmix64(seed + (j + 1) * 0x9E3779B97F4A7C15) >> 11) * 2^-53,
+ *     where splitmix64 is the output mix of Steele, Lea & Flood (OOPSLA 2014).
+ *   - sample j: r_j = u_j * total; the outcome is the smallest physical index i whose
+ *     prefix is >= r_j (the tie rule of the reference's `p0 < r`,
+ *     src/qcs.cu:2615-2617), reported as a logical index.
+ */
+#pragma once
+#include "qcs.h"
+
+#ifdef __cplusplus
+extern "C"
+{
+#endif
+
+/* Kernel classes reported by qcs_ext_profile_get. */
+enum qcs_ext_kernel_class
+{
+    QCS_EXT_K_PAIR = 0,     /* 2x2 on an amplitude pair: dense 1-qubit, swap-like 2-qubit */
+    QCS_EXT_K_PHASE = 1,    /* one phase on a controlled subspace: z/s/t/p/cz/cp/global phase */
+    QCS_EXT_K_DIAG = 2,     /* diagonal on 1-2 targets: rz, rzz */
+    QCS_EXT_K_DENSE4 = 3,   /* dense 4x4: rxx, ryy, rzx, ecr */
+    QCS_EXT_K_MONO = 4,     /* one nonzero per row on up to 4 targets: dcx, rccx, rcccx */
+    QCS_EXT_K_REDUCE = 5,   /* probability reductions (measure, norm, prefix) */
+    QCS_EXT_K_COLLAPSE = 6, /* measurement collapse (+ renormalisation) */
+    QCS_EXT_K_SAMPLE = 7,   /* sampling search */
+    QCS_EXT_K_PERMUTE = 8,  /* qubit permutation / swap-in */
+    QCS_EXT_K_INIT = 9,     /* state preparation */
+    QCS_EXT_K_FUSED = 10,   /* fused k-qubit block (precision form) */
+    QCS_EXT_K_CAST = 11,    /* precision conversion */
+    QCS_EXT_K_COUNT = 12
+};
+
+/* Seed of the measurement engine (std::mt19937_64), which otherwise comes from
+ * std::random_device as in the reference (src/qcs.cu:1845-1852). */
+int qcs_ext_set_seed(qcs_simulator *sim, uint64_t seed);
+
+/* Copy the whole state out / in, logical order, 2 * num_amplitudes doubles. */
+int qcs_ext_get_statevector(qcs_simulator *sim, double *out, uint64_t num_amplitudes);
+int qcs_ext_set_statevector(qcs_simulator *sim, const double *in, uint64_t num_amplitudes);
+
+/* Seeded sampling of num_shots full-register outcomes (logical indices). */
+int qcs_ext_sample(qcs_simulator *sim, uint64_t seed, uint64_t num_shots, uint64_t *outcomes);
+
+/* The probability total of the prefix defined above, and the canonical sum of p(i). */
+int qcs_ext_probability_total(qcs_simulator *sim, double *total);
+int qcs_ext_norm(qcs_simulator *sim, double *norm);
+
+/* <Z_q> = sum_i p(i) * (1 - 2 * bit_q(i)) for logical qubits q < count, FP64. */
+int qcs_ext_expect_z(qcs_simulator *sim, double *out, int count);
+
+/* Renormalise the kept branch after a measurement collapse (default off: the reference
+ * leaves the collapsed state unnormalised, src/qcs.cu:1106-1123). */
+int qcs_ext_set_renormalize(qcs_simulator *sim, int enable);
+
+/* Treat physical qubits >= num_local_qubits as "global": a gate or measurement that
+ * targets one first swaps it into local slot num_local_qubits-1-i with the in-HBM
+ * permutation kernel and updates the mapping as the reference does
+ * (src/qcs.cu:2206-2332).  Default: all qubits local. */
+int qcs_ext_set_local_qubits(qcs_simulator *sim, bit_num_t num_local_qubits);
+
+/* Current physical -> logical mapping. */
+int qcs_ext_get_mapping(qcs_simulator *sim, bit_num_t *perm_p2l, bit_num_t count);
+
+/* Move the data so that the physical layout becomes new_perm_p2l; the logical state is
+ * unchanged. */
+int qcs_ext_permute(qcs_simulator *sim, const bit_num_t *new_perm_p2l, bit_num_t count);
+
+/* Precision form.  mode 0: complex128, one kernel per gate (default).  mode 1: complex64
+ * state, gates queued and fused into blocks of at most max_block_qubits qubits applied as
+ * dense products (FP32 accumulation); FP64 only in reductions and renormalisation.  Must
+ * be called before allocate_memory. */
+int qcs_ext_set_precision(qcs_simulator *sim, int mode, int max_block_qubits);
+
+/* Apply all queued gates now (no-op in mode 0). */
+int qcs_ext_flush(qcs_simulator *sim);
+
+/* Per-kernel-class timing with HIP events around every launch (oracle: host clock). */
+int qcs_ext_profile(qcs_simulator *sim, int enable);
+int qcs_ext_profile_get(qcs_simulator *sim, int kernel_class, uint64_t *launches, double *seconds, double *bytes);
+int qcs_ext_profile_reset(qcs_simulator *sim);
+
+#ifdef __cplusplus
+}
+#endif

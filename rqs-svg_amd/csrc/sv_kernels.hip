@@ -1,0 +1,790 @@
+// State-vector kernels for gfx950 (MI355X), complex128 plain method.
+//
+// Layout: one contiguous array of 2^n amplitudes, interleaved (re, im) doubles, 16 B per
+// amplitude; bit q of the array index is physical qubit q.
+//
+// Every gate kernel walks the "work items" of its gate: item t is mapped to an amplitude
+// index by inserting a 0 bit at each sorted operand position and OR-ing the positive
+// control bits (the same mapping as src/qcs.cu:156-376 of the reference, done here with
+// kernel-argument operands instead of a __constant__ descriptor).  Consecutive lanes take
+// consecutive items, so every operand above bit 2 leaves the access fully coalesced.
+//
+// Streaming choices measured on MI355X (in-place 1-qubit update of a 16 GiB state):
+//  - 2 items per thread, 256-thread workgroups;
+//  - non-temporal 16-byte loads/stores (the state never fits in cache at the sizes that
+//    matter; 20-qubit states are launch-bound either way);
+//  - workgroup ids remapped so that each of the 8 XCDs streams one contiguous eighth of
+//    the tiles of a launch (consecutive workgroups are dispatched round-robin over XCDs).
+#include "sv_kernels.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace qsv
+{
+    typedef double v2d __attribute__((ext_vector_type(2)));
+
+    void check(hipError_t e, const char *what)
+    {
+        if (e != hipSuccess)
+            throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+    }
+
+    namespace
+    {
+        constexpr int kBlock = 256;
+        constexpr uint64_t kMaxGrid = uint64_t(1) << 22;
+
+        __device__ __forceinline__ v2d ld(const c128 *p) { return __builtin_nontemporal_load(reinterpret_cast<const v2d *>(p)); }
+        __device__ __forceinline__ void st(c128 *p, v2d v) { __builtin_nontemporal_store(v, reinterpret_cast<v2d *>(p)); }
+        __device__ __forceinline__ v2d ldc(const c128 *p) { return *reinterpret_cast<const v2d *>(p); }
+
+        __device__ __forceinline__ v2d cmul(c128 m, v2d a)
+        {
+            v2d r;
+            r.x = m.re * a.x - m.im * a.y;
+            r.y = m.re * a.y + m.im * a.x;
+            return r;
+        }
+        __device__ __forceinline__ v2d cfma(c128 m, v2d a, v2d acc)
+        {
+            acc.x += m.re * a.x - m.im * a.y;
+            acc.y += m.re * a.y + m.im * a.x;
+            return acc;
+        }
+
+        __device__ __forceinline__ uint64_t expand(uint64_t t, const Subspace &sub)
+        {
+            for (int k = 0; k < sub.nops; ++k)
+            {
+                uint64_t lo = t & ((uint64_t(1) << sub.ops[k]) - 1);
+                t = lo | ((t ^ lo) << 1);
+            }
+            return t | sub.set_bits;
+        }
+
+        // Workgroup id with the 8 XCDs each given a contiguous range.
+        __device__ __forceinline__ uint64_t xcd_block()
+        {
+            uint64_t nb = gridDim.x, b = blockIdx.x;
+            if ((nb & 7) == 0)
+                b = (b & 7) * (nb >> 3) + (b >> 3);
+            return b;
+        }
+
+        // p(i) = fma(re, re, im*im), the probability definition shared with the oracle.
+        __device__ __forceinline__ double prob(v2d a) { return __builtin_fma(a.x, a.x, a.y * a.y); }
+
+        template <int PER>
+        __global__ __launch_bounds__(kBlock) void k_pair(c128 *__restrict__ s, uint64_t nitems, Subspace sub, PairOp op)
+        {
+            const uint64_t per_tile = uint64_t(kBlock) * PER;
+            const uint64_t ntiles = (nitems + per_tile - 1) / per_tile;
+            for (uint64_t tile = xcd_block(); tile < ntiles; tile += gridDim.x)
+            {
+                uint64_t idx[PER];
+                v2d a[PER], b[PER];
+                bool ok[PER];
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                {
+                    uint64_t item = tile * per_tile + uint64_t(j) * kBlock + threadIdx.x;
+                    ok[j] = item < nitems;
+                    idx[j] = ok[j] ? expand(item, sub) : 0;
+                    a[j] = (ok[j] && !(op.zero_mask & 1)) ? ld(s + (idx[j] | op.offA)) : v2d{0.0, 0.0};
+                    b[j] = (ok[j] && !(op.zero_mask & 2)) ? ld(s + (idx[j] | op.offB)) : v2d{0.0, 0.0};
+                }
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                {
+                    if (!ok[j])
+                        continue;
+                    v2d oa = cfma(op.m[1], b[j], cmul(op.m[0], a[j]));
+                    v2d ob = cfma(op.m[3], b[j], cmul(op.m[2], a[j]));
+                    st(s + (idx[j] | op.offA), oa);
+                    st(s + (idx[j] | op.offB), ob);
+                }
+            }
+        }
+
+        template <int PER>
+        __global__ __launch_bounds__(kBlock) void k_phase(c128 *__restrict__ s, uint64_t nitems, Subspace sub, c128 phase)
+        {
+            const uint64_t per_tile = uint64_t(kBlock) * PER;
+            const uint64_t ntiles = (nitems + per_tile - 1) / per_tile;
+            for (uint64_t tile = xcd_block(); tile < ntiles; tile += gridDim.x)
+            {
+                uint64_t idx[PER];
+                v2d a[PER];
+                bool ok[PER];
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                {
+                    uint64_t item = tile * per_tile + uint64_t(j) * kBlock + threadIdx.x;
+                    ok[j] = item < nitems;
+                    idx[j] = ok[j] ? expand(item, sub) : 0;
+                    a[j] = ok[j] ? ld(s + idx[j]) : v2d{0.0, 0.0};
+                }
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                    if (ok[j])
+                        st(s + idx[j], cmul(phase, a[j]));
+            }
+        }
+
+        template <int PER>
+        __global__ __launch_bounds__(kBlock) void k_diag(c128 *__restrict__ s, uint64_t nitems, Subspace sub, DiagOp op)
+        {
+            const uint64_t per_tile = uint64_t(kBlock) * PER;
+            const uint64_t ntiles = (nitems + per_tile - 1) / per_tile;
+            for (uint64_t tile = xcd_block(); tile < ntiles; tile += gridDim.x)
+            {
+                uint64_t idx[PER];
+                v2d a[PER];
+                bool ok[PER];
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                {
+                    uint64_t item = tile * per_tile + uint64_t(j) * kBlock + threadIdx.x;
+                    ok[j] = item < nitems;
+                    idx[j] = ok[j] ? expand(item, sub) : 0;
+                    a[j] = ok[j] ? ld(s + idx[j]) : v2d{0.0, 0.0};
+                }
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                {
+                    if (!ok[j])
+                        continue;
+                    const bool b0 = (idx[j] >> op.t[0]) & 1;
+                    const bool b1 = op.ntargets == 2 && ((idx[j] >> op.t[1]) & 1);
+                    const c128 ph = b1 ? (b0 ? op.phase[3] : op.phase[2]) : (b0 ? op.phase[1] : op.phase[0]);
+                    st(s + idx[j], cmul(ph, a[j]));
+                }
+            }
+        }
+
+        __global__ __launch_bounds__(kBlock) void k_dense4(c128 *__restrict__ s, uint64_t nitems, Subspace sub, Dense4Op op)
+        {
+            const uint64_t ntiles = (nitems + kBlock - 1) / kBlock;
+            for (uint64_t tile = xcd_block(); tile < ntiles; tile += gridDim.x)
+            {
+                uint64_t item = tile * kBlock + threadIdx.x;
+                if (item >= nitems)
+                    continue;
+                uint64_t base = expand(item, sub);
+                v2d in[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    in[k] = ((op.read_mask >> k) & 1) ? ld(s + (base | op.off[k])) : v2d{0.0, 0.0};
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                {
+                    if (!((op.write_mask >> r) & 1))
+                        continue;
+                    v2d acc = cmul(op.m[4 * r], in[0]);
+                    acc = cfma(op.m[4 * r + 1], in[1], acc);
+                    acc = cfma(op.m[4 * r + 2], in[2], acc);
+                    acc = cfma(op.m[4 * r + 3], in[3], acc);
+                    st(s + (base | op.off[r]), acc);
+                }
+            }
+        }
+
+        __global__ __launch_bounds__(kBlock) void k_mono(c128 *__restrict__ s, uint64_t nitems, Subspace sub, MonoOp op)
+        {
+            const uint64_t ntiles = (nitems + kBlock - 1) / kBlock;
+            for (uint64_t tile = xcd_block(); tile < ntiles; tile += gridDim.x)
+            {
+                uint64_t item = tile * kBlock + threadIdx.x;
+                if (item >= nitems)
+                    continue;
+                uint64_t base = expand(item, sub);
+                v2d in[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                {
+                    in[k] = v2d{0.0, 0.0};
+                    if (k < op.nrows && ((op.read_mask >> op.src[k]) & 1))
+                        in[k] = ld(s + (base | op.off[op.src[k]]));
+                }
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                    if (k < op.nrows)
+                        st(s + (base | op.off[op.row[k]]), cmul(op.coef[k], in[k]));
+            }
+        }
+
+        // ---------------- state preparation ----------------
+        __global__ void k_fill_basis(c128 *s, uint64_t count, uint64_t index)
+        {
+            uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+                st(s + i, v2d{i == index ? 1.0 : 0.0, 0.0});
+        }
+        __global__ void k_fill_sequential(c128 *s, uint64_t count)
+        {
+            uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+                st(s + i, v2d{double(i), 0.0});
+        }
+        __global__ void k_fill_flat(c128 *s, uint64_t count)
+        {
+            uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+                st(s + i, v2d{1.0, 0.0});
+        }
+        __device__ __forceinline__ uint64_t splitmix64(uint64_t z)
+        {
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        }
+        __device__ __forceinline__ double uniform(uint64_t seed, uint64_t j)
+        {
+            return double(splitmix64(seed + (j + 1) * 0x9E3779B97F4A7C15ull) >> 11) * 0x1.0p-53;
+        }
+        __global__ void k_fill_random(c128 *s, uint64_t count, uint64_t seed)
+        {
+            uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+            {
+                double u1 = uniform(seed, 2 * i), u2 = uniform(seed, 2 * i + 1);
+                double rad = sqrt(-2.0 * log(1.0 - u1));
+                double ang = 2.0 * M_PI * u2;
+                st(s + i, v2d{rad * cos(ang), rad * sin(ang)});
+            }
+        }
+        __global__ void k_set_one(c128 *s, uint64_t index) { s[index] = c128{1.0, 0.0}; }
+
+        // ---------------- canonical reductions ----------------
+        // Tile = 4096 items, 256 threads x 16 items at stride 256.  The register tree, the
+        // two LDS steps and the wave shuffles together form the stride-halving tree over
+        // the tile (strides 2048 .. 1).
+        constexpr int kTile = 4096;
+
+        template <int NV>
+        __device__ __forceinline__ void tile_tree(double (&y)[NV][16], double *out_block, uint64_t out_stride)
+        {
+            __shared__ double buf[NV][kBlock];
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+            {
+#pragma unroll
+                for (int st_ = 8; st_ >= 1; st_ /= 2)
+#pragma unroll
+                    for (int j = 0; j < st_; ++j)
+                        y[v][j] = y[v][j] + y[v][j + st_];
+                buf[v][threadIdx.x] = y[v][0];
+            }
+            __syncthreads();
+            if (threadIdx.x < 128)
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    buf[v][threadIdx.x] = buf[v][threadIdx.x] + buf[v][threadIdx.x + 128];
+            __syncthreads();
+            if (threadIdx.x < 64)
+            {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                {
+                    double x = buf[v][threadIdx.x] + buf[v][threadIdx.x + 64];
+#pragma unroll
+                    for (int sh = 32; sh >= 1; sh /= 2)
+                        x = x + __shfl_down(x, sh, 64);
+                    if (threadIdx.x == 0)
+                        out_block[v * out_stride] = x;
+                }
+            }
+        }
+
+        __global__ __launch_bounds__(kBlock) void k_reduce_pair(const c128 *__restrict__ s, uint64_t nitems, Subspace sub, uint64_t tbit, double *__restrict__ out, uint64_t out_stride)
+        {
+            double y[2][16];
+            uint64_t base = uint64_t(blockIdx.x) * kTile + threadIdx.x;
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+            {
+                uint64_t item = base + uint64_t(j) * kBlock;
+                if (item < nitems)
+                {
+                    uint64_t i0 = expand(item, sub);
+                    y[0][j] = prob(ld(s + i0));
+                    y[1][j] = prob(ld(s + (i0 | tbit)));
+                }
+                else
+                {
+                    y[0][j] = 0.0;
+                    y[1][j] = 0.0;
+                }
+            }
+            tile_tree<2>(y, out + blockIdx.x, out_stride);
+        }
+
+        __global__ __launch_bounds__(kBlock) void k_reduce_probs(const c128 *__restrict__ s, uint64_t count, uint64_t zbit, double *__restrict__ out)
+        {
+            double y[1][16];
+            uint64_t base = uint64_t(blockIdx.x) * kTile + threadIdx.x;
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+            {
+                uint64_t i = base + uint64_t(j) * kBlock;
+                double p = i < count ? prob(ld(s + i)) : 0.0;
+                y[0][j] = (zbit && (i & zbit)) ? -p : p;
+            }
+            tile_tree<1>(y, out + blockIdx.x, 0);
+        }
+
+        // in: NV arrays of length m at stride in_stride; out: NV arrays at out_stride.
+        template <int NV>
+        __global__ __launch_bounds__(kBlock) void k_reduce_tiles(const double *__restrict__ in, uint64_t m, uint64_t in_stride, double *__restrict__ out, uint64_t out_stride)
+        {
+            double y[NV][16];
+            uint64_t base = uint64_t(blockIdx.x) * kTile + threadIdx.x;
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                {
+                    uint64_t i = base + uint64_t(j) * kBlock;
+                    y[v][j] = i < m ? in[v * in_stride + i] : 0.0;
+                }
+            tile_tree<NV>(y, out + blockIdx.x, out_stride);
+        }
+
+        // ---------------- collapse ----------------
+        __global__ __launch_bounds__(kBlock) void k_collapse(c128 *__restrict__ s, uint64_t half, int tq, int result, double scale, int rescale)
+        {
+            uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+            const uint64_t bit = uint64_t(1) << tq;
+            for (; i < half; i += uint64_t(gridDim.x) * kBlock)
+            {
+                uint64_t lo = i & (bit - 1);
+                uint64_t i0 = lo | ((i ^ lo) << 1);
+                uint64_t kill = result ? i0 : (i0 | bit);
+                uint64_t keep = result ? (i0 | bit) : i0;
+                st(s + kill, v2d{0.0, 0.0});
+                if (rescale)
+                {
+                    v2d a = ld(s + keep);
+                    st(s + keep, v2d{a.x * scale, a.y * scale});
+                }
+            }
+        }
+
+        // ---------------- sampling ----------------
+        // One wave folds 64 consecutive blocks of 64 entries (left to right).
+        template <bool FROM_STATE>
+        __global__ __launch_bounds__(64) void k_fold64(const void *__restrict__ src, uint64_t count, double *__restrict__ out, uint64_t nout)
+        {
+            __shared__ double tile[64][65];
+            const uint64_t group = blockIdx.x;
+            const int lane = threadIdx.x;
+#pragma unroll 8
+            for (int j = 0; j < 64; ++j)
+            {
+                uint64_t e = group * 4096 + uint64_t(j) * 64 + lane;
+                double v = 0.0;
+                if (e < count)
+                {
+                    if (FROM_STATE)
+                        v = prob(ld(static_cast<const c128 *>(src) + e));
+                    else
+                        v = static_cast<const double *>(src)[e];
+                }
+                tile[j][lane] = v;
+            }
+            __syncthreads();
+            uint64_t ob = group * 64 + lane;
+            if (ob < nout)
+            {
+                double acc = tile[lane][0];
+                for (int e = 1; e < 64; ++e)
+                    acc = acc + tile[lane][e];
+                out[ob] = acc;
+            }
+        }
+
+        struct Levels
+        {
+            int nlev;               // number of levels including level 0 (the state)
+            uint64_t size[12];
+            const double *ptr[12];  // ptr[0] unused (state)
+        };
+
+        struct P2L
+        {
+            int n;
+            int p2l[kMaxQubits];
+        };
+
+        __device__ __forceinline__ double level_value(const c128 *s, const Levels &L, int lev, uint64_t j)
+        {
+            return lev == 0 ? prob(ldc(s + j)) : L.ptr[lev][j];
+        }
+
+        __global__ void k_sample_search(const c128 *__restrict__ s, Levels L, double total, uint64_t seed, uint64_t nshots, P2L map, uint64_t *__restrict__ out)
+        {
+            uint64_t shot = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (shot >= nshots)
+                return;
+            const double r = uniform(seed, shot) * total;
+            double offs[12];
+            int depth = 0;
+            uint64_t block = 0;
+            for (int lev = L.nlev - 2; lev >= 0; --lev)
+            {
+                uint64_t b0 = block * 64, b1 = L.size[lev] < b0 + 64 ? L.size[lev] : b0 + 64;
+                double E = 0.0;
+                uint64_t pick = b1 - 1;
+                for (uint64_t j = b0; j < b1; ++j)
+                {
+                    double x = level_value(s, L, lev, j);
+                    double incl = (j == b0) ? x : E + x;
+                    double v = incl;
+                    for (int d = depth - 1; d >= 0; --d)
+                        v = offs[d] + v;
+                    if (r <= v)
+                    {
+                        pick = j;
+                        break;
+                    }
+                    E = incl;
+                }
+                double Ep = 0.0;
+                for (uint64_t j = b0; j < pick; ++j)
+                {
+                    double x = level_value(s, L, lev, j);
+                    Ep = (j == b0) ? x : Ep + x;
+                }
+                offs[depth++] = Ep;
+                block = pick;
+            }
+            uint64_t logical = 0;
+            for (int q = 0; q < map.n; ++q)
+                logical |= ((block >> q) & 1) << map.p2l[q];
+            out[shot] = logical;
+        }
+
+        // ---------------- layout ----------------
+        struct L2P
+        {
+            int n;
+            int l2p[kMaxQubits];
+        };
+        __device__ __forceinline__ uint64_t to_physical(uint64_t l, const L2P &m)
+        {
+            uint64_t p = 0;
+            for (int q = 0; q < m.n; ++q)
+                p |= ((l >> q) & 1) << m.l2p[q];
+            return p;
+        }
+        __global__ void k_gather_logical(const c128 *__restrict__ s, c128 *__restrict__ out, uint64_t first, uint64_t len, L2P m)
+        {
+            uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            for (; j < len; j += uint64_t(gridDim.x) * blockDim.x)
+                out[j] = s[to_physical(first + j, m)];
+        }
+        __global__ void k_scatter_logical(c128 *__restrict__ s, const c128 *__restrict__ in, uint64_t first, uint64_t len, L2P m)
+        {
+            uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            for (; j < len; j += uint64_t(gridDim.x) * blockDim.x)
+                s[to_physical(first + j, m)] = in[j];
+        }
+
+        __global__ void k_cast_down(const c128 *__restrict__ in, float2 *__restrict__ out, uint64_t count)
+        {
+            uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+            {
+                v2d a = ld(in + i);
+                out[i] = make_float2(float(a.x), float(a.y));
+            }
+        }
+        __global__ void k_cast_up(const float2 *__restrict__ in, c128 *__restrict__ out, uint64_t count)
+        {
+            uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+            {
+                float2 a = in[i];
+                st(out + i, v2d{double(a.x), double(a.y)});
+            }
+        }
+
+        uint64_t grid_for(uint64_t items, uint64_t per_block)
+        {
+            uint64_t g = (items + per_block - 1) / per_block;
+            g = std::max<uint64_t>(1, std::min(g, kMaxGrid));
+            return g;
+        }
+
+        void after_launch(const char *what) { check(hipGetLastError(), what); }
+
+        uint64_t items_of(int n, const Subspace &sub, const char *what)
+        {
+            if (sub.nops < 0 || sub.nops > n || n > kMaxQubits)
+                throw std::runtime_error(std::string(what) + ": operand count out of range");
+            for (int k = 0; k < sub.nops; ++k)
+                if (sub.ops[k] < 0 || sub.ops[k] >= n || (k > 0 && sub.ops[k] <= sub.ops[k - 1]))
+                    throw std::runtime_error(std::string(what) + ": operands must be ascending and < n");
+            if (n < 64 && (sub.set_bits >> n) != 0)
+                throw std::runtime_error(std::string(what) + ": control bit outside the state");
+            return uint64_t(1) << (n - sub.nops);
+        }
+
+        void check_offset(int n, uint64_t off, const char *what)
+        {
+            if (n < 64 && (off >> n) != 0)
+                throw std::runtime_error(std::string(what) + ": target offset outside the state");
+        }
+    } // namespace
+
+    void launch_pair(hipStream_t s, c128 *state, int n, const Subspace &sub, const PairOp &op)
+    {
+        uint64_t items = items_of(n, sub, "pair");
+        check_offset(n, op.offA | op.offB, "pair");
+        if (items >= (uint64_t(1) << 16))
+        {
+            hipLaunchKernelGGL(k_pair<2>, dim3(grid_for(items, kBlock * 2)), dim3(kBlock), 0, s, state, items, sub, op);
+        }
+        else
+        {
+            hipLaunchKernelGGL(k_pair<1>, dim3(grid_for(items, kBlock)), dim3(kBlock), 0, s, state, items, sub, op);
+        }
+        after_launch("pair kernel");
+    }
+
+    void launch_phase(hipStream_t s, c128 *state, int n, const Subspace &sub, c128 phase)
+    {
+        uint64_t items = items_of(n, sub, "phase");
+        if (items >= (uint64_t(1) << 16))
+            hipLaunchKernelGGL(k_phase<2>, dim3(grid_for(items, kBlock * 2)), dim3(kBlock), 0, s, state, items, sub, phase);
+        else
+            hipLaunchKernelGGL(k_phase<1>, dim3(grid_for(items, kBlock)), dim3(kBlock), 0, s, state, items, sub, phase);
+        after_launch("phase kernel");
+    }
+
+    void launch_diag(hipStream_t s, c128 *state, int n, const Subspace &sub, const DiagOp &op)
+    {
+        uint64_t items = items_of(n, sub, "diag");
+        if (op.ntargets < 1 || op.ntargets > 2 || op.t[0] < 0 || op.t[0] >= n || (op.ntargets == 2 && (op.t[1] < 0 || op.t[1] >= n)))
+            throw std::runtime_error("diag: bad targets");
+        if (items >= (uint64_t(1) << 16))
+            hipLaunchKernelGGL(k_diag<2>, dim3(grid_for(items, kBlock * 2)), dim3(kBlock), 0, s, state, items, sub, op);
+        else
+            hipLaunchKernelGGL(k_diag<1>, dim3(grid_for(items, kBlock)), dim3(kBlock), 0, s, state, items, sub, op);
+        after_launch("diag kernel");
+    }
+
+    void launch_dense4(hipStream_t s, c128 *state, int n, const Subspace &sub, const Dense4Op &op)
+    {
+        uint64_t items = items_of(n, sub, "dense4");
+        for (int k = 0; k < 4; ++k)
+            check_offset(n, op.off[k], "dense4");
+        hipLaunchKernelGGL(k_dense4, dim3(grid_for(items, kBlock)), dim3(kBlock), 0, s, state, items, sub, op);
+        after_launch("dense4 kernel");
+    }
+
+    void launch_mono(hipStream_t s, c128 *state, int n, const Subspace &sub, const MonoOp &op)
+    {
+        uint64_t items = items_of(n, sub, "mono");
+        if (op.ntargets < 1 || op.ntargets > 4 || op.nrows < 1 || op.nrows > 16)
+            throw std::runtime_error("mono: bad shape");
+        for (int k = 0; k < (1 << op.ntargets); ++k)
+            check_offset(n, op.off[k], "mono");
+        for (int k = 0; k < op.nrows; ++k)
+            if (op.row[k] < 0 || op.row[k] >= (1 << op.ntargets) || op.src[k] < 0 || op.src[k] >= (1 << op.ntargets))
+                throw std::runtime_error("mono: row index out of range");
+        hipLaunchKernelGGL(k_mono, dim3(grid_for(items, kBlock)), dim3(kBlock), 0, s, state, items, sub, op);
+        after_launch("mono kernel");
+    }
+
+    void launch_fill_basis(hipStream_t s, c128 *state, uint64_t count, uint64_t index)
+    {
+        hipLaunchKernelGGL(k_fill_basis, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count, index);
+        after_launch("fill kernel");
+    }
+    void launch_fill_sequential(hipStream_t s, c128 *state, uint64_t count)
+    {
+        hipLaunchKernelGGL(k_fill_sequential, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count);
+        after_launch("fill kernel");
+    }
+    void launch_fill_flat(hipStream_t s, c128 *state, uint64_t count)
+    {
+        hipLaunchKernelGGL(k_fill_flat, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count);
+        after_launch("fill kernel");
+    }
+    void launch_fill_random(hipStream_t s, c128 *state, uint64_t count, uint64_t seed)
+    {
+        hipLaunchKernelGGL(k_fill_random, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count, seed);
+        after_launch("fill kernel");
+    }
+    void launch_set_one(hipStream_t s, c128 *state, uint64_t index)
+    {
+        hipLaunchKernelGGL(k_set_one, dim3(1), dim3(1), 0, s, state, index);
+        after_launch("set kernel");
+    }
+
+    uint64_t reduce_workspace(uint64_t nitems)
+    {
+        uint64_t total = 0, m = nitems;
+        while (m > 1)
+        {
+            m = (m + kTile - 1) / kTile;
+            total += m;
+        }
+        return 2 * (total + 2);
+    }
+
+    namespace
+    {
+        // Reduce NV arrays (at stride `stride` in `work`, length m each) down to one value
+        // each, written to work_out[v].
+        template <int NV>
+        void finish_tiles(hipStream_t s, double *cur, uint64_t m, uint64_t cur_stride, double *next, double *dev_out)
+        {
+            while (m > 1)
+            {
+                uint64_t g = (m + kTile - 1) / kTile;
+                uint64_t out_stride = g;
+                double *dst = (g == 1) ? dev_out : next;
+                if (g == 1)
+                    out_stride = 1;
+                hipLaunchKernelGGL(k_reduce_tiles<NV>, dim3(g), dim3(kBlock), 0, s, cur, m, cur_stride, dst, out_stride);
+                after_launch("reduce kernel");
+                cur = dst;
+                cur_stride = out_stride;
+                next = dst + NV * g;
+                m = g;
+            }
+        }
+    } // namespace
+
+    void reduce_pair_probs(hipStream_t s, const c128 *state, const Subspace &sub, uint64_t tbit, uint64_t nitems, double *work, double out[2])
+    {
+        // work layout: [result(2)] [level arrays ...]
+        double *dev_out = work;
+        uint64_t g = (nitems + kTile - 1) / kTile;
+        double *lvl = work + 2;
+        if (g == 1)
+        {
+            hipLaunchKernelGGL(k_reduce_pair, dim3(1), dim3(kBlock), 0, s, state, nitems, sub, tbit, dev_out, uint64_t(1));
+            after_launch("reduce kernel");
+        }
+        else
+        {
+            hipLaunchKernelGGL(k_reduce_pair, dim3(g), dim3(kBlock), 0, s, state, nitems, sub, tbit, lvl, g);
+            after_launch("reduce kernel");
+            finish_tiles<2>(s, lvl, g, g, lvl + 2 * g, dev_out);
+        }
+        check(hipMemcpyAsync(out, dev_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s), "reduce copy");
+        check(hipStreamSynchronize(s), "reduce sync");
+    }
+
+    void reduce_probs(hipStream_t s, const c128 *state, uint64_t count, uint64_t zbit, double *work, double *out)
+    {
+        double *dev_out = work;
+        uint64_t g = (count + kTile - 1) / kTile;
+        double *lvl = work + 2;
+        if (g == 1)
+        {
+            hipLaunchKernelGGL(k_reduce_probs, dim3(1), dim3(kBlock), 0, s, state, count, zbit, dev_out);
+            after_launch("reduce kernel");
+        }
+        else
+        {
+            hipLaunchKernelGGL(k_reduce_probs, dim3(g), dim3(kBlock), 0, s, state, count, zbit, lvl);
+            after_launch("reduce kernel");
+            finish_tiles<1>(s, lvl, g, g, lvl + g, dev_out);
+        }
+        check(hipMemcpyAsync(out, dev_out, sizeof(double), hipMemcpyDeviceToHost, s), "reduce copy");
+        check(hipStreamSynchronize(s), "reduce sync");
+    }
+
+    void launch_collapse(hipStream_t s, c128 *state, int n, int tq, int result, double scale)
+    {
+        if (tq < 0 || tq >= n || n < 1)
+            throw std::runtime_error("collapse: qubit out of range");
+        uint64_t half = uint64_t(1) << (n - 1);
+        hipLaunchKernelGGL(k_collapse, dim3(grid_for(half, kBlock)), dim3(kBlock), 0, s, state, half, tq, result, scale, scale != 1.0 ? 1 : 0);
+        after_launch("collapse kernel");
+    }
+
+    std::vector<uint64_t> fold_level_sizes(uint64_t count)
+    {
+        std::vector<uint64_t> sz{count};
+        while (sz.back() > 1)
+            sz.push_back((sz.back() + 63) / 64);
+        return sz;
+    }
+
+    void build_fold_levels(hipStream_t s, const c128 *state, uint64_t count, const std::vector<double *> &levels)
+    {
+        auto sz = fold_level_sizes(count);
+        if (levels.size() + 1 != sz.size())
+            throw std::runtime_error("fold: level buffer count mismatch");
+        for (size_t lev = 1; lev < sz.size(); ++lev)
+        {
+            uint64_t nout = sz[lev];
+            uint64_t groups = (nout + 63) / 64;
+            if (lev == 1)
+                hipLaunchKernelGGL(k_fold64<true>, dim3(groups), dim3(64), 0, s, static_cast<const void *>(state), sz[0], levels[0], nout);
+            else
+                hipLaunchKernelGGL(k_fold64<false>, dim3(groups), dim3(64), 0, s, static_cast<const void *>(levels[lev - 2]), sz[lev - 1], levels[lev - 1], nout);
+            after_launch("fold kernel");
+        }
+    }
+
+    void launch_sample_search(hipStream_t s, const c128 *state, uint64_t count, const std::vector<double *> &levels, double total, uint64_t seed, uint64_t nshots, const int *p2l, int n, uint64_t *out_dev)
+    {
+        auto sz = fold_level_sizes(count);
+        if (sz.size() > 12 || levels.size() + 1 != sz.size() || n > kMaxQubits)
+            throw std::runtime_error("sample: bad level layout");
+        Levels L{};
+        L.nlev = (int)sz.size();
+        for (size_t i = 0; i < sz.size(); ++i)
+        {
+            L.size[i] = sz[i];
+            L.ptr[i] = i == 0 ? nullptr : levels[i - 1];
+        }
+        P2L m{};
+        m.n = n;
+        for (int q = 0; q < n; ++q)
+            m.p2l[q] = p2l[q];
+        if (nshots == 0)
+            return;
+        hipLaunchKernelGGL(k_sample_search, dim3((nshots + 63) / 64), dim3(64), 0, s, state, L, total, seed, nshots, m, out_dev);
+        after_launch("sample kernel");
+    }
+
+    void launch_gather_logical(hipStream_t s, const c128 *state, c128 *out, uint64_t first, uint64_t len, const int *l2p, int n)
+    {
+        L2P m{};
+        m.n = n;
+        for (int q = 0; q < n; ++q)
+            m.l2p[q] = l2p[q];
+        hipLaunchKernelGGL(k_gather_logical, dim3(grid_for(len, kBlock)), dim3(kBlock), 0, s, state, out, first, len, m);
+        after_launch("gather kernel");
+    }
+
+    void launch_scatter_logical(hipStream_t s, c128 *state, const c128 *in, uint64_t first, uint64_t len, const int *l2p, int n)
+    {
+        L2P m{};
+        m.n = n;
+        for (int q = 0; q < n; ++q)
+            m.l2p[q] = l2p[q];
+        hipLaunchKernelGGL(k_scatter_logical, dim3(grid_for(len, kBlock)), dim3(kBlock), 0, s, state, in, first, len, m);
+        after_launch("scatter kernel");
+    }
+
+    void launch_cast_c128_to_c64(hipStream_t s, const c128 *in, float2 *out, uint64_t count)
+    {
+        hipLaunchKernelGGL(k_cast_down, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, in, out, count);
+        after_launch("cast kernel");
+    }
+    void launch_cast_c64_to_c128(hipStream_t s, const float2 *in, c128 *out, uint64_t count)
+    {
+        hipLaunchKernelGGL(k_cast_up, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, in, out, count);
+        after_launch("cast kernel");
+    }
+} // namespace qsv

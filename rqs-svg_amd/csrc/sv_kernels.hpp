@@ -1,0 +1,125 @@
+// Launch interface of the state-vector kernels (sv_kernels.hip).
+//
+// All functions enqueue work on the given stream and return immediately; hipError_t
+// results are turned into std::runtime_error by the caller-side check in the launchers.
+// Shapes are validated by the host core (simulator.cpp) before these are called; the
+// launchers re-check what the kernels' indexing relies on.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace qsv
+{
+    constexpr int kMaxQubits = 40;
+
+    // Sorted operand positions (controls and targets) of one gate and the bits that
+    // positive controls force to 1.  A work item t maps to the amplitude index obtained by
+    // inserting a 0 at each operand position of t (ascending) and OR-ing set_bits.
+    struct Subspace
+    {
+        uint64_t set_bits;
+        int nops;
+        int ops[kMaxQubits];
+    };
+
+    struct c128
+    {
+        double re, im;
+    };
+
+    // 2x2 on the amplitude pair (base|offA, base|offB): dense 1-qubit gates (offA = 0,
+    // offB = 1<<t) and the swap-like 2-qubit gates.  zero_mask bit 0/1: input A/B reads as
+    // zero (branch of a measured target that was not observed).
+    struct PairOp
+    {
+        uint64_t offA, offB;
+        c128 m[4]; // row-major m00 m01 m10 m11
+        unsigned zero_mask;
+    };
+
+    // Diagonal on up to 2 targets over the control subspace: amplitude i gets
+    // phase[bit(i,t0) | bit(i,t1)<<1].
+    struct DiagOp
+    {
+        int ntargets;
+        int t[2];
+        c128 phase[4];
+    };
+
+    // Dense 4x4 on targets (t0,t1); rows with write_mask bit clear are left untouched.
+    struct Dense4Op
+    {
+        uint64_t off[4];
+        c128 m[16];
+        unsigned write_mask;
+        unsigned read_mask; // inputs that may be non-zero (measured-target rule)
+    };
+
+    // Up to 16 rows out[row[k]] = coef[k] * in[src[k]] on T <= 4 targets.
+    struct MonoOp
+    {
+        int ntargets;
+        int nrows;
+        uint64_t off[16];
+        int row[16];
+        int src[16];
+        c128 coef[16];
+        unsigned read_mask;
+    };
+
+    enum class Launch
+    {
+        Pair,
+        Phase,
+        Diag,
+        Dense4,
+        Mono
+    };
+
+    // ---- gate kernels (complex128 state) ----
+    void launch_pair(hipStream_t s, c128 *state, int n, const Subspace &sub, const PairOp &op);
+    void launch_phase(hipStream_t s, c128 *state, int n, const Subspace &sub, c128 phase);
+    void launch_diag(hipStream_t s, c128 *state, int n, const Subspace &sub, const DiagOp &op);
+    void launch_dense4(hipStream_t s, c128 *state, int n, const Subspace &sub, const Dense4Op &op);
+    void launch_mono(hipStream_t s, c128 *state, int n, const Subspace &sub, const MonoOp &op);
+
+    // ---- state preparation ----
+    void launch_fill_basis(hipStream_t s, c128 *state, uint64_t count, uint64_t index); // |index>
+    void launch_fill_sequential(hipStream_t s, c128 *state, uint64_t count);
+    void launch_fill_flat(hipStream_t s, c128 *state, uint64_t count);
+    void launch_fill_random(hipStream_t s, c128 *state, uint64_t count, uint64_t seed);
+    void launch_set_one(hipStream_t s, c128 *state, uint64_t index);
+
+    // ---- reductions (FP64, canonical order of include/qcs_ext.h) ----
+    // Workspace size in doubles for reduce_* on `nitems` items.
+    uint64_t reduce_workspace(uint64_t nitems);
+    // {p0,p1} over the subspace `sub` (target bit `tq` not in the insertion list, see
+    // simulator.cpp); result written to host `out[2]` after a stream sync.
+    void reduce_pair_probs(hipStream_t s, const c128 *state, const Subspace &sub, uint64_t tbit, uint64_t nitems, double *work, double out[2]);
+    // canonical sum of p(i) (sign = -1 where bit `zbit` is set, if zbit != 0) over all amplitudes
+    void reduce_probs(hipStream_t s, const c128 *state, uint64_t count, uint64_t zbit, double *work, double *out);
+
+    // ---- measurement collapse ----
+    void launch_collapse(hipStream_t s, c128 *state, int n, int tq, int result, double scale);
+
+    // ---- sampling ----
+    // Level sizes of the 64-ary fold for 2^n entries; level 0 is the state itself.
+    std::vector<uint64_t> fold_level_sizes(uint64_t count);
+    // Build the level totals into `levels` (one device buffer per level >= 1).
+    void build_fold_levels(hipStream_t s, const c128 *state, uint64_t count, const std::vector<double *> &levels);
+    void launch_sample_search(hipStream_t s, const c128 *state, uint64_t count, const std::vector<double *> &levels, double total, uint64_t seed, uint64_t nshots, const int *p2l, int n, uint64_t *out_dev);
+
+    // ---- layout ----
+    // out[j] = state[phys(first + j)] for j < len, where phys maps a logical index to
+    // the physical one through l2p.
+    void launch_gather_logical(hipStream_t s, const c128 *state, c128 *out, uint64_t first, uint64_t len, const int *l2p, int n);
+    void launch_scatter_logical(hipStream_t s, c128 *state, const c128 *in, uint64_t first, uint64_t len, const int *l2p, int n);
+
+    // ---- precision conversion ----
+    void launch_cast_c128_to_c64(hipStream_t s, const c128 *in, float2 *out, uint64_t count);
+    void launch_cast_c64_to_c128(hipStream_t s, const float2 *in, c128 *out, uint64_t count);
+
+    void check(hipError_t e, const char *what);
+} // namespace qsv

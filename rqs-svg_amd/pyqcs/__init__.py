@@ -1,0 +1,344 @@
+"""ctypes binding of the qcs C ABI (include/qcs.h + include/qcs_ext.h).
+
+The same class drives either shared library that exports the ABI:
+
+* ``libqcs.so`` -- the product: host C++ plus HIP kernels for gfx950
+  (``rqs-svg_amd/lib/libqcs.so``);
+* ``libqcs_oracle.so`` -- the CPU restatement of the reference plain method, used only
+  by tests and by the benchmark's CPU-baseline leg.
+
+Gate calls follow the reference signature (src/qcs.cu:3383-3759, include/qcs.h:108-141):
+``gate(name, targets, params=(), ctrls=(), negctrls=())``.  Every C call returns 1/0;
+a 0 raises :class:`QcsError` carrying the message the library sent to its exception
+callback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+PKG = HERE.parent
+REPO = PKG.parent
+GPU_LIB = PKG / "lib" / "libqcs.so"
+ORACLE_LIB = REPO / "oracle" / "_build" / "libqcs_oracle.so"
+
+bit_num_t = ctypes.c_int16
+bit_t = ctypes.c_uint8
+
+# name -> number of double parameters (include/qcs.h:108-141)
+GATE_PARAMS = {
+    "h": 0, "x": 0, "y": 0, "z": 0, "s": 0, "sdg": 0, "t": 0, "tdg": 0, "sx": 0,
+    "sxdg": 0, "id": 0, "swap": 0, "iswap": 0, "dcx": 0, "ecr": 0, "rccx": 0, "rcccx": 0,
+    "rx": 1, "ry": 1, "rz": 1, "u1": 1, "p": 1, "rxx": 1, "ryy": 1, "rzz": 1, "rzx": 1,
+    "u2": 2, "r": 2, "xx_plus_yy": 2, "xx_minus_yy": 2,
+    "u3": 3, "u": 3, "u4": 4,
+}
+GATE_TARGETS = {
+    "h": 1, "x": 1, "y": 1, "z": 1, "s": 1, "sdg": 1, "t": 1, "tdg": 1, "sx": 1,
+    "sxdg": 1, "id": 1, "rx": 1, "ry": 1, "rz": 1, "u1": 1, "p": 1, "u2": 1, "r": 1,
+    "u3": 1, "u": 1, "u4": 1, "swap": 2, "iswap": 2, "dcx": 2, "ecr": 2, "rxx": 2,
+    "ryy": 2, "rzz": 2, "rzx": 2, "xx_plus_yy": 2, "xx_minus_yy": 2, "rccx": 3,
+    "rcccx": 4, "global_phase": 0,
+}
+
+KERNEL_CLASSES = ["pair", "phase", "diag", "dense4", "mono", "reduce", "collapse",
+                  "sample", "permute", "init", "fused", "cast"]
+
+
+class QcsError(RuntimeError):
+    pass
+
+
+_CB_TYPE = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_size_t)
+
+
+class Library:
+    """One loaded ABI library plus its exception-callback plumbing."""
+
+    _cache: dict = {}
+
+    def __new__(cls, path, mode=None):
+        key = (str(path), mode)
+        if key not in cls._cache:
+            obj = super().__new__(cls)
+            obj._init(path, mode)
+            cls._cache[key] = obj
+        return cls._cache[key]
+
+    def _init(self, path, mode):
+        path = pathlib.Path(path)
+        if not path.exists():
+            raise FileNotFoundError(f"{path} is not built; run __graft_entry__.build()")
+        self.path = path
+        self.lib = ctypes.CDLL(str(path), mode=mode if mode is not None else ctypes.RTLD_LOCAL)
+        self.last_error = ""
+
+        def _cb(msg, length):
+            self.last_error = ctypes.string_at(msg, length).decode(errors="replace")
+
+        self._cb = _CB_TYPE(_cb)
+        self.lib.qcs_set_exception_callback(self._cb)
+        P = ctypes.POINTER
+        vp = ctypes.c_void_p
+        lib = self.lib
+        lib.qcs_simulator_create.argtypes = [P(vp)]
+        for name in ["destroy", "allocate_memory", "dispose", "set_zero_state",
+                     "set_sequential_state", "set_flat_state", "set_entangled_state",
+                     "set_random_state", "reset_clbits", "reset_measurement_state",
+                     "reinitialize_mapping"]:
+            getattr(lib, "qcs_simulator_" + name).argtypes = [vp]
+        for name in ["get_num_procs", "get_proc_num", "get_num_qubits", "get_num_clbits"]:
+            getattr(lib, "qcs_simulator_" + name).argtypes = [vp, P(bit_t)]
+        lib.qcs_simulator_set_num_qubits.argtypes = [vp, bit_num_t]
+        lib.qcs_simulator_set_num_clbits.argtypes = [vp, bit_num_t]
+        lib.qcs_simulator_set_mapping.argtypes = [vp, P(bit_num_t), bit_num_t]
+        lib.qcs_simulator_get_clbits.argtypes = [vp, P(bit_t)]
+        lib.qcs_simulator_get_clbits_string.argtypes = [vp, ctypes.c_char_p]
+        lib.qcs_simulator_reset.argtypes = [vp, bit_num_t]
+        lib.qcs_simulator_measure.argtypes = [vp, bit_num_t, P(bit_t)]
+        lib.qcs_simulator_measure_to_clbit.argtypes = [vp, bit_num_t, bit_num_t, P(bit_t)]
+        lib.qcs_simulator_read.argtypes = [vp, bit_num_t, P(bit_t)]
+        lib.qcs_simulator_save_statevector.argtypes = [vp, ctypes.c_char_p]
+        lib.qcs_simulator_event_create.argtypes = [vp, P(bit_t)]
+        lib.qcs_simulator_event_record.argtypes = [vp, ctypes.c_int]
+        lib.qcs_simulator_event_get_elapsed_time.argtypes = [vp, ctypes.c_int16, ctypes.c_int16, P(ctypes.c_double)]
+        lists = [P(bit_num_t), bit_num_t] * 3
+        for g, npar in GATE_PARAMS.items():
+            getattr(lib, "qcs_simulator_gate_" + g).argtypes = [vp] + [ctypes.c_double] * npar + lists
+        lib.qcs_simulator_gate_global_phase.argtypes = [vp, ctypes.c_double] + [P(bit_num_t), bit_num_t] * 2
+        u64 = ctypes.c_uint64
+        lib.qcs_ext_set_seed.argtypes = [vp, u64]
+        lib.qcs_ext_get_statevector.argtypes = [vp, vp, u64]
+        lib.qcs_ext_set_statevector.argtypes = [vp, vp, u64]
+        lib.qcs_ext_sample.argtypes = [vp, u64, u64, vp]
+        lib.qcs_ext_probability_total.argtypes = [vp, P(ctypes.c_double)]
+        lib.qcs_ext_norm.argtypes = [vp, P(ctypes.c_double)]
+        lib.qcs_ext_expect_z.argtypes = [vp, vp, ctypes.c_int]
+        lib.qcs_ext_set_renormalize.argtypes = [vp, ctypes.c_int]
+        lib.qcs_ext_set_local_qubits.argtypes = [vp, bit_num_t]
+        lib.qcs_ext_get_mapping.argtypes = [vp, P(bit_num_t), bit_num_t]
+        lib.qcs_ext_permute.argtypes = [vp, P(bit_num_t), bit_num_t]
+        lib.qcs_ext_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        lib.qcs_ext_flush.argtypes = [vp]
+        lib.qcs_ext_profile.argtypes = [vp, ctypes.c_int]
+        lib.qcs_ext_profile_get.argtypes = [vp, ctypes.c_int, P(u64), P(ctypes.c_double), P(ctypes.c_double)]
+        lib.qcs_ext_profile_reset.argtypes = [vp]
+
+    def check(self, rc, what):
+        if rc != 1:
+            msg, self.last_error = self.last_error, ""
+            raise QcsError(f"{what} failed: {msg}")
+
+
+def gpu_library() -> Library:
+    return Library(GPU_LIB)
+
+
+def oracle_library() -> Library:
+    return Library(ORACLE_LIB)
+
+
+def _arr(values):
+    values = [int(v) for v in values]
+    if not values:
+        return None, 0
+    return (bit_num_t * len(values))(*values), len(values)
+
+
+class Simulator:
+    """Owns one qcs_simulator handle (create ... destroy)."""
+
+    def __init__(self, library: Library, num_qubits: int, num_clbits: int = 0,
+                 mapping=None, seed: int | None = None, precision: int = 0,
+                 block_qubits: int = 5, allocate: bool = True):
+        self.L = library
+        self.lib = library.lib
+        h = ctypes.c_void_p()
+        self._call(self.lib.qcs_simulator_create(ctypes.byref(h)), "create")
+        self.h = h
+        self.n = num_qubits
+        self._call(self.lib.qcs_simulator_set_num_qubits(self.h, num_qubits), "set_num_qubits")
+        self._call(self.lib.qcs_simulator_set_num_clbits(self.h, num_clbits), "set_num_clbits")
+        if mapping is not None:
+            a, c = _arr(mapping)
+            self._call(self.lib.qcs_simulator_set_mapping(self.h, a, c), "set_mapping")
+        if seed is not None:
+            self._call(self.lib.qcs_ext_set_seed(self.h, seed), "set_seed")
+        if precision:
+            self._call(self.lib.qcs_ext_set_precision(self.h, precision, block_qubits), "set_precision")
+        if allocate:
+            self.allocate()
+
+    def _call(self, rc, what):
+        self.L.check(rc, what)
+
+    def allocate(self):
+        self._call(self.lib.qcs_simulator_allocate_memory(self.h), "allocate_memory")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value is not None:
+            self.lib.qcs_simulator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- gates -------------------------------------------------------------------
+    def gate(self, name, targets=(), params=(), ctrls=(), negctrls=()):
+        n_ng, c_ng = _arr(negctrls)
+        n_ps, c_ps = _arr(ctrls)
+        if name == "global_phase":
+            rc = self.lib.qcs_simulator_gate_global_phase(self.h, float(params[0]), n_ng, c_ng, n_ps, c_ps)
+        else:
+            t, ct = _arr(targets)
+            fn = getattr(self.lib, "qcs_simulator_gate_" + name)
+            rc = fn(self.h, *[float(p) for p in params], t, ct, n_ng, c_ng, n_ps, c_ps)
+        self._call(rc, "gate_" + name)
+
+    def run(self, circuit):
+        """Apply a list of (name, targets, params, ctrls, negctrls) tuples."""
+        for op in circuit:
+            self.gate(*op)
+
+    # --- measurement / state ----------------------------------------------------------
+    def measure(self, q):
+        r = bit_t()
+        self._call(self.lib.qcs_simulator_measure(self.h, q, ctypes.byref(r)), "measure")
+        return r.value
+
+    def measure_to_clbit(self, q, c):
+        r = bit_t()
+        self._call(self.lib.qcs_simulator_measure_to_clbit(self.h, q, c, ctypes.byref(r)), "measure_to_clbit")
+        return r.value
+
+    def read(self, c):
+        r = bit_t()
+        self._call(self.lib.qcs_simulator_read(self.h, c, ctypes.byref(r)), "read")
+        return r.value
+
+    def reset(self, q):
+        self._call(self.lib.qcs_simulator_reset(self.h, q), "reset")
+
+    def clbits_string(self):
+        nc = self.get("num_clbits")
+        buf = ctypes.create_string_buffer(nc + 1)
+        self._call(self.lib.qcs_simulator_get_clbits_string(self.h, buf), "get_clbits_string")
+        return buf.value.decode()
+
+    def clbits(self):
+        nc = self.get("num_clbits")
+        buf = (bit_t * max(nc, 1))()
+        self._call(self.lib.qcs_simulator_get_clbits(self.h, buf), "get_clbits")
+        return [buf[i] for i in range(nc)]
+
+    def get(self, what):
+        r = bit_t()
+        self._call(getattr(self.lib, "qcs_simulator_get_" + what)(self.h, ctypes.byref(r)), "get_" + what)
+        return r.value
+
+    def call(self, what):
+        self._call(getattr(self.lib, "qcs_simulator_" + what)(self.h), what)
+
+    def statevector(self):
+        out = np.empty(1 << self.n, dtype=np.complex128)
+        self._call(self.lib.qcs_ext_get_statevector(self.h, out.ctypes.data, out.size), "get_statevector")
+        return out
+
+    def set_statevector(self, psi):
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        self._call(self.lib.qcs_ext_set_statevector(self.h, psi.ctypes.data, psi.size), "set_statevector")
+
+    def save(self, path):
+        self._call(self.lib.qcs_simulator_save_statevector(self.h, str(path).encode()), "save_statevector")
+
+    def sample(self, shots, seed):
+        out = np.empty(shots, dtype=np.uint64)
+        self._call(self.lib.qcs_ext_sample(self.h, seed, shots, out.ctypes.data), "sample")
+        return out
+
+    def probability_total(self):
+        r = ctypes.c_double()
+        self._call(self.lib.qcs_ext_probability_total(self.h, ctypes.byref(r)), "probability_total")
+        return r.value
+
+    def norm(self):
+        r = ctypes.c_double()
+        self._call(self.lib.qcs_ext_norm(self.h, ctypes.byref(r)), "norm")
+        return r.value
+
+    def expect_z(self):
+        out = np.empty(self.n, dtype=np.float64)
+        self._call(self.lib.qcs_ext_expect_z(self.h, out.ctypes.data, self.n), "expect_z")
+        return out
+
+    def set_seed(self, seed):
+        self._call(self.lib.qcs_ext_set_seed(self.h, seed), "set_seed")
+
+    def set_renormalize(self, on):
+        self._call(self.lib.qcs_ext_set_renormalize(self.h, int(bool(on))), "set_renormalize")
+
+    def set_local_qubits(self, nl):
+        self._call(self.lib.qcs_ext_set_local_qubits(self.h, nl), "set_local_qubits")
+
+    def mapping(self):
+        buf = (bit_num_t * self.n)()
+        self._call(self.lib.qcs_ext_get_mapping(self.h, buf, self.n), "get_mapping")
+        return [buf[i] for i in range(self.n)]
+
+    def permute(self, p2l):
+        a, c = _arr(p2l)
+        self._call(self.lib.qcs_ext_permute(self.h, a, c), "permute")
+
+    def flush(self):
+        self._call(self.lib.qcs_ext_flush(self.h), "flush")
+
+    # --- timing -----------------------------------------------------------------------
+    def event_create(self):
+        r = bit_t()
+        self._call(self.lib.qcs_simulator_event_create(self.h, ctypes.byref(r)), "event_create")
+        return r.value
+
+    def event_record(self, e):
+        self._call(self.lib.qcs_simulator_event_record(self.h, e), "event_record")
+
+    def elapsed(self, a, b):
+        r = ctypes.c_double()
+        self._call(self.lib.qcs_simulator_event_get_elapsed_time(self.h, a, b, ctypes.byref(r)), "elapsed")
+        return r.value
+
+    def profile(self, on=True):
+        self._call(self.lib.qcs_ext_profile(self.h, int(bool(on))), "profile")
+
+    def profile_reset(self):
+        self._call(self.lib.qcs_ext_profile_reset(self.h), "profile_reset")
+
+    def profile_get(self):
+        res = {}
+        for i, name in enumerate(KERNEL_CLASSES):
+            n = ctypes.c_uint64()
+            s = ctypes.c_double()
+            b = ctypes.c_double()
+            self._call(self.lib.qcs_ext_profile_get(self.h, i, ctypes.byref(n), ctypes.byref(s), ctypes.byref(b)), "profile_get")
+            if n.value:
+                res[name] = {"launches": n.value, "seconds": s.value, "bytes": b.value}
+        return res
+
+
+def load_statevector_file(path, n):
+    data = np.fromfile(path, dtype=np.complex128)
+    if data.size != (1 << n):
+        raise ValueError(f"{path}: {data.size} amplitudes, expected {1 << n}")
+    return data

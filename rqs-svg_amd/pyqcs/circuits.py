@@ -1,0 +1,171 @@
+"""Seeded circuit generators for the benchmark workloads.
+
+A circuit is a list of ``(name, targets, params, ctrls, negctrls)`` tuples in the argument
+order of the C API.  All randomness comes from SplitMix64 (Steele, Lea & Flood, OOPSLA
+2014) so that the same seed gives the same circuit in every process and language:
+``u = (splitmix64(state += 0x9E3779B97F4A7C15) >> 11) * 2**-53``.
+
+Workloads:
+
+* ``ghz_u3_cz`` -- H(q0), CNOT q0->qi for i = 1..n-1 (the GHZ construction of the
+  reference example examples/standalone/ghz_circuit.c:36-41), then ``layers`` layers of
+  U3(theta, phi, lambda ~ U[0, 2pi)) on every qubit followed by CZ on a random perfect
+  matching (Fisher-Yates shuffle, consecutive pairs).
+* ``brick`` -- ``depth`` layers of U3 on every qubit, then CZ on the even pairs
+  (0,1),(2,3),... in even layers and the odd pairs (1,2),(3,4),... in odd layers; plus
+  ``mcx`` multi-controlled X gates with 3 controls inserted after random layers, each with
+  at least one operand among the top 6 qubits (high-stride qubits).
+* ``variational`` -- ``depth`` layers of RY and RZ (U[-pi, pi]) on every qubit plus a
+  CNOT ladder q_i -> q_{i+1}.
+* ``qft`` -- H plus controlled-phase ladder plus the final swaps.
+"""
+from __future__ import annotations
+
+import math
+
+MASK = (1 << 64) - 1
+GOLDEN = 0x9E3779B97F4A7C15
+
+
+def splitmix64(z: int) -> int:
+    z &= MASK
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+    return z ^ (z >> 31)
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + GOLDEN) & MASK
+        return splitmix64(self.state)
+
+    def uniform(self) -> float:
+        return (self.next_u64() >> 11) * (2.0 ** -53)
+
+    def randbelow(self, k: int) -> int:
+        return min(int(self.uniform() * k), k - 1)
+
+
+def uniform_stream(seed: int, count: int):
+    """The sampling uniforms of qcs_ext_sample: u_j = splitmix64(seed + (j+1)*gamma)."""
+    return [(splitmix64(seed + (j + 1) * GOLDEN) >> 11) * (2.0 ** -53) for j in range(count)]
+
+
+def _u3_layer(rng, n):
+    out = []
+    for q in range(n):
+        th, ph, la = (2 * math.pi * rng.uniform() for _ in range(3))
+        out.append(("u3", (q,), (th, ph, la), (), ()))
+    return out
+
+
+def ghz_u3_cz(n: int, layers: int, seed: int):
+    rng = SplitMix64(seed)
+    circ = [("h", (0,), (), (), ())]
+    for q in range(1, n):
+        circ.append(("x", (q,), (), (0,), ()))
+    for _ in range(layers):
+        circ += _u3_layer(rng, n)
+        perm = list(range(n))
+        for i in range(n - 1, 0, -1):
+            j = rng.randbelow(i + 1)
+            perm[i], perm[j] = perm[j], perm[i]
+        for k in range(0, n - 1, 2):
+            a, b = perm[k], perm[k + 1]
+            circ.append(("z", (b,), (), (a,), ()))
+    return circ
+
+
+def brick(n: int, depth: int, seed: int, mcx: int = 0):
+    rng = SplitMix64(seed)
+    after = sorted(rng.randbelow(depth) for _ in range(mcx))
+    circ = []
+    for layer in range(depth):
+        circ += _u3_layer(rng, n)
+        start = layer % 2
+        for a in range(start, n - 1, 2):
+            circ.append(("z", (a + 1,), (), (a,), ()))
+        while after and after[0] == layer:
+            after.pop(0)
+            high = n - 1 - rng.randbelow(min(6, n))
+            ops = [high]
+            while len(ops) < 4:
+                q = rng.randbelow(n)
+                if q not in ops:
+                    ops.append(q)
+            # shuffle which operand is the target
+            t = ops.pop(rng.randbelow(4))
+            circ.append(("x", (t,), (), tuple(ops), ()))
+    return circ
+
+
+def variational(n: int, depth: int, seed: int):
+    rng = SplitMix64(seed)
+    circ = []
+    for _ in range(depth):
+        for q in range(n):
+            circ.append(("ry", (q,), ((2 * rng.uniform() - 1) * math.pi,), (), ()))
+            circ.append(("rz", (q,), ((2 * rng.uniform() - 1) * math.pi,), (), ()))
+        for q in range(n - 1):
+            circ.append(("x", (q + 1,), (), (q,), ()))
+    return circ
+
+
+def qft(n: int):
+    circ = []
+    for j in range(n - 1, -1, -1):
+        circ.append(("h", (j,), (), (), ()))
+        for k in range(j - 1, -1, -1):
+            circ.append(("p", (j,), (math.pi / (1 << (j - k)),), (k,), ()))
+    for i in range(n // 2):
+        circ.append(("swap", (i, n - 1 - i), (), (), ()))
+    return circ
+
+
+def random_all_gates(n: int, count: int, seed: int, max_ctrls: int = 2):
+    """Random circuit over the whole gate set, random positive/negative controls."""
+    from . import GATE_PARAMS, GATE_TARGETS
+
+    names = sorted(GATE_TARGETS)
+    rng = SplitMix64(seed)
+    circ = []
+    while len(circ) < count:
+        name = names[rng.randbelow(len(names))]
+        t = GATE_TARGETS[name]
+        nc = rng.randbelow(max_ctrls + 1)
+        if t + nc > n:
+            continue
+        qs = list(range(n))
+        for i in range(n - 1, 0, -1):
+            j = rng.randbelow(i + 1)
+            qs[i], qs[j] = qs[j], qs[i]
+        targets = tuple(qs[:t])
+        ctl = qs[t:t + nc]
+        pos = tuple(c for c in ctl if rng.uniform() < 0.6)
+        neg = tuple(c for c in ctl if c not in pos)
+        npar = 1 if name == "global_phase" else GATE_PARAMS.get(name, 0)
+        params = tuple(2 * math.pi * rng.uniform() - math.pi for _ in range(npar))
+        circ.append((name, targets, params, pos, neg))
+    return circ
+
+
+def algorithmic_bytes(n: int, op, amp_bytes: int = 16) -> float:
+    """Bytes a gate must move: 2 * S * A, A = amplitudes its matrix can change."""
+    name, targets, params, ctrls, negctrls = op
+    c = len(ctrls) + len(negctrls)
+    if name in ("z", "s", "sdg", "t", "tdg", "p", "u1", "swap", "iswap", "xx_plus_yy", "xx_minus_yy"):
+        a = 2 ** (n - c - 1)
+    elif name == "dcx":
+        a = 3 * 2 ** (n - c - 2)
+    elif name == "rccx":
+        a = 3 * 2 ** (n - c - 3)
+    elif name == "rcccx":
+        a = 2 ** (n - c - 2)
+    elif name == "id":
+        a = 0
+    else:
+        a = 2 ** (n - c)
+    return 2.0 * amp_bytes * a
