@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--shots", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run one write-only fill and one read-only norm pass of the "
+                         "full state (known HBM bytes, for scaling PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -131,6 +134,9 @@ def main():
     for k, v in prof.items():
         gbs = v["bytes"] / v["seconds"] / 1e9 if v["seconds"] else 0
         log(f"  {k:9s} launches {v['launches']:6d}  {v['seconds']:.4f}s  avg {v['seconds'] / v['launches'] * 1e3:.3f} ms  {gbs:.1f} GB/s")
+    if args.calibrate:
+        sim.call("set_zero_state")  # writes 16 * 2^n bytes
+        log(f"calibration norm {sim.norm():.3f}")  # reads 16 * 2^n bytes
     sim.close()
 
     pair = prof.get("pair", {"launches": 1, "seconds": 1, "bytes": 0})
