@@ -101,6 +101,14 @@ int qcs_ext_set_precision(qcs_simulator *sim, int mode, int max_block_qubits);
 /* Apply all queued gates now (no-op in mode 0). */
 int qcs_ext_flush(qcs_simulator *sim);
 
+/* Gates applied through fused blocks and number of block launches so far (mode 1). */
+int qcs_ext_fusion_stats(qcs_simulator *sim, uint64_t *gates, uint64_t *blocks);
+
+/* <a|b> of two simulators with the same size and mapping on the same device, FP64:
+ * out[0] = real part, out[1] = imaginary part, out[2] = max_i |a_i - b_i|.  Either state
+ * may be complex128 or complex64.  The fidelity is |<a|b>|^2 / (<a|a> <b|b>). */
+int qcs_ext_overlap(qcs_simulator *a, qcs_simulator *b, double *out);
+
 /* Per-kernel-class timing with HIP events around every launch (oracle: host clock). */
 int qcs_ext_profile(qcs_simulator *sim, int enable);
 int qcs_ext_profile_get(qcs_simulator *sim, int kernel_class, uint64_t *launches, double *seconds, double *bytes);

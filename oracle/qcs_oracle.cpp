@@ -1281,6 +1281,34 @@ extern "C"
         });
     }
     int qcs_ext_flush(qcs_simulator *s) { return guard([&] { (void)core(s); }); }
+    int qcs_ext_fusion_stats(qcs_simulator *s, uint64_t *gates, uint64_t *blocks)
+    {
+        return guard([&] {
+            (void)core(s);
+            put(gates, 0);
+            put(blocks, 0);
+        });
+    }
+    int qcs_ext_overlap(qcs_simulator *a, qcs_simulator *b, double *out)
+    {
+        return guard([&] {
+            auto &x = core(a), &y = core(b);
+            x.need_alloc();
+            y.need_alloc();
+            if (x.n != y.n || x.p2l != y.p2l || out == nullptr)
+                throw std::runtime_error("overlap needs two states of the same size and qubit mapping");
+            oracle::cplx acc = 0.0;
+            double md = 0.0;
+            for (size_t i = 0; i < x.state.size(); ++i)
+            {
+                acc += std::conj(x.state[i]) * y.state[i];
+                md = std::max(md, std::abs(x.state[i] - y.state[i]));
+            }
+            out[0] = acc.real();
+            out[1] = acc.imag();
+            out[2] = md;
+        });
+    }
     int qcs_ext_profile(qcs_simulator *s, int en) { return guard([&] { core(s).profile = en != 0; }); }
     int qcs_ext_profile_get(qcs_simulator *s, int cls, uint64_t *launches, double *seconds, double *bytes)
     {

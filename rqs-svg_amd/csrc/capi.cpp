@@ -409,6 +409,22 @@ extern "C"
     int qcs_ext_permute(qcs_simulator *s, const bit_num_t *p, bit_num_t count) { return guard([&] { core(s).permute(vec(p, count)); }); }
     int qcs_ext_set_precision(qcs_simulator *s, int mode, int k) { return guard([&] { core(s).set_precision(mode, k); }); }
     int qcs_ext_flush(qcs_simulator *s) { return guard([&] { core(s).flush(); }); }
+    int qcs_ext_fusion_stats(qcs_simulator *s, uint64_t *gates, uint64_t *blocks)
+    {
+        return guard([&] {
+            if (!gates || !blocks)
+                throw std::runtime_error("result pointer must not be NULL");
+            core(s).fusion_stats(gates, blocks);
+        });
+    }
+    int qcs_ext_overlap(qcs_simulator *a, qcs_simulator *b, double *out)
+    {
+        return guard([&] {
+            if (!out)
+                throw std::runtime_error("result pointer must not be NULL");
+            core(a).overlap(core(b), out);
+        });
+    }
     int qcs_ext_profile(qcs_simulator *s, int en) { return guard([&] { core(s).profile_enable(en != 0); }); }
     int qcs_ext_profile_get(qcs_simulator *s, int cls, uint64_t *launches, double *seconds, double *bytes)
     {

@@ -1,24 +1,365 @@
-// Precision form (complex64 fused blocks): not available in this build yet.
+// Precision form: gate queue, greedy fusion planner, FP64 block composition, block launches
+// on the complex64 state.  FP64 is used for the block matrices (host) and, through the
+// shared support kernels, for every reduction, prefix sum and renormalisation; the state
+// itself and the block products are FP32.
 #include "fused.hpp"
 
+#include <qcs_ext.h>
+
+#include <algorithm>
+#include <cstring>
 #include <stdexcept>
 
 namespace qcore
 {
+    namespace
+    {
+        std::vector<int> gate_qubits(const ResolvedGate &g)
+        {
+            std::vector<int> q = g.tphys;
+            q.insert(q.end(), g.pphys.begin(), g.pphys.end());
+            q.insert(q.end(), g.nphys.begin(), g.nphys.end());
+            return q;
+        }
+
+        int union_size(const std::vector<int> &a, const std::vector<int> &b)
+        {
+            int extra = 0;
+            for (int x : b)
+                if (std::find(a.begin(), a.end(), x) == a.end())
+                    ++extra;
+            return (int)a.size() + extra;
+        }
+
+        int local_pos(const std::vector<int> &qubits, int q)
+        {
+            auto it = std::find(qubits.begin(), qubits.end(), q);
+            return it == qubits.end() ? -1 : int(it - qubits.begin());
+        }
+    } // namespace
+
+    std::vector<PlannedBlock> plan_greedy(const std::vector<ResolvedGate> &gates, int k)
+    {
+        std::vector<PlannedBlock> blocks;
+        std::vector<int> last(qsv::kMaxQubits, -1);
+        const int kWindow = 64; // blocks examined after the dependency frontier
+        for (int gi = 0; gi < (int)gates.size(); ++gi)
+        {
+            auto Q = gate_qubits(gates[gi]);
+            int lo = -1;
+            for (int q : Q)
+                lo = std::max(lo, last[q]);
+            int chosen = -1;
+            if ((int)Q.size() <= k)
+            {
+                int best_cost = 1 << 30;
+                int start = std::max(lo, 0);
+                int stop = std::min((int)blocks.size(), start + kWindow);
+                for (int j = start; j < stop; ++j)
+                {
+                    if (blocks[j].wide)
+                        continue;
+                    int u = union_size(blocks[j].qubits, Q);
+                    if (u > k)
+                        continue;
+                    int cost = u - (int)blocks[j].qubits.size();
+                    if (cost < best_cost)
+                    {
+                        best_cost = cost;
+                        chosen = j;
+                    }
+                }
+            }
+            if (chosen < 0)
+            {
+                PlannedBlock b;
+                b.wide = (int)Q.size() > k;
+                blocks.push_back(b);
+                chosen = (int)blocks.size() - 1;
+            }
+            auto &b = blocks[chosen];
+            for (int q : Q)
+                if (std::find(b.qubits.begin(), b.qubits.end(), q) == b.qubits.end())
+                    b.qubits.push_back(q);
+            b.gates.push_back(gi);
+            for (int q : Q)
+                last[q] = chosen;
+        }
+        return blocks;
+    }
+
+    BlockLayout layout_block(const std::vector<ResolvedGate> &gates, const PlannedBlock &b, int kq, int n)
+    {
+        BlockLayout lay;
+        if (!b.wide)
+        {
+            lay.qubits = b.qubits;
+        }
+        else
+        {
+            // one gate wider than the block: targets inside, controls outside as far as the
+            // free qubits allow padding; otherwise some controls move inside
+            const ResolvedGate &g = gates[b.gates[0]];
+            lay.qubits = g.tphys;
+            std::vector<int> ctrls = g.pphys;
+            ctrls.insert(ctrls.end(), g.nphys.begin(), g.nphys.end());
+            int free_q = n - (int)g.tphys.size() - (int)ctrls.size();
+            int need = kq - (int)lay.qubits.size();
+            int inside = std::max(0, need - free_q);
+            for (int i = 0; i < inside; ++i)
+                lay.qubits.push_back(ctrls[i]);
+            for (size_t i = inside; i < ctrls.size(); ++i)
+            {
+                int q = ctrls[i];
+                if (std::find(g.pphys.begin(), g.pphys.end(), q) != g.pphys.end())
+                    lay.pos_out.push_back(q);
+                else
+                    lay.neg_out.push_back(q);
+            }
+        }
+        if ((int)lay.qubits.size() > kq)
+            throw std::runtime_error("fusion: block larger than the kernel block size");
+        // pad with the highest free qubits (low qubits stay in the column index, which keeps
+        // consecutive lanes on consecutive amplitudes)
+        for (int q = n - 1; q >= 0 && (int)lay.qubits.size() < kq; --q)
+        {
+            bool used = local_pos(lay.qubits, q) >= 0 || local_pos(lay.pos_out, q) >= 0 || local_pos(lay.neg_out, q) >= 0;
+            if (!used)
+                lay.qubits.push_back(q);
+        }
+        if ((int)lay.qubits.size() != kq)
+            throw std::runtime_error("fusion: not enough qubits to pad the block");
+        return lay;
+    }
+
+    std::vector<cd> block_matrix(const std::vector<ResolvedGate> &gates, const PlannedBlock &b, const BlockLayout &lay)
+    {
+        const int kq = (int)lay.qubits.size();
+        const int K = 1 << kq;
+        std::vector<cd> M(size_t(K) * K, 0.0);
+        for (int i = 0; i < K; ++i)
+            M[size_t(i) * K + i] = 1.0;
+        std::vector<cd> col(K), in(16), out(16);
+        for (int gi : b.gates)
+        {
+            const ResolvedGate &g = gates[gi];
+            const int t = g.spec.ntargets;
+            const int T = 1 << t;
+            auto U = dense_matrix(g.spec);
+            int tpos[4] = {0, 0, 0, 0};
+            for (int j = 0; j < t; ++j)
+                tpos[j] = local_pos(lay.qubits, g.tphys[j]);
+            unsigned need1 = 0, need0 = 0; // control bits inside the block
+            for (int q : g.pphys)
+            {
+                int p = local_pos(lay.qubits, q);
+                if (p >= 0)
+                    need1 |= 1u << p;
+            }
+            for (int q : g.nphys)
+            {
+                int p = local_pos(lay.qubits, q);
+                if (p >= 0)
+                    need0 |= 1u << p;
+            }
+            unsigned tmask = 0;
+            for (int j = 0; j < t; ++j)
+                tmask |= 1u << tpos[j];
+            // M <- G M, applying the controlled gate to every column of M
+            for (int c = 0; c < K; ++c)
+            {
+                for (int r = 0; r < K; ++r)
+                    col[r] = M[size_t(r) * K + c];
+                for (int x = 0; x < K; ++x)
+                {
+                    if (x & tmask)
+                        continue;
+                    if ((x & need1) != need1 || (x & need0) != 0)
+                        continue;
+                    for (int r = 0; r < T; ++r)
+                    {
+                        int y = x;
+                        for (int j = 0; j < t; ++j)
+                            if ((r >> j) & 1)
+                                y |= 1 << tpos[j];
+                        in[r] = col[y];
+                    }
+                    for (int r = 0; r < T; ++r)
+                    {
+                        cd acc = 0.0;
+                        for (int cc = 0; cc < T; ++cc)
+                            acc += U[size_t(r) * T + cc] * in[cc];
+                        out[r] = acc;
+                    }
+                    for (int r = 0; r < T; ++r)
+                    {
+                        int y = x;
+                        for (int j = 0; j < t; ++j)
+                            if ((r >> j) & 1)
+                                y |= 1 << tpos[j];
+                        col[y] = out[r];
+                    }
+                }
+                for (int r = 0; r < K; ++r)
+                    M[size_t(r) * K + c] = col[r];
+            }
+        }
+        return M;
+    }
+
     FusedEngine::FusedEngine(Simulator &sim, int k) : sim_(sim), k_(k)
     {
-        throw std::runtime_error("precision mode 1 is not available in this build");
+        const int n = sim_.n_;
+        if (n < 4)
+            throw std::runtime_error("precision mode needs at least 4 qubits");
+        kq_ = (k <= 4 || n < 5) ? 4 : 5;
+        if (k_ > kq_)
+            k_ = kq_;
+        qsv::check(hipMalloc((void **)&state_, (uint64_t(1) << n) * sizeof(qsv::c64)), "hipMalloc c64 state");
+        qsv::check(hipEventCreateWithFlags(&copy_done_, hipEventDisableTiming), "hipEventCreate");
     }
-    FusedEngine::~FusedEngine() {}
-    void FusedEngine::enqueue(const ResolvedGate &) {}
-    void FusedEngine::flush() {}
-    void FusedEngine::fill(int) {}
-    void FusedEngine::swap_physical(int, int) {}
-    int FusedEngine::measure(int) { return 0; }
-    void FusedEngine::get_statevector(double *) {}
-    void FusedEngine::set_statevector(const double *) {}
-    double FusedEngine::probability_total() { return 0; }
-    void FusedEngine::sample(uint64_t, uint64_t, uint64_t *) {}
-    double FusedEngine::norm() { return 0; }
-    void FusedEngine::expect_z(double *, int) {}
+
+    FusedEngine::~FusedEngine()
+    {
+        (void)hipStreamSynchronize(sim_.stream_);
+        if (state_)
+            (void)hipFree(state_);
+        if (rdev_)
+            (void)hipFree(rdev_);
+        if (rhost_)
+            (void)hipHostFree(rhost_);
+        if (copy_done_)
+            (void)hipEventDestroy(copy_done_);
+    }
+
+    void FusedEngine::reserve_matrices(size_t floats)
+    {
+        if (copy_pending_)
+        {
+            qsv::check(hipEventSynchronize(copy_done_), "hipEventSynchronize");
+            copy_pending_ = false;
+        }
+        if (floats <= rcap_)
+            return;
+        // the device buffer may still be read by queued kernels: wait before freeing it
+        qsv::check(hipStreamSynchronize(sim_.stream_), "hipStreamSynchronize");
+        if (rdev_)
+            (void)hipFree(rdev_);
+        if (rhost_)
+            (void)hipHostFree(rhost_);
+        rdev_ = nullptr;
+        rhost_ = nullptr;
+        size_t cap = std::max<size_t>(floats, 64 * 64 * 64);
+        qsv::check(hipMalloc((void **)&rdev_, cap * sizeof(float)), "hipMalloc block matrices");
+        qsv::check(hipHostMalloc((void **)&rhost_, cap * sizeof(float), hipHostMallocDefault), "hipHostMalloc block matrices");
+        rcap_ = cap;
+    }
+
+    void FusedEngine::enqueue(const ResolvedGate &g)
+    {
+        if (g.mmask)
+        {
+            // Inputs on the unobserved branch of a measured target read as zero
+            // (src/qcs.cu:1153-1171): zero that branch in memory first, then apply normally.
+            flush();
+            for (int j = 0; j < g.spec.ntargets; ++j)
+                if ((g.mmask >> j) & 1)
+                    qsv::launch_collapse(sim_.stream_, state_, sim_.n_, g.tphys[j], (g.mval >> j) & 1, 1.0);
+            ResolvedGate h = g;
+            h.mmask = h.mval = 0;
+            queue_.push_back(h);
+        }
+        else
+        {
+            queue_.push_back(g);
+        }
+        if (queue_.size() >= 4096)
+            flush();
+    }
+
+    void FusedEngine::flush()
+    {
+        if (queue_.empty())
+            return;
+        std::vector<ResolvedGate> gates;
+        gates.swap(queue_);
+        auto plan = plan_greedy(gates, k_);
+        apply_blocks(gates, plan);
+    }
+
+    void FusedEngine::swap_physical(int a, int b)
+    {
+        flush();
+        ResolvedGate g;
+        double none[1] = {0.0};
+        g.spec = make_gate("swap", none);
+        g.tphys = {a, b};
+        std::vector<ResolvedGate> gates{g};
+        apply_blocks(gates, plan_greedy(gates, k_));
+    }
+
+    void FusedEngine::apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan)
+    {
+        const int n = sim_.n_;
+        const int D = 2 << kq_;
+        const size_t per = size_t(D) * D;
+        reserve_matrices(per * plan.size());
+        std::vector<BlockLayout> lays;
+        lays.reserve(plan.size());
+        for (size_t bi = 0; bi < plan.size(); ++bi)
+        {
+            lays.push_back(layout_block(gates, plan[bi], kq_, n));
+            auto M = block_matrix(gates, plan[bi], lays.back());
+            const int K = 1 << kq_;
+            float *R = rhost_ + bi * per;
+            for (int r = 0; r < K; ++r)
+                for (int c = 0; c < K; ++c)
+                {
+                    const cd z = M[size_t(r) * K + c];
+                    const float re = float(z.real()), im = float(z.imag());
+                    R[(2 * r) * D + 2 * c] = re;
+                    R[(2 * r) * D + 2 * c + 1] = -im;
+                    R[(2 * r + 1) * D + 2 * c] = im;
+                    R[(2 * r + 1) * D + 2 * c + 1] = re;
+                }
+        }
+        qsv::check(hipMemcpyAsync(rdev_, rhost_, per * plan.size() * sizeof(float), hipMemcpyHostToDevice, sim_.stream_), "matrix upload");
+        qsv::check(hipEventRecord(copy_done_, sim_.stream_), "hipEventRecord");
+        copy_pending_ = true;
+        for (size_t bi = 0; bi < plan.size(); ++bi)
+        {
+            const BlockLayout &lay = lays[bi];
+            qsv::BlockOp op{};
+            op.kq = kq_;
+            for (int e = 0; e < (1 << kq_); ++e)
+            {
+                uint64_t o = 0;
+                for (int j = 0; j < kq_; ++j)
+                    if ((e >> j) & 1)
+                        o |= uint64_t(1) << lay.qubits[j];
+                op.off[e] = o;
+            }
+            op.R = rdev_ + bi * per;
+            std::vector<int> ops = lay.qubits;
+            uint64_t setb = 0;
+            for (int q : lay.pos_out)
+            {
+                ops.push_back(q);
+                setb |= uint64_t(1) << q;
+            }
+            ops.insert(ops.end(), lay.neg_out.begin(), lay.neg_out.end());
+            std::sort(ops.begin(), ops.end());
+            qsv::Subspace sub{};
+            sub.set_bits = setb;
+            sub.nops = (int)ops.size();
+            for (int i = 0; i < sub.nops; ++i)
+                sub.ops[i] = ops[i];
+            const double bytes = 2.0 * 8.0 * double(uint64_t(1) << (n - sub.nops + kq_));
+            sim_.prof_begin(QCS_EXT_K_FUSED, bytes);
+            qsv::launch_block(sim_.stream_, state_, n, sub, op);
+            sim_.prof_end();
+            blocks_applied_ += 1;
+            gates_applied_ += plan[bi].gates.size();
+        }
+    }
 } // namespace qcore

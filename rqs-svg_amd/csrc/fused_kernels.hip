@@ -1,0 +1,140 @@
+// Precision form: dense k-qubit blocks on a complex64 state, FP32 MFMA (gfx950).
+//
+// A block acts on KQ qubits (KQ = 4 or 5).  Its 2^KQ x 2^KQ complex matrix M is given as
+// the real D x D matrix R (D = 2^(KQ+1)) on interleaved (re, im) vectors:
+//   R[2r][2c] = Re M[r][c], R[2r][2c+1] = -Im M[r][c], R[2r+1][2c] = Im M[r][c],
+//   R[2r+1][2c+1] = Re M[r][c],
+// so that y = R x for the interleaved float vector x of the 2^KQ amplitudes of one
+// "column" (the amplitudes that differ only in the block's qubits).
+//
+// Work split: one wave takes 32 columns at a time.  The product R (D x D) * X (D x 32) runs
+// as D/32 output tiles of v_mfma_f32_32x32x2f32 with the K dimension split in two halves:
+// MFMA step s consumes k = s (lanes 0-31) and k = s + D/2 (lanes 32-63).  Lane (h, c)
+// therefore needs the contiguous half h of column c -- 2^KQ/2 amplitudes loaded straight
+// from HBM -- and keeps its rows of R in registers for the whole launch.  The D/32
+// accumulators hold, per lane, pairs of consecutive amplitudes of column c, written back
+// in place.  Accumulation is FP32.
+#include "fused_kernels.hpp"
+
+#include <stdexcept>
+#include <string>
+
+namespace qsv
+{
+    namespace
+    {
+        typedef float v2f __attribute__((ext_vector_type(2)));
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        typedef float v16f __attribute__((ext_vector_type(16)));
+
+        constexpr int kWaves = 4;
+        constexpr int kThreads = 64 * kWaves;
+
+        __device__ __forceinline__ uint64_t expand(uint64_t t, const Subspace &sub)
+        {
+            for (int k = 0; k < sub.nops; ++k)
+            {
+                uint64_t lo = t & ((uint64_t(1) << sub.ops[k]) - 1);
+                t = lo | ((t ^ lo) << 1);
+            }
+            return t | sub.set_bits;
+        }
+
+        template <int KQ>
+        __global__ __launch_bounds__(kThreads) void k_block(c64 *__restrict__ s, uint64_t ncols, Subspace sub, BlockOp op)
+        {
+            constexpr int D = 2 << KQ;   // real dimension
+            constexpr int MT = D / 32;   // 32-row output tiles
+            constexpr int H = D / 2;     // K values per lane
+            constexpr int AMPS = H / 2;  // amplitudes per lane
+
+            const int lane = threadIdx.x & 63;
+            const int c = lane & 31;
+            const int h = lane >> 5;
+
+            // rows 32*mt + c of R, columns [h*H, h*H + H): the A operands of every step
+            float a[MT][H];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+            {
+                const v4f *rp = reinterpret_cast<const v4f *>(op.R + (32 * mt + c) * D + h * H);
+#pragma unroll
+                for (int q = 0; q < H / 4; ++q)
+                {
+                    v4f v = rp[q];
+                    a[mt][4 * q + 0] = v.x;
+                    a[mt][4 * q + 1] = v.y;
+                    a[mt][4 * q + 2] = v.z;
+                    a[mt][4 * q + 3] = v.w;
+                }
+            }
+
+            const uint64_t ntiles = (ncols + 31) / 32;
+            const uint64_t nwaves = uint64_t(gridDim.x) * kWaves;
+            for (uint64_t tile = uint64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6); tile < ntiles; tile += nwaves)
+            {
+                const uint64_t col = tile * 32 + c;
+                const bool ok = col < ncols;
+                const uint64_t base = ok ? expand(col, sub) : 0;
+                float b[H];
+#pragma unroll
+                for (int j = 0; j < AMPS; ++j)
+                {
+                    const uint64_t off = h ? op.off[AMPS + j] : op.off[j];
+                    v2f v = ok ? __builtin_nontemporal_load(reinterpret_cast<const v2f *>(s + (base | off))) : v2f{0.f, 0.f};
+                    b[2 * j] = v.x;
+                    b[2 * j + 1] = v.y;
+                }
+                v16f d[MT];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    d[mt] = v16f{0.f};
+#pragma unroll
+                for (int st = 0; st < H; ++st)
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                        d[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt][st], b[st], d[mt], 0, 0, 0);
+                if (!ok)
+                    continue;
+                // accumulator register r of tile mt holds row 32*mt + 8*(r/4) + 4*h + r%4
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                    {
+                        const int e0 = 16 * mt + 4 * q; // + 2h
+                        const uint64_t o0 = h ? op.off[e0 + 2] : op.off[e0];
+                        const uint64_t o1 = h ? op.off[e0 + 3] : op.off[e0 + 1];
+                        __builtin_nontemporal_store(v2f{d[mt][4 * q + 0], d[mt][4 * q + 1]}, reinterpret_cast<v2f *>(s + (base | o0)));
+                        __builtin_nontemporal_store(v2f{d[mt][4 * q + 2], d[mt][4 * q + 3]}, reinterpret_cast<v2f *>(s + (base | o1)));
+                    }
+            }
+        }
+    } // namespace
+
+    void launch_block(hipStream_t st, c64 *state, int n, const Subspace &sub, const BlockOp &op)
+    {
+        if (op.kq != 4 && op.kq != 5)
+            throw std::runtime_error("block: kq must be 4 or 5");
+        if (sub.nops < op.kq || sub.nops > n)
+            throw std::runtime_error("block: operand count out of range");
+        for (int k = 0; k < sub.nops; ++k)
+            if (sub.ops[k] < 0 || sub.ops[k] >= n || (k > 0 && sub.ops[k] <= sub.ops[k - 1]))
+                throw std::runtime_error("block: operands must be ascending and < n");
+        for (int e = 0; e < (1 << op.kq); ++e)
+            if (n < 64 && (op.off[e] >> n) != 0)
+                throw std::runtime_error("block: offset outside the state");
+        if (op.R == nullptr)
+            throw std::runtime_error("block: matrix pointer is NULL");
+        const uint64_t ncols = uint64_t(1) << (n - sub.nops);
+        const uint64_t tiles = (ncols + 31) / 32;
+        uint64_t grid = (tiles + kWaves - 1) / kWaves;
+        if (grid > 4096)
+            grid = 4096;
+        if (op.kq == 5)
+            hipLaunchKernelGGL(k_block<5>, dim3(grid), dim3(kThreads), 0, st, state, ncols, sub, op);
+        else
+            hipLaunchKernelGGL(k_block<4>, dim3(grid), dim3(kThreads), 0, st, state, ncols, sub, op);
+        check(hipGetLastError(), "block kernel");
+    }
+} // namespace qsv

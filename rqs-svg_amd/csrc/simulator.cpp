@@ -55,6 +55,16 @@ namespace qcore
         }
     } // namespace
 
+    // Runs f on the state pointer of the current precision (complex128 or complex64).
+    template <class F>
+    void Simulator::with_state(F &&f)
+    {
+        if (fused_)
+            f(fused_->state());
+        else
+            f(state_);
+    }
+
     Simulator::Simulator()
     {
         qsv::check(hipGetDevice(&device_), "hipGetDevice");
@@ -208,9 +218,9 @@ namespace qcore
         need_alloc();
         clear_measurement_state();
         if (fused_)
-            return fused_->fill(0);
-        prof_begin(QCS_EXT_K_INIT, 16.0 * (1ull << n_));
-        qsv::launch_fill_basis(stream_, state_, uint64_t(1) << n_, 0);
+            fused_->drop(); // queued gates act on a state that is being overwritten
+        prof_begin(QCS_EXT_K_INIT, (fused_ ? 8.0 : 16.0) * (1ull << n_));
+        with_state([&](auto *st) { qsv::launch_fill_basis(stream_, st, uint64_t(1) << n_, 0); });
         prof_end();
     }
 
@@ -219,8 +229,8 @@ namespace qcore
         need_alloc();
         clear_measurement_state();
         if (fused_)
-            return fused_->fill(1);
-        qsv::launch_fill_sequential(stream_, state_, uint64_t(1) << n_);
+            fused_->drop();
+        with_state([&](auto *st) { qsv::launch_fill_sequential(stream_, st, uint64_t(1) << n_); });
     }
 
     void Simulator::set_flat_state()
@@ -228,17 +238,16 @@ namespace qcore
         need_alloc();
         clear_measurement_state();
         if (fused_)
-            return fused_->fill(2);
-        qsv::launch_fill_flat(stream_, state_, uint64_t(1) << n_);
+            fused_->drop();
+        with_state([&](auto *st) { qsv::launch_fill_flat(stream_, st, uint64_t(1) << n_); });
     }
 
     void Simulator::set_entangled_state()
     { // src/qcs.cu:2081-2095 with one rank: amp[0] = 1, nothing else written
         need_alloc();
         clear_measurement_state();
-        if (fused_)
-            return fused_->fill(3);
-        qsv::launch_set_one(stream_, state_, 0);
+        flush(); // amp[0] = 1 on top of the current state
+        with_state([&](auto *st) { qsv::launch_set_one(stream_, st, 0); });
     }
 
     void Simulator::set_random_state()
@@ -246,8 +255,8 @@ namespace qcore
       // The measured lists are kept, as in the reference.
         need_alloc();
         if (fused_)
-            return fused_->fill(4);
-        qsv::launch_fill_random(stream_, state_, uint64_t(1) << n_, seed_);
+            fused_->drop();
+        with_state([&](auto *st) { qsv::launch_fill_random(stream_, st, uint64_t(1) << n_, seed_); });
     }
 
     void Simulator::clear_measurement_state()
@@ -545,8 +554,7 @@ namespace qcore
             return 0;
         if (contains(m1_, q))
             return 1;
-        if (fused_)
-            return fused_->measure(q);
+        flush();
         std::vector<int> tphys = {l2p_[q]};
         ensure_local(tphys);
         const int tq = tphys[0];
@@ -561,8 +569,9 @@ namespace qcore
             ops.push_back(l2p_[x]);
         auto sub = make_sub(ops, setb);
         const uint64_t items = uint64_t(1) << (n_ - sub.nops);
-        prof_begin(QCS_EXT_K_REDUCE, 32.0 * items);
-        qsv::reduce_pair_probs(stream_, state_, sub, uint64_t(1) << tq, items, work_, pinned_);
+        const double S = fused_ ? 8.0 : 16.0;
+        prof_begin(QCS_EXT_K_REDUCE, 2.0 * S * items);
+        with_state([&](auto *st) { qsv::reduce_pair_probs(stream_, st, sub, uint64_t(1) << tq, items, work_, pinned_); });
         prof_end();
         const double p0 = pinned_[0], p1 = pinned_[1];
         std::uniform_real_distribution<double> dist(0, p0 + p1);
@@ -571,8 +580,8 @@ namespace qcore
         (result ? m1_ : m0_).push_back(q);
         const double pk = result ? p1 : p0;
         const double scale = (renormalize_ && pk > 0) ? 1.0 / std::sqrt(pk) : 1.0;
-        prof_begin(QCS_EXT_K_COLLAPSE, 16.0 * (1ull << (n_ - 1)) * (scale != 1.0 ? 3.0 : 1.0));
-        qsv::launch_collapse(stream_, state_, n_, tq, result, scale);
+        prof_begin(QCS_EXT_K_COLLAPSE, S * (1ull << (n_ - 1)) * (scale != 1.0 ? 3.0 : 1.0));
+        with_state([&](auto *st) { qsv::launch_collapse(stream_, st, n_, tq, result, scale); });
         prof_end();
         return result;
     }
@@ -610,26 +619,15 @@ namespace qcore
         try
         {
             const uint64_t count = uint64_t(1) << n_;
-            std::vector<double> buf;
-            if (fused_)
+            ensure_stage();
+            const uint64_t chunk = logical_chunk();
+            for (uint64_t first = 0; first < count; first += chunk)
             {
-                buf.resize(2 * count);
-                fused_->get_statevector(buf.data());
-                if (std::fwrite(buf.data(), 16, count, fp) != count)
+                with_state([&](auto *st) { qsv::launch_gather_logical(stream_, st, stage_dev_, first, chunk, l2p_.data(), n_); });
+                qsv::check(hipMemcpyAsync(stage_host_, stage_dev_, chunk * sizeof(qsv::c128), hipMemcpyDeviceToHost, stream_), "save copy");
+                qsv::check(hipStreamSynchronize(stream_), "save sync");
+                if (std::fwrite(stage_host_, sizeof(qsv::c128), chunk, fp) != chunk)
                     throw std::runtime_error("fwrite failed");
-            }
-            else
-            {
-                ensure_stage();
-                const uint64_t chunk = logical_chunk();
-                for (uint64_t first = 0; first < count; first += chunk)
-                {
-                    qsv::launch_gather_logical(stream_, state_, stage_dev_, first, chunk, l2p_.data(), n_);
-                    qsv::check(hipMemcpyAsync(stage_host_, stage_dev_, chunk * sizeof(qsv::c128), hipMemcpyDeviceToHost, stream_), "save copy");
-                    qsv::check(hipStreamSynchronize(stream_), "save sync");
-                    if (std::fwrite(stage_host_, sizeof(qsv::c128), chunk, fp) != chunk)
-                        throw std::runtime_error("fwrite failed");
-                }
             }
         }
         catch (...)
@@ -647,13 +645,11 @@ namespace qcore
         flush();
         if (out == nullptr || count != (uint64_t(1) << n_))
             throw std::runtime_error("statevector buffer size mismatch");
-        if (fused_)
-            return fused_->get_statevector(out);
         ensure_stage();
         const uint64_t chunk = logical_chunk();
         for (uint64_t first = 0; first < count; first += chunk)
         {
-            qsv::launch_gather_logical(stream_, state_, stage_dev_, first, chunk, l2p_.data(), n_);
+            with_state([&](auto *st) { qsv::launch_gather_logical(stream_, st, stage_dev_, first, chunk, l2p_.data(), n_); });
             qsv::check(hipMemcpyAsync(stage_host_, stage_dev_, chunk * sizeof(qsv::c128), hipMemcpyDeviceToHost, stream_), "copy");
             qsv::check(hipStreamSynchronize(stream_), "sync");
             std::memcpy(out + 2 * first, stage_host_, chunk * sizeof(qsv::c128));
@@ -667,7 +663,7 @@ namespace qcore
         if (in == nullptr || count != (uint64_t(1) << n_))
             throw std::runtime_error("statevector buffer size mismatch");
         if (fused_)
-            return fused_->set_statevector(in);
+            fused_->drop();
         ensure_stage();
         const uint64_t chunk = logical_chunk();
         for (uint64_t first = 0; first < count; first += chunk)
@@ -675,7 +671,7 @@ namespace qcore
             qsv::check(hipStreamSynchronize(stream_), "sync");
             std::memcpy(stage_host_, in + 2 * first, chunk * sizeof(qsv::c128));
             qsv::check(hipMemcpyAsync(stage_dev_, stage_host_, chunk * sizeof(qsv::c128), hipMemcpyHostToDevice, stream_), "copy");
-            qsv::launch_scatter_logical(stream_, state_, stage_dev_, first, chunk, l2p_.data(), n_);
+            with_state([&](auto *st) { qsv::launch_scatter_logical(stream_, st, stage_dev_, first, chunk, l2p_.data(), n_); });
         }
         qsv::check(hipStreamSynchronize(stream_), "sync");
     }
@@ -700,11 +696,9 @@ namespace qcore
     {
         need_alloc();
         flush();
-        if (fused_)
-            return fused_->probability_total();
         ensure_levels();
-        prof_begin(QCS_EXT_K_REDUCE, 16.0 * (1ull << n_));
-        qsv::build_fold_levels(stream_, state_, uint64_t(1) << n_, levels_);
+        prof_begin(QCS_EXT_K_REDUCE, (fused_ ? 8.0 : 16.0) * (1ull << n_));
+        with_state([&](auto *st) { qsv::build_fold_levels(stream_, st, uint64_t(1) << n_, levels_); });
         prof_end();
         qsv::check(hipMemcpyAsync(pinned_, levels_.back(), sizeof(double), hipMemcpyDeviceToHost, stream_), "copy");
         qsv::check(hipStreamSynchronize(stream_), "sync");
@@ -717,8 +711,6 @@ namespace qcore
         if (shots > 0 && out == nullptr)
             throw std::runtime_error("outcome buffer must not be NULL");
         flush();
-        if (fused_)
-            return fused_->sample(seed, shots, out);
         const double total = probability_total();
         if (shots == 0)
             return;
@@ -727,7 +719,7 @@ namespace qcore
         try
         {
             prof_begin(QCS_EXT_K_SAMPLE, 0.0);
-            qsv::launch_sample_search(stream_, state_, uint64_t(1) << n_, levels_, total, seed, shots, p2l_.data(), n_, dev);
+            with_state([&](auto *st) { qsv::launch_sample_search(stream_, st, uint64_t(1) << n_, levels_, total, seed, shots, p2l_.data(), n_, dev); });
             prof_end();
             qsv::check(hipMemcpyAsync(out, dev, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_), "copy");
             qsv::check(hipStreamSynchronize(stream_), "sync");
@@ -744,10 +736,8 @@ namespace qcore
     {
         need_alloc();
         flush();
-        if (fused_)
-            return fused_->norm();
-        prof_begin(QCS_EXT_K_REDUCE, 16.0 * (1ull << n_));
-        qsv::reduce_probs(stream_, state_, uint64_t(1) << n_, 0, work_, pinned_);
+        prof_begin(QCS_EXT_K_REDUCE, (fused_ ? 8.0 : 16.0) * (1ull << n_));
+        with_state([&](auto *st) { qsv::reduce_probs(stream_, st, uint64_t(1) << n_, 0, work_, pinned_); });
         prof_end();
         return pinned_[0];
     }
@@ -758,12 +748,10 @@ namespace qcore
         flush();
         if (count < 0 || count > n_ || (count > 0 && out == nullptr))
             throw std::runtime_error("expect_z count out of range");
-        if (fused_)
-            return fused_->expect_z(out, count);
         for (int q = 0; q < count; ++q)
         {
-            prof_begin(QCS_EXT_K_REDUCE, 16.0 * (1ull << n_));
-            qsv::reduce_probs(stream_, state_, uint64_t(1) << n_, uint64_t(1) << l2p_[q], work_, pinned_);
+            prof_begin(QCS_EXT_K_REDUCE, (fused_ ? 8.0 : 16.0) * (1ull << n_));
+            with_state([&](auto *st) { qsv::reduce_probs(stream_, st, uint64_t(1) << n_, uint64_t(1) << l2p_[q], work_, pinned_); });
             prof_end();
             out[q] = pinned_[0];
         }
@@ -812,6 +800,39 @@ namespace qcore
             l2p_[lc] = q;
             l2p_[lq] = cur;
         }
+    }
+
+    void Simulator::overlap(Simulator &other, double out[3])
+    {
+        need_alloc();
+        other.need_alloc();
+        flush();
+        other.flush();
+        if (n_ != other.n_ || p2l_ != other.p2l_)
+            throw std::runtime_error("overlap needs two states of the same size and qubit mapping");
+        if (device_ != other.device_)
+            throw std::runtime_error("overlap needs both simulators on the same device");
+        qsv::check(hipStreamSynchronize(other.stream_), "hipStreamSynchronize");
+        const uint64_t count = uint64_t(1) << n_;
+        double *work = nullptr;
+        qsv::check(hipMalloc((void **)&work, qsv::overlap_workspace(count) * sizeof(double)), "hipMalloc overlap");
+        try
+        {
+            with_state([&](auto *a) { other.with_state([&](auto *b) { qsv::overlap(stream_, a, b, count, work, out); }); });
+        }
+        catch (...)
+        {
+            (void)hipFree(work);
+            throw;
+        }
+        (void)hipFree(work);
+    }
+
+    void Simulator::fusion_stats(uint64_t *gates, uint64_t *blocks)
+    {
+        flush();
+        *gates = fused_ ? fused_->gates_applied() : 0;
+        *blocks = fused_ ? fused_->blocks_applied() : 0;
     }
 
     void Simulator::flush()

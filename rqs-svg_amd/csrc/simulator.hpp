@@ -63,6 +63,8 @@ namespace qcore
         void permute(const std::vector<int> &want_p2l);
         void set_precision(int mode, int k);
         void flush();
+        void overlap(Simulator &other, double out[3]);
+        void fusion_stats(uint64_t *gates, uint64_t *blocks);
 
         int event_create();
         void event_record(int e);
@@ -78,6 +80,8 @@ namespace qcore
         hipStream_t stream() const { return stream_; }
 
     private:
+        template <class F>
+        void with_state(F &&f);
         void need_alloc() const;
         void check_operands(const std::vector<int> &t, const std::vector<int> &ng, const std::vector<int> &ps) const;
         void ensure_local(std::vector<int> &tphys);

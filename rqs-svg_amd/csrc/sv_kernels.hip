@@ -38,7 +38,6 @@ namespace qsv
 
         __device__ __forceinline__ v2d ld(const c128 *p) { return __builtin_nontemporal_load(reinterpret_cast<const v2d *>(p)); }
         __device__ __forceinline__ void st(c128 *p, v2d v) { __builtin_nontemporal_store(v, reinterpret_cast<v2d *>(p)); }
-        __device__ __forceinline__ v2d ldc(const c128 *p) { return *reinterpret_cast<const v2d *>(p); }
 
         __device__ __forceinline__ v2d cmul(c128 m, v2d a)
         {
@@ -75,6 +74,28 @@ namespace qsv
 
         // p(i) = fma(re, re, im*im), the probability definition shared with the oracle.
         __device__ __forceinline__ double prob(v2d a) { return __builtin_fma(a.x, a.x, a.y * a.y); }
+
+        typedef float v2f __attribute__((ext_vector_type(2)));
+
+        // Amplitude load/store in FP64 registers for both storage types; complex64 values are
+        // widened exactly (float -> double) and rounded to nearest on the way back.
+        __device__ __forceinline__ v2d lda(const c128 *p) { return ld(p); }
+        __device__ __forceinline__ v2d lda(const c64 *p)
+        {
+            v2f f = __builtin_nontemporal_load(reinterpret_cast<const v2f *>(p));
+            return v2d{double(f.x), double(f.y)};
+        }
+        __device__ __forceinline__ void sta(c128 *p, v2d v) { st(p, v); }
+        __device__ __forceinline__ void sta(c64 *p, v2d v)
+        {
+            __builtin_nontemporal_store(v2f{float(v.x), float(v.y)}, reinterpret_cast<v2f *>(p));
+        }
+        __device__ __forceinline__ v2d ldca(const c128 *p) { return *reinterpret_cast<const v2d *>(p); }
+        __device__ __forceinline__ v2d ldca(const c64 *p)
+        {
+            v2f f = *reinterpret_cast<const v2f *>(p);
+            return v2d{double(f.x), double(f.y)};
+        }
 
         template <int PER>
         __global__ __launch_bounds__(kBlock) void k_pair(c128 *__restrict__ s, uint64_t nitems, Subspace sub, PairOp op)
@@ -216,23 +237,26 @@ namespace qsv
         }
 
         // ---------------- state preparation ----------------
-        __global__ void k_fill_basis(c128 *s, uint64_t count, uint64_t index)
+        template <class A>
+        __global__ void k_fill_basis(A *s, uint64_t count, uint64_t index)
         {
             uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
             for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
-                st(s + i, v2d{i == index ? 1.0 : 0.0, 0.0});
+                sta(s + i, v2d{i == index ? 1.0 : 0.0, 0.0});
         }
-        __global__ void k_fill_sequential(c128 *s, uint64_t count)
+        template <class A>
+        __global__ void k_fill_sequential(A *s, uint64_t count)
         {
             uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
             for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
-                st(s + i, v2d{double(i), 0.0});
+                sta(s + i, v2d{double(i), 0.0});
         }
-        __global__ void k_fill_flat(c128 *s, uint64_t count)
+        template <class A>
+        __global__ void k_fill_flat(A *s, uint64_t count)
         {
             uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
             for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
-                st(s + i, v2d{1.0, 0.0});
+                sta(s + i, v2d{1.0, 0.0});
         }
         __device__ __forceinline__ uint64_t splitmix64(uint64_t z)
         {
@@ -244,7 +268,8 @@ namespace qsv
         {
             return double(splitmix64(seed + (j + 1) * 0x9E3779B97F4A7C15ull) >> 11) * 0x1.0p-53;
         }
-        __global__ void k_fill_random(c128 *s, uint64_t count, uint64_t seed)
+        template <class A>
+        __global__ void k_fill_random(A *s, uint64_t count, uint64_t seed)
         {
             uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
             for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
@@ -252,10 +277,11 @@ namespace qsv
                 double u1 = uniform(seed, 2 * i), u2 = uniform(seed, 2 * i + 1);
                 double rad = sqrt(-2.0 * log(1.0 - u1));
                 double ang = 2.0 * M_PI * u2;
-                st(s + i, v2d{rad * cos(ang), rad * sin(ang)});
+                sta(s + i, v2d{rad * cos(ang), rad * sin(ang)});
             }
         }
-        __global__ void k_set_one(c128 *s, uint64_t index) { s[index] = c128{1.0, 0.0}; }
+        template <class A>
+        __global__ void k_set_one(A *s, uint64_t index) { sta(s + index, v2d{1.0, 0.0}); }
 
         // ---------------- canonical reductions ----------------
         // Tile = 4096 items, 256 threads x 16 items at stride 256.  The register tree, the
@@ -298,7 +324,8 @@ namespace qsv
             }
         }
 
-        __global__ __launch_bounds__(kBlock) void k_reduce_pair(const c128 *__restrict__ s, uint64_t nitems, Subspace sub, uint64_t tbit, double *__restrict__ out, uint64_t out_stride)
+        template <class A>
+        __global__ __launch_bounds__(kBlock) void k_reduce_pair(const A *__restrict__ s, uint64_t nitems, Subspace sub, uint64_t tbit, double *__restrict__ out, uint64_t out_stride)
         {
             double y[2][16];
             uint64_t base = uint64_t(blockIdx.x) * kTile + threadIdx.x;
@@ -309,8 +336,8 @@ namespace qsv
                 if (item < nitems)
                 {
                     uint64_t i0 = expand(item, sub);
-                    y[0][j] = prob(ld(s + i0));
-                    y[1][j] = prob(ld(s + (i0 | tbit)));
+                    y[0][j] = prob(lda(s + i0));
+                    y[1][j] = prob(lda(s + (i0 | tbit)));
                 }
                 else
                 {
@@ -321,7 +348,8 @@ namespace qsv
             tile_tree<2>(y, out + blockIdx.x, out_stride);
         }
 
-        __global__ __launch_bounds__(kBlock) void k_reduce_probs(const c128 *__restrict__ s, uint64_t count, uint64_t zbit, double *__restrict__ out)
+        template <class A>
+        __global__ __launch_bounds__(kBlock) void k_reduce_probs(const A *__restrict__ s, uint64_t count, uint64_t zbit, double *__restrict__ out)
         {
             double y[1][16];
             uint64_t base = uint64_t(blockIdx.x) * kTile + threadIdx.x;
@@ -329,7 +357,7 @@ namespace qsv
             for (int j = 0; j < 16; ++j)
             {
                 uint64_t i = base + uint64_t(j) * kBlock;
-                double p = i < count ? prob(ld(s + i)) : 0.0;
+                double p = i < count ? prob(lda(s + i)) : 0.0;
                 y[0][j] = (zbit && (i & zbit)) ? -p : p;
             }
             tile_tree<1>(y, out + blockIdx.x, 0);
@@ -353,7 +381,8 @@ namespace qsv
         }
 
         // ---------------- collapse ----------------
-        __global__ __launch_bounds__(kBlock) void k_collapse(c128 *__restrict__ s, uint64_t half, int tq, int result, double scale, int rescale)
+        template <class A>
+        __global__ __launch_bounds__(kBlock) void k_collapse(A *__restrict__ s, uint64_t half, int tq, int result, double scale, int rescale)
         {
             uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
             const uint64_t bit = uint64_t(1) << tq;
@@ -363,18 +392,18 @@ namespace qsv
                 uint64_t i0 = lo | ((i ^ lo) << 1);
                 uint64_t kill = result ? i0 : (i0 | bit);
                 uint64_t keep = result ? (i0 | bit) : i0;
-                st(s + kill, v2d{0.0, 0.0});
+                sta(s + kill, v2d{0.0, 0.0});
                 if (rescale)
                 {
-                    v2d a = ld(s + keep);
-                    st(s + keep, v2d{a.x * scale, a.y * scale});
+                    v2d a = lda(s + keep);
+                    sta(s + keep, v2d{a.x * scale, a.y * scale});
                 }
             }
         }
 
         // ---------------- sampling ----------------
         // One wave folds 64 consecutive blocks of 64 entries (left to right).
-        template <bool FROM_STATE>
+        template <bool FROM_STATE, class A>
         __global__ __launch_bounds__(64) void k_fold64(const void *__restrict__ src, uint64_t count, double *__restrict__ out, uint64_t nout)
         {
             __shared__ double tile[64][65];
@@ -388,7 +417,7 @@ namespace qsv
                 if (e < count)
                 {
                     if (FROM_STATE)
-                        v = prob(ld(static_cast<const c128 *>(src) + e));
+                        v = prob(lda(static_cast<const A *>(src) + e));
                     else
                         v = static_cast<const double *>(src)[e];
                 }
@@ -418,12 +447,14 @@ namespace qsv
             int p2l[kMaxQubits];
         };
 
-        __device__ __forceinline__ double level_value(const c128 *s, const Levels &L, int lev, uint64_t j)
+        template <class A>
+        __device__ __forceinline__ double level_value(const A *s, const Levels &L, int lev, uint64_t j)
         {
-            return lev == 0 ? prob(ldc(s + j)) : L.ptr[lev][j];
+            return lev == 0 ? prob(ldca(s + j)) : L.ptr[lev][j];
         }
 
-        __global__ void k_sample_search(const c128 *__restrict__ s, Levels L, double total, uint64_t seed, uint64_t nshots, P2L map, uint64_t *__restrict__ out)
+        template <class A>
+        __global__ void k_sample_search(const A *__restrict__ s, Levels L, double total, uint64_t seed, uint64_t nshots, P2L map, uint64_t *__restrict__ out)
         {
             uint64_t shot = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
             if (shot >= nshots)
@@ -479,36 +510,77 @@ namespace qsv
                 p |= ((l >> q) & 1) << m.l2p[q];
             return p;
         }
-        __global__ void k_gather_logical(const c128 *__restrict__ s, c128 *__restrict__ out, uint64_t first, uint64_t len, L2P m)
+        template <class A>
+        __global__ void k_gather_logical(const A *__restrict__ s, c128 *__restrict__ out, uint64_t first, uint64_t len, L2P m)
         {
             uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
             for (; j < len; j += uint64_t(gridDim.x) * blockDim.x)
-                out[j] = s[to_physical(first + j, m)];
+            {
+                v2d a = ldca(s + to_physical(first + j, m));
+                out[j] = c128{a.x, a.y};
+            }
         }
-        __global__ void k_scatter_logical(c128 *__restrict__ s, const c128 *__restrict__ in, uint64_t first, uint64_t len, L2P m)
+        template <class A>
+        __global__ void k_scatter_logical(A *__restrict__ s, const c128 *__restrict__ in, uint64_t first, uint64_t len, L2P m)
         {
             uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
             for (; j < len; j += uint64_t(gridDim.x) * blockDim.x)
-                s[to_physical(first + j, m)] = in[j];
+                sta(s + to_physical(first + j, m), v2d{in[j].re, in[j].im});
         }
 
-        __global__ void k_cast_down(const c128 *__restrict__ in, float2 *__restrict__ out, uint64_t count)
+        // <a|b> over two states of equal size, FP64, canonical tree per component; also the
+        // largest |a_i - b_i| (as a max-reduction, order independent).
+        template <class A, class B>
+        __global__ __launch_bounds__(kBlock) void k_overlap(const A *__restrict__ a, const B *__restrict__ b, uint64_t count, double *__restrict__ out, uint64_t out_stride, double *__restrict__ maxdiff)
         {
-            uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-            for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+            double y[2][16];
+            double md = 0.0;
+            uint64_t base = uint64_t(blockIdx.x) * kTile + threadIdx.x;
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
             {
-                v2d a = ld(in + i);
-                out[i] = make_float2(float(a.x), float(a.y));
+                uint64_t i = base + uint64_t(j) * kBlock;
+                y[0][j] = 0.0;
+                y[1][j] = 0.0;
+                if (i < count)
+                {
+                    v2d x = ldca(a + i), z = ldca(b + i);
+                    y[0][j] = x.x * z.x + x.y * z.y; // Re(conj(x) z)
+                    y[1][j] = x.x * z.y - x.y * z.x; // Im(conj(x) z)
+                    double dx = x.x - z.x, dy = x.y - z.y;
+                    md = fmax(md, sqrt(dx * dx + dy * dy));
+                }
             }
+            tile_tree<2>(y, out + blockIdx.x, out_stride);
+            // max over the workgroup with an LDS tree (max is exact in any order)
+            __shared__ double mbuf[kBlock];
+            mbuf[threadIdx.x] = md;
+            __syncthreads();
+            for (int st_ = kBlock / 2; st_ >= 1; st_ /= 2)
+            {
+                if (threadIdx.x < st_)
+                    mbuf[threadIdx.x] = fmax(mbuf[threadIdx.x], mbuf[threadIdx.x + st_]);
+                __syncthreads();
+            }
+            if (threadIdx.x == 0)
+                maxdiff[blockIdx.x] = mbuf[0];
         }
-        __global__ void k_cast_up(const float2 *__restrict__ in, c128 *__restrict__ out, uint64_t count)
+        __global__ void k_max_tiles(const double *__restrict__ in, uint64_t m, double *__restrict__ out)
         {
-            uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-            for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+            __shared__ double mbuf[kBlock];
+            double md = 0.0;
+            for (uint64_t i = threadIdx.x; i < m; i += kBlock)
+                md = fmax(md, in[i]);
+            mbuf[threadIdx.x] = md;
+            __syncthreads();
+            for (int st_ = kBlock / 2; st_ >= 1; st_ /= 2)
             {
-                float2 a = in[i];
-                st(out + i, v2d{double(a.x), double(a.y)});
+                if (threadIdx.x < st_)
+                    mbuf[threadIdx.x] = fmax(mbuf[threadIdx.x], mbuf[threadIdx.x + st_]);
+                __syncthreads();
             }
+            if (threadIdx.x == 0)
+                out[0] = mbuf[0];
         }
 
         uint64_t grid_for(uint64_t items, uint64_t per_block)
@@ -599,29 +671,35 @@ namespace qsv
         after_launch("mono kernel");
     }
 
-    void launch_fill_basis(hipStream_t s, c128 *state, uint64_t count, uint64_t index)
+    // ---- support launchers for both amplitude types ----
+    template <class A>
+    void launch_fill_basis(hipStream_t s, A *state, uint64_t count, uint64_t index)
     {
-        hipLaunchKernelGGL(k_fill_basis, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count, index);
+        hipLaunchKernelGGL(k_fill_basis<A>, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count, index);
         after_launch("fill kernel");
     }
-    void launch_fill_sequential(hipStream_t s, c128 *state, uint64_t count)
+    template <class A>
+    void launch_fill_sequential(hipStream_t s, A *state, uint64_t count)
     {
-        hipLaunchKernelGGL(k_fill_sequential, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count);
+        hipLaunchKernelGGL(k_fill_sequential<A>, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count);
         after_launch("fill kernel");
     }
-    void launch_fill_flat(hipStream_t s, c128 *state, uint64_t count)
+    template <class A>
+    void launch_fill_flat(hipStream_t s, A *state, uint64_t count)
     {
-        hipLaunchKernelGGL(k_fill_flat, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count);
+        hipLaunchKernelGGL(k_fill_flat<A>, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count);
         after_launch("fill kernel");
     }
-    void launch_fill_random(hipStream_t s, c128 *state, uint64_t count, uint64_t seed)
+    template <class A>
+    void launch_fill_random(hipStream_t s, A *state, uint64_t count, uint64_t seed)
     {
-        hipLaunchKernelGGL(k_fill_random, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count, seed);
+        hipLaunchKernelGGL(k_fill_random<A>, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, state, count, seed);
         after_launch("fill kernel");
     }
-    void launch_set_one(hipStream_t s, c128 *state, uint64_t index)
+    template <class A>
+    void launch_set_one(hipStream_t s, A *state, uint64_t index)
     {
-        hipLaunchKernelGGL(k_set_one, dim3(1), dim3(1), 0, s, state, index);
+        hipLaunchKernelGGL(k_set_one<A>, dim3(1), dim3(1), 0, s, state, index);
         after_launch("set kernel");
     }
 
@@ -633,13 +711,13 @@ namespace qsv
             m = (m + kTile - 1) / kTile;
             total += m;
         }
-        return 2 * (total + 2);
+        return 2 * (total + 2) + 2;
     }
 
     namespace
     {
-        // Reduce NV arrays (at stride `stride` in `work`, length m each) down to one value
-        // each, written to work_out[v].
+        // Reduce NV arrays (length m each, at stride cur_stride) down to one value each,
+        // written to dev_out[v].
         template <int NV>
         void finish_tiles(hipStream_t s, double *cur, uint64_t m, uint64_t cur_stride, double *next, double *dev_out)
         {
@@ -660,7 +738,8 @@ namespace qsv
         }
     } // namespace
 
-    void reduce_pair_probs(hipStream_t s, const c128 *state, const Subspace &sub, uint64_t tbit, uint64_t nitems, double *work, double out[2])
+    template <class A>
+    void reduce_pair_probs(hipStream_t s, const A *state, const Subspace &sub, uint64_t tbit, uint64_t nitems, double *work, double out[2])
     {
         // work layout: [result(2)] [level arrays ...]
         double *dev_out = work;
@@ -668,12 +747,12 @@ namespace qsv
         double *lvl = work + 2;
         if (g == 1)
         {
-            hipLaunchKernelGGL(k_reduce_pair, dim3(1), dim3(kBlock), 0, s, state, nitems, sub, tbit, dev_out, uint64_t(1));
+            hipLaunchKernelGGL(k_reduce_pair<A>, dim3(1), dim3(kBlock), 0, s, state, nitems, sub, tbit, dev_out, uint64_t(1));
             after_launch("reduce kernel");
         }
         else
         {
-            hipLaunchKernelGGL(k_reduce_pair, dim3(g), dim3(kBlock), 0, s, state, nitems, sub, tbit, lvl, g);
+            hipLaunchKernelGGL(k_reduce_pair<A>, dim3(g), dim3(kBlock), 0, s, state, nitems, sub, tbit, lvl, g);
             after_launch("reduce kernel");
             finish_tiles<2>(s, lvl, g, g, lvl + 2 * g, dev_out);
         }
@@ -681,19 +760,20 @@ namespace qsv
         check(hipStreamSynchronize(s), "reduce sync");
     }
 
-    void reduce_probs(hipStream_t s, const c128 *state, uint64_t count, uint64_t zbit, double *work, double *out)
+    template <class A>
+    void reduce_probs(hipStream_t s, const A *state, uint64_t count, uint64_t zbit, double *work, double *out)
     {
         double *dev_out = work;
         uint64_t g = (count + kTile - 1) / kTile;
         double *lvl = work + 2;
         if (g == 1)
         {
-            hipLaunchKernelGGL(k_reduce_probs, dim3(1), dim3(kBlock), 0, s, state, count, zbit, dev_out);
+            hipLaunchKernelGGL(k_reduce_probs<A>, dim3(1), dim3(kBlock), 0, s, state, count, zbit, dev_out);
             after_launch("reduce kernel");
         }
         else
         {
-            hipLaunchKernelGGL(k_reduce_probs, dim3(g), dim3(kBlock), 0, s, state, count, zbit, lvl);
+            hipLaunchKernelGGL(k_reduce_probs<A>, dim3(g), dim3(kBlock), 0, s, state, count, zbit, lvl);
             after_launch("reduce kernel");
             finish_tiles<1>(s, lvl, g, g, lvl + g, dev_out);
         }
@@ -701,12 +781,41 @@ namespace qsv
         check(hipStreamSynchronize(s), "reduce sync");
     }
 
-    void launch_collapse(hipStream_t s, c128 *state, int n, int tq, int result, double scale)
+    template <class A, class B>
+    void overlap(hipStream_t s, const A *a, const B *b, uint64_t count, double *work, double out[3])
+    {
+        // work: [result(2)] [maxdiff result(1)] [per-tile maxdiff (g)] [level arrays ...]
+        uint64_t g = (count + kTile - 1) / kTile;
+        double *dev_out = work;
+        double *md_out = work + 2;
+        double *md_tiles = work + 3;
+        double *lvl = md_tiles + g;
+        if (g == 1)
+            hipLaunchKernelGGL((k_overlap<A, B>), dim3(1), dim3(kBlock), 0, s, a, b, count, dev_out, uint64_t(1), md_tiles);
+        else
+            hipLaunchKernelGGL((k_overlap<A, B>), dim3(g), dim3(kBlock), 0, s, a, b, count, lvl, g, md_tiles);
+        after_launch("overlap kernel");
+        if (g > 1)
+            finish_tiles<2>(s, lvl, g, g, lvl + 2 * g, dev_out);
+        hipLaunchKernelGGL(k_max_tiles, dim3(1), dim3(kBlock), 0, s, md_tiles, g, md_out);
+        after_launch("max kernel");
+        check(hipMemcpyAsync(out, dev_out, 3 * sizeof(double), hipMemcpyDeviceToHost, s), "overlap copy");
+        check(hipStreamSynchronize(s), "overlap sync");
+    }
+
+    uint64_t overlap_workspace(uint64_t count)
+    {
+        uint64_t g = (count + kTile - 1) / kTile;
+        return reduce_workspace(count) + g + 4;
+    }
+
+    template <class A>
+    void launch_collapse(hipStream_t s, A *state, int n, int tq, int result, double scale)
     {
         if (tq < 0 || tq >= n || n < 1)
             throw std::runtime_error("collapse: qubit out of range");
         uint64_t half = uint64_t(1) << (n - 1);
-        hipLaunchKernelGGL(k_collapse, dim3(grid_for(half, kBlock)), dim3(kBlock), 0, s, state, half, tq, result, scale, scale != 1.0 ? 1 : 0);
+        hipLaunchKernelGGL(k_collapse<A>, dim3(grid_for(half, kBlock)), dim3(kBlock), 0, s, state, half, tq, result, scale, scale != 1.0 ? 1 : 0);
         after_launch("collapse kernel");
     }
 
@@ -718,7 +827,8 @@ namespace qsv
         return sz;
     }
 
-    void build_fold_levels(hipStream_t s, const c128 *state, uint64_t count, const std::vector<double *> &levels)
+    template <class A>
+    void build_fold_levels(hipStream_t s, const A *state, uint64_t count, const std::vector<double *> &levels)
     {
         auto sz = fold_level_sizes(count);
         if (levels.size() + 1 != sz.size())
@@ -728,14 +838,15 @@ namespace qsv
             uint64_t nout = sz[lev];
             uint64_t groups = (nout + 63) / 64;
             if (lev == 1)
-                hipLaunchKernelGGL(k_fold64<true>, dim3(groups), dim3(64), 0, s, static_cast<const void *>(state), sz[0], levels[0], nout);
+                hipLaunchKernelGGL((k_fold64<true, A>), dim3(groups), dim3(64), 0, s, static_cast<const void *>(state), sz[0], levels[0], nout);
             else
-                hipLaunchKernelGGL(k_fold64<false>, dim3(groups), dim3(64), 0, s, static_cast<const void *>(levels[lev - 2]), sz[lev - 1], levels[lev - 1], nout);
+                hipLaunchKernelGGL((k_fold64<false, A>), dim3(groups), dim3(64), 0, s, static_cast<const void *>(levels[lev - 2]), sz[lev - 1], levels[lev - 1], nout);
             after_launch("fold kernel");
         }
     }
 
-    void launch_sample_search(hipStream_t s, const c128 *state, uint64_t count, const std::vector<double *> &levels, double total, uint64_t seed, uint64_t nshots, const int *p2l, int n, uint64_t *out_dev)
+    template <class A>
+    void launch_sample_search(hipStream_t s, const A *state, uint64_t count, const std::vector<double *> &levels, double total, uint64_t seed, uint64_t nshots, const int *p2l, int n, uint64_t *out_dev)
     {
         auto sz = fold_level_sizes(count);
         if (sz.size() > 12 || levels.size() + 1 != sz.size() || n > kMaxQubits)
@@ -753,38 +864,50 @@ namespace qsv
             m.p2l[q] = p2l[q];
         if (nshots == 0)
             return;
-        hipLaunchKernelGGL(k_sample_search, dim3((nshots + 63) / 64), dim3(64), 0, s, state, L, total, seed, nshots, m, out_dev);
+        hipLaunchKernelGGL(k_sample_search<A>, dim3((nshots + 63) / 64), dim3(64), 0, s, state, L, total, seed, nshots, m, out_dev);
         after_launch("sample kernel");
     }
 
-    void launch_gather_logical(hipStream_t s, const c128 *state, c128 *out, uint64_t first, uint64_t len, const int *l2p, int n)
+    template <class A>
+    void launch_gather_logical(hipStream_t s, const A *state, c128 *out, uint64_t first, uint64_t len, const int *l2p, int n)
     {
         L2P m{};
         m.n = n;
         for (int q = 0; q < n; ++q)
             m.l2p[q] = l2p[q];
-        hipLaunchKernelGGL(k_gather_logical, dim3(grid_for(len, kBlock)), dim3(kBlock), 0, s, state, out, first, len, m);
+        hipLaunchKernelGGL(k_gather_logical<A>, dim3(grid_for(len, kBlock)), dim3(kBlock), 0, s, state, out, first, len, m);
         after_launch("gather kernel");
     }
 
-    void launch_scatter_logical(hipStream_t s, c128 *state, const c128 *in, uint64_t first, uint64_t len, const int *l2p, int n)
+    template <class A>
+    void launch_scatter_logical(hipStream_t s, A *state, const c128 *in, uint64_t first, uint64_t len, const int *l2p, int n)
     {
         L2P m{};
         m.n = n;
         for (int q = 0; q < n; ++q)
             m.l2p[q] = l2p[q];
-        hipLaunchKernelGGL(k_scatter_logical, dim3(grid_for(len, kBlock)), dim3(kBlock), 0, s, state, in, first, len, m);
+        hipLaunchKernelGGL(k_scatter_logical<A>, dim3(grid_for(len, kBlock)), dim3(kBlock), 0, s, state, in, first, len, m);
         after_launch("scatter kernel");
     }
 
-    void launch_cast_c128_to_c64(hipStream_t s, const c128 *in, float2 *out, uint64_t count)
-    {
-        hipLaunchKernelGGL(k_cast_down, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, in, out, count);
-        after_launch("cast kernel");
-    }
-    void launch_cast_c64_to_c128(hipStream_t s, const float2 *in, c128 *out, uint64_t count)
-    {
-        hipLaunchKernelGGL(k_cast_up, dim3(grid_for(count, kBlock)), dim3(kBlock), 0, s, in, out, count);
-        after_launch("cast kernel");
-    }
+#define QSV_INSTANTIATE(A)                                                                                                                              \
+    template void launch_fill_basis<A>(hipStream_t, A *, uint64_t, uint64_t);                                                                           \
+    template void launch_fill_sequential<A>(hipStream_t, A *, uint64_t);                                                                                \
+    template void launch_fill_flat<A>(hipStream_t, A *, uint64_t);                                                                                      \
+    template void launch_fill_random<A>(hipStream_t, A *, uint64_t, uint64_t);                                                                          \
+    template void launch_set_one<A>(hipStream_t, A *, uint64_t);                                                                                        \
+    template void reduce_pair_probs<A>(hipStream_t, const A *, const Subspace &, uint64_t, uint64_t, double *, double *);                               \
+    template void reduce_probs<A>(hipStream_t, const A *, uint64_t, uint64_t, double *, double *);                                                      \
+    template void launch_collapse<A>(hipStream_t, A *, int, int, int, double);                                                                          \
+    template void build_fold_levels<A>(hipStream_t, const A *, uint64_t, const std::vector<double *> &);                                                \
+    template void launch_sample_search<A>(hipStream_t, const A *, uint64_t, const std::vector<double *> &, double, uint64_t, uint64_t, const int *, int, uint64_t *); \
+    template void launch_gather_logical<A>(hipStream_t, const A *, c128 *, uint64_t, uint64_t, const int *, int);                                      \
+    template void launch_scatter_logical<A>(hipStream_t, A *, const c128 *, uint64_t, uint64_t, const int *, int);
+
+    QSV_INSTANTIATE(c128)
+    QSV_INSTANTIATE(c64)
+    template void overlap<c128, c128>(hipStream_t, const c128 *, const c128 *, uint64_t, double *, double *);
+    template void overlap<c64, c128>(hipStream_t, const c64 *, const c128 *, uint64_t, double *, double *);
+    template void overlap<c128, c64>(hipStream_t, const c128 *, const c64 *, uint64_t, double *, double *);
+    template void overlap<c64, c64>(hipStream_t, const c64 *, const c64 *, uint64_t, double *, double *);
 } // namespace qsv

@@ -28,6 +28,10 @@ namespace qsv
     {
         double re, im;
     };
+    struct c64
+    {
+        float re, im;
+    };
 
     // 2x2 on the amplitude pair (base|offA, base|offB): dense 1-qubit gates (offA = 0,
     // offB = 1<<t) and the swap-like 2-qubit gates.  zero_mask bit 0/1: input A/B reads as
@@ -85,41 +89,43 @@ namespace qsv
     void launch_dense4(hipStream_t s, c128 *state, int n, const Subspace &sub, const Dense4Op &op);
     void launch_mono(hipStream_t s, c128 *state, int n, const Subspace &sub, const MonoOp &op);
 
-    // ---- state preparation ----
-    void launch_fill_basis(hipStream_t s, c128 *state, uint64_t count, uint64_t index); // |index>
-    void launch_fill_sequential(hipStream_t s, c128 *state, uint64_t count);
-    void launch_fill_flat(hipStream_t s, c128 *state, uint64_t count);
-    void launch_fill_random(hipStream_t s, c128 *state, uint64_t count, uint64_t seed);
-    void launch_set_one(hipStream_t s, c128 *state, uint64_t index);
+    // ---- state preparation (A = c128 or c64) ----
+    template <class A> void launch_fill_basis(hipStream_t s, A *state, uint64_t count, uint64_t index); // |index>
+    template <class A> void launch_fill_sequential(hipStream_t s, A *state, uint64_t count);
+    template <class A> void launch_fill_flat(hipStream_t s, A *state, uint64_t count);
+    template <class A> void launch_fill_random(hipStream_t s, A *state, uint64_t count, uint64_t seed);
+    template <class A> void launch_set_one(hipStream_t s, A *state, uint64_t index);
 
     // ---- reductions (FP64, canonical order of include/qcs_ext.h) ----
     // Workspace size in doubles for reduce_* on `nitems` items.
     uint64_t reduce_workspace(uint64_t nitems);
-    // {p0,p1} over the subspace `sub` (target bit `tq` not in the insertion list, see
-    // simulator.cpp); result written to host `out[2]` after a stream sync.
-    void reduce_pair_probs(hipStream_t s, const c128 *state, const Subspace &sub, uint64_t tbit, uint64_t nitems, double *work, double out[2]);
-    // canonical sum of p(i) (sign = -1 where bit `zbit` is set, if zbit != 0) over all amplitudes
-    void reduce_probs(hipStream_t s, const c128 *state, uint64_t count, uint64_t zbit, double *work, double *out);
+    // {p0,p1} over the subspace `sub` (p1 at index | tbit); host out[2], stream synchronised.
+    template <class A>
+    void reduce_pair_probs(hipStream_t s, const A *state, const Subspace &sub, uint64_t tbit, uint64_t nitems, double *work, double out[2]);
+    // canonical sum of p(i) (negated where bit `zbit` is set, if zbit != 0) over all amplitudes
+    template <class A>
+    void reduce_probs(hipStream_t s, const A *state, uint64_t count, uint64_t zbit, double *work, double *out);
+    // out[0..1] = <a|b> (canonical tree per component), out[2] = max |a_i - b_i|
+    uint64_t overlap_workspace(uint64_t count);
+    template <class A, class B>
+    void overlap(hipStream_t s, const A *a, const B *b, uint64_t count, double *work, double out[3]);
 
     // ---- measurement collapse ----
-    void launch_collapse(hipStream_t s, c128 *state, int n, int tq, int result, double scale);
+    template <class A> void launch_collapse(hipStream_t s, A *state, int n, int tq, int result, double scale);
 
     // ---- sampling ----
     // Level sizes of the 64-ary fold for 2^n entries; level 0 is the state itself.
     std::vector<uint64_t> fold_level_sizes(uint64_t count);
-    // Build the level totals into `levels` (one device buffer per level >= 1).
-    void build_fold_levels(hipStream_t s, const c128 *state, uint64_t count, const std::vector<double *> &levels);
-    void launch_sample_search(hipStream_t s, const c128 *state, uint64_t count, const std::vector<double *> &levels, double total, uint64_t seed, uint64_t nshots, const int *p2l, int n, uint64_t *out_dev);
+    template <class A>
+    void build_fold_levels(hipStream_t s, const A *state, uint64_t count, const std::vector<double *> &levels);
+    template <class A>
+    void launch_sample_search(hipStream_t s, const A *state, uint64_t count, const std::vector<double *> &levels, double total, uint64_t seed, uint64_t nshots, const int *p2l, int n, uint64_t *out_dev);
 
-    // ---- layout ----
-    // out[j] = state[phys(first + j)] for j < len, where phys maps a logical index to
-    // the physical one through l2p.
-    void launch_gather_logical(hipStream_t s, const c128 *state, c128 *out, uint64_t first, uint64_t len, const int *l2p, int n);
-    void launch_scatter_logical(hipStream_t s, c128 *state, const c128 *in, uint64_t first, uint64_t len, const int *l2p, int n);
-
-    // ---- precision conversion ----
-    void launch_cast_c128_to_c64(hipStream_t s, const c128 *in, float2 *out, uint64_t count);
-    void launch_cast_c64_to_c128(hipStream_t s, const float2 *in, c128 *out, uint64_t count);
+    // ---- layout: logical-order staging is always complex128 ----
+    template <class A>
+    void launch_gather_logical(hipStream_t s, const A *state, c128 *out, uint64_t first, uint64_t len, const int *l2p, int n);
+    template <class A>
+    void launch_scatter_logical(hipStream_t s, A *state, const c128 *in, uint64_t first, uint64_t len, const int *l2p, int n);
 
     void check(hipError_t e, const char *what);
 } // namespace qsv

@@ -124,6 +124,8 @@ class Library:
         lib.qcs_ext_permute.argtypes = [vp, P(bit_num_t), bit_num_t]
         lib.qcs_ext_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.qcs_ext_flush.argtypes = [vp]
+        lib.qcs_ext_fusion_stats.argtypes = [vp, P(u64), P(u64)]
+        lib.qcs_ext_overlap.argtypes = [vp, vp, vp]
         lib.qcs_ext_profile.argtypes = [vp, ctypes.c_int]
         lib.qcs_ext_profile_get.argtypes = [vp, ctypes.c_int, P(u64), P(ctypes.c_double), P(ctypes.c_double)]
         lib.qcs_ext_profile_reset.argtypes = [vp]
@@ -304,6 +306,17 @@ class Simulator:
 
     def flush(self):
         self._call(self.lib.qcs_ext_flush(self.h), "flush")
+
+    def fusion_stats(self):
+        g, b = ctypes.c_uint64(), ctypes.c_uint64()
+        self._call(self.lib.qcs_ext_fusion_stats(self.h, ctypes.byref(g), ctypes.byref(b)), "fusion_stats")
+        return g.value, b.value
+
+    def overlap(self, other):
+        """(<self|other>, max |self_i - other_i|); both simulators from the same library."""
+        out = np.empty(3, dtype=np.float64)
+        self._call(self.lib.qcs_ext_overlap(self.h, other.h, out.ctypes.data), "overlap")
+        return complex(out[0], out[1]), float(out[2])
 
     # --- timing -----------------------------------------------------------------------
     def event_create(self):

@@ -1,0 +1,153 @@
+"""Precision form on the GPU: complex64 state, gates fused into k-qubit blocks on FP32 MFMA.
+
+Bars: fidelity >= 1 - 1e-6 against the FP64 oracle (and against the GPU FP64 plain path at
+sizes the oracle does not reach); max|d amp| is reported and bounded by 1e-5 (FP32 level).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import pyqcs
+from pyqcs import circuits
+from conftest import random_state, fidelity
+
+pytestmark = pytest.mark.gpu
+
+FID_TOL = 1e-6
+AMP_TOL = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def fused_vs_oracle(gpu, oracle, n, circ, k, mapping=None, psi=None):
+    with pyqcs.Simulator(gpu, n, precision=1, block_qubits=k, mapping=mapping) as f:
+        if psi is not None:
+            f.set_statevector(psi)
+        f.run(circ)
+        got = f.statevector()
+        stats = f.fusion_stats()
+    with pyqcs.Simulator(oracle, n, mapping=mapping) as o:
+        if psi is not None:
+            o.set_statevector(psi)
+        o.run(circ)
+        want = o.statevector()
+    return got, want, stats
+
+
+@pytest.mark.parametrize("k", [4, 5])
+def test_random_all_gates(gpu, oracle, k):
+    for n, seed in ((6, 1), (9, 2), (13, 3)):
+        circ = circuits.random_all_gates(n, 200, seed=seed, max_ctrls=3)
+        say(f"fused random circuit n={n} k={k}")
+        got, want, stats = fused_vs_oracle(gpu, oracle, n, circ, k)
+        fid = fidelity(got, want)
+        err = np.max(np.abs(got - want))
+        say(f"  fidelity {fid:.12f} max|d| {err:.2e} gates {stats[0]} blocks {stats[1]}")
+        assert fid >= 1 - FID_TOL and err <= AMP_TOL
+        assert stats[0] == len(circ) and stats[1] <= len(circ)
+
+
+def test_each_gate_single(gpu, oracle):
+    n = 8
+    for name in sorted(pyqcs.GATE_TARGETS):
+        rng = np.random.default_rng(len(name))
+        t = pyqcs.GATE_TARGETS[name]
+        qs = [int(q) for q in rng.permutation(n)]
+        targets, ctl = qs[:t], qs[t:t + 2]
+        k = 1 if name == "global_phase" else pyqcs.GATE_PARAMS[name]
+        params = tuple(rng.uniform(-math.pi, math.pi, size=k))
+        psi = random_state(n, 5)
+        got, want, _ = fused_vs_oracle(gpu, oracle, n, [(name, targets, params, ctl[:1], ctl[1:])], 5, psi=psi)
+        assert np.max(np.abs(got - want)) <= AMP_TOL, name
+
+
+def test_wide_gates(gpu, oracle):
+    # gates with more qubits than the block size: controls handled as a subspace
+    n = 9
+    circ = [("h", (q,), (), (), ()) for q in range(n)]
+    circ += [("rcccx", (0, 1, 2, 3), (), (4, 5), (6,)),
+             ("x", (8,), (), (0, 1, 2, 3, 4), ()),
+             ("u3", (7,), (0.3, 0.2, 0.1), (1, 2, 3, 4), (5, 6)),
+             ("swap", (2, 6), (), (0, 1, 3), ())]
+    for k in (4, 5):
+        got, want, stats = fused_vs_oracle(gpu, oracle, n, circ, k)
+        assert np.max(np.abs(got - want)) <= AMP_TOL
+
+
+def test_brick_mapping_and_fusion_counts(gpu, oracle):
+    n = 14
+    circ = circuits.brick(n, 6, seed=4, mcx=2)
+    mapping = [int(q) for q in np.random.default_rng(3).permutation(n)]
+    got, want, stats = fused_vs_oracle(gpu, oracle, n, circ, 5, mapping=mapping)
+    assert fidelity(got, want) >= 1 - FID_TOL
+    # 6 layers x (14 U3 + 6-7 CZ) + 2 MCX fuse into far fewer sweeps
+    assert stats[1] * 3 <= stats[0]
+
+
+def test_measure_sample_in_fused_mode(gpu, oracle):
+    n = 10
+    circ = circuits.random_all_gates(n, 80, seed=9)
+    mism = 0
+    for seed in range(4):
+        outs = []
+        for lib, prec in ((gpu, 1), (oracle, 0)):
+            with pyqcs.Simulator(lib, n, num_clbits=n, seed=seed, precision=prec) as sim:
+                sim.run(circ[:40])
+                a = [sim.measure_to_clbit(q, q) for q in (1, 4)]
+                sim.run(circ[40:])
+                b = [sim.measure(q) for q in range(n)]
+                outs.append(a + b)
+        mism += sum(x != y for x, y in zip(*outs))
+    assert mism <= 1
+    # sampling from the fused state follows |amp|^2 (FP64 prefix over the complex64 state)
+    psi = random_state(8, 2)
+    with pyqcs.Simulator(gpu, 8, precision=1) as sim:
+        sim.set_statevector(psi)
+        assert abs(sim.probability_total() - 1) < 1e-6
+        out = sim.sample(40000, seed=3)
+    counts = np.bincount(out.astype(np.int64), minlength=256)
+    p = np.abs(psi) ** 2
+    chi2 = np.sum((counts - 40000 * p) ** 2 / (40000 * p))
+    assert chi2 < 256 + 6 * math.sqrt(512)
+
+
+def test_window_in_fused_mode(gpu, oracle):
+    n = 10
+    circ = circuits.random_all_gates(n, 100, seed=12)
+    res = []
+    for lib, prec in ((gpu, 1), (oracle, 0)):
+        with pyqcs.Simulator(lib, n, seed=2, precision=prec) as sim:
+            sim.set_local_qubits(7)
+            sim.run(circ)
+            res.append((sim.mapping(), sim.statevector()))
+    assert res[0][0] == res[1][0]
+    assert fidelity(res[0][1], res[1][1]) >= 1 - FID_TOL
+
+
+def test_overlap_with_plain_gpu_path(gpu):
+    n = 22
+    circ = circuits.brick(n, 10, seed=7, mcx=4)
+    say("overlap fused vs plain n=22")
+    with pyqcs.Simulator(gpu, n) as plain, pyqcs.Simulator(gpu, n, precision=1) as fused:
+        plain.run(circ)
+        fused.run(circ)
+        ov, md = fused.overlap(plain)
+        fid = abs(ov) ** 2 / (fused.norm() * plain.norm())
+        say(f"  fidelity {fid:.12f} max|d| {md:.2e}")
+        assert fid >= 1 - FID_TOL and md <= AMP_TOL
+        ov2, md2 = plain.overlap(plain)
+        assert abs(ov2 - plain.norm()) < 1e-12 and md2 == 0.0
+
+
+def test_precision_mode_errors(gpu):
+    sim = pyqcs.Simulator(gpu, 3, precision=1, allocate=False)
+    with pytest.raises(pyqcs.QcsError):
+        sim.allocate()  # fewer qubits than the smallest block
+    sim.close()
+    with pyqcs.Simulator(gpu, 6) as sim:
+        assert sim.lib.qcs_ext_set_precision(sim.h, 1, 5) == 0  # only before allocate_memory
