@@ -13,7 +13,12 @@ Also reported:
                    its average duration from HIP events recorded around each launch on the
                    simulator's stream during the timed steps, against 8 TB/s;
   cpu_baseline  -- the oracle (the reference's plain method restated in C++/OpenMP) on the
-                   host cores, on a bounded sample of the same workload.
+                   host cores, on a bounded sample of the same workload;
+  fused_c64     -- the precision form on the 32-qubit brick circuit of depth 32 (complex64
+                   state, gates fused into k=5 blocks on FP32 MFMA) against the plain FP64
+                   per-gate path on the same GPU: time of each, speedup, and fidelity /
+                   max |d amp| of the fused state against the FP64 state (both resident in
+                   HBM, compared on the GPU in FP64).
 """
 import argparse
 import json
@@ -75,6 +80,48 @@ def cpu_baseline(n, circ, budget_s=20.0):
     }
 
 
+def fused_leg(n, depth, k, shots):
+    """Config: 32 qubits, brick depth 32; fused complex64 vs plain complex128 on one GPU."""
+    lib = pyqcs.gpu_library()
+    circ = circuits.brick(n, depth, seed=CIRCUIT_SEED + 3)
+    warm = circuits.brick(n, 1, seed=1)[: n + 2]
+    res = {"config": f"brick{n}: {n} qubits, {depth} brick layers (U3 all + CZ even/odd), "
+                     f"{len(circ)} gates, {shots} samples",
+           "gates": len(circ), "block_qubits": k}
+    plain = pyqcs.Simulator(lib, n, seed=1)
+    plain.run(warm)
+    plain.call("set_zero_state")
+    plain.norm()
+    t0 = time.perf_counter()
+    plain.run(circ)
+    plain.probability_total()
+    t_plain = time.perf_counter() - t0
+    log(f"fused leg: plain c128 {t_plain:.3f}s")
+    fused = pyqcs.Simulator(lib, n, seed=1, precision=1, block_qubits=k)
+    fused.run(warm)
+    fused.call("set_zero_state")
+    fused.norm()
+    g0, b0 = fused.fusion_stats()
+    t0 = time.perf_counter()
+    fused.run(circ)
+    total = fused.probability_total()
+    t_fused = time.perf_counter() - t0
+    g1, b1 = fused.fusion_stats()
+    ts = time.perf_counter()
+    fused.sample(shots, seed=SAMPLE_SEED)
+    t_sample = time.perf_counter() - ts
+    ov, maxdiff = fused.overlap(plain)
+    fid = abs(ov) ** 2 / (fused.norm() * plain.norm())
+    fused.close()
+    plain.close()
+    log(f"fused leg: fused c64 {t_fused:.3f}s ({b1 - b0} blocks), fidelity {fid:.12f}, max|d| {maxdiff:.2e}")
+    res.update({"plain_c128_s": t_plain, "fused_c64_s": t_fused, "speedup": t_plain / t_fused,
+                "blocks": b1 - b0,
+                "fidelity": fid, "infidelity": 1 - fid, "max_abs_diff": maxdiff,
+                "probability_total": total, "sample_s": t_sample})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +132,10 @@ def main():
     ap.add_argument("--shots", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-fused", action="store_true")
+    ap.add_argument("--fused-qubits", type=int, default=32)
+    ap.add_argument("--fused-depth", type=int, default=32)
+    ap.add_argument("--fused-k", type=int, default=5)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one write-only fill and one read-only norm pass of the "
                          "full state (known HBM bytes, for scaling PMC byte counters)")
@@ -178,6 +229,8 @@ def main():
         "sweeps_per_s": alg_bytes / (2 * 16 * (1 << n)) * args.steps / wall,
         "hbm_frac_whole_step": alg_bytes * args.steps / wall / HBM_PEAK,
     }
+    if not args.no_fused:
+        result["fused_c64"] = fused_leg(args.fused_qubits, args.fused_depth, args.fused_k, args.shots)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(n, circ, args.cpu_budget)
     print(json.dumps(result), flush=True)

@@ -109,6 +109,24 @@ int qcs_ext_fusion_stats(qcs_simulator *sim, uint64_t *gates, uint64_t *blocks);
  * may be complex128 or complex64.  The fidelity is |<a|b>|^2 / (<a|a> <b|b>). */
 int qcs_ext_overlap(qcs_simulator *a, qcs_simulator *b, double *out);
 
+/* Batching form: many independent circuits of 1-target gates (each with at most one
+ * positive control), every circuit starting from |0...0>, outputs <Z_q> for all qubits of
+ * every circuit (FP64 reductions).  libqcs.so runs the whole batch as one hipGraph launch:
+ * circuits of up to lds_max_qubits qubits in a persistent LDS-resident kernel, larger ones
+ * as fused blocks on complex64 states in HBM.  The oracle runs the plain method per circuit. */
+typedef struct qcs_ext_batch qcs_ext_batch;
+int qcs_ext_batch_create(qcs_ext_batch **out);
+int qcs_ext_batch_destroy(qcs_ext_batch *batch);
+/* gate_names[i] is a qcs.h gate name with one target (e.g. "ry", "rz", "x", "u3");
+ * controls[i] is a control qubit or -1; params holds 4 doubles per gate (unused ones 0). */
+int qcs_ext_batch_add_circuit(qcs_ext_batch *batch, bit_num_t num_qubits, int32_t num_gates, const char *const *gate_names, const bit_num_t *targets, const bit_num_t *controls, const double *params);
+int qcs_ext_batch_set_tuning(qcs_ext_batch *batch, int lds_max_qubits, int block_size, int block_qubits, int streams);
+/* Searches the tuning parameters on this batch; writes the chosen 4 values to tuning. */
+int qcs_ext_batch_autotune(qcs_ext_batch *batch, int *tuning);
+/* expz receives sum(num_qubits) doubles, circuit by circuit; seconds = device time. */
+int qcs_ext_batch_run(qcs_ext_batch *batch, double *expz, uint64_t count, double *seconds);
+int qcs_ext_batch_info(qcs_ext_batch *batch, int *num_circuits, int *total_qubits, uint64_t *graph_nodes);
+
 /* Per-kernel-class timing with HIP events around every launch (oracle: host clock). */
 int qcs_ext_profile(qcs_simulator *sim, int enable);
 int qcs_ext_profile_get(qcs_simulator *sim, int kernel_class, uint64_t *launches, double *seconds, double *bytes);

@@ -702,6 +702,24 @@ struct qcs_simulator
     oracle::Sim *core;
 };
 
+/* Batching form in the oracle: the plain method, circuit after circuit. */
+struct qcs_ext_batch
+{
+    struct Gate
+    {
+        std::string name;
+        int target, control;
+        double params[4];
+    };
+    struct Circuit
+    {
+        int n;
+        std::vector<Gate> gates;
+    };
+    std::vector<Circuit> circuits;
+    int total_qubits = 0;
+};
+
 namespace
 {
     qcs_exception_callback g_callback = nullptr;
@@ -1319,6 +1337,85 @@ extern "C"
             put(launches, c.prof_launches[cls]);
             put(seconds, c.prof_seconds[cls]);
             put(bytes, 0.0);
+        });
+    }
+    int qcs_ext_batch_create(qcs_ext_batch **out)
+    {
+        return guard([&] {
+            if (!out)
+                throw std::runtime_error("result pointer must not be NULL");
+            *out = new qcs_ext_batch();
+        });
+    }
+    int qcs_ext_batch_destroy(qcs_ext_batch *b) { return guard([&] { delete b; }); }
+    int qcs_ext_batch_add_circuit(qcs_ext_batch *b, bit_num_t nq, int32_t ng, const char *const *names, const bit_num_t *targets, const bit_num_t *controls, const double *params)
+    {
+        return guard([&] {
+            if (!b || nq < 1 || nq > 30 || ng < 0 || (ng > 0 && (!names || !targets || !controls || !params)))
+                throw std::runtime_error("bad batch circuit");
+            qcs_ext_batch::Circuit c{nq, {}};
+            for (int i = 0; i < ng; ++i)
+            {
+                qcs_ext_batch::Gate g{names[i], targets[i], controls[i], {params[4 * i], params[4 * i + 1], params[4 * i + 2], params[4 * i + 3]}};
+                if (oracle::make_gate(g.name, g.params).ntargets != 1 || g.target < 0 || g.target >= nq || g.control >= nq || g.control == g.target)
+                    throw std::runtime_error("bad batch gate");
+                c.gates.push_back(g);
+            }
+            b->circuits.push_back(c);
+            b->total_qubits += nq;
+        });
+    }
+    int qcs_ext_batch_set_tuning(qcs_ext_batch *b, int, int, int, int) { return guard([&] { if (!b) throw std::runtime_error("batch is NULL"); }); }
+    int qcs_ext_batch_autotune(qcs_ext_batch *b, int *tuning)
+    {
+        return guard([&] {
+            if (!b || !tuning)
+                throw std::runtime_error("batch is NULL");
+            tuning[0] = tuning[1] = tuning[2] = tuning[3] = 0;
+        });
+    }
+    int qcs_ext_batch_run(qcs_ext_batch *b, double *expz, uint64_t count, double *seconds)
+    {
+        return guard([&] {
+            if (!b || (expz && count != (uint64_t)b->total_qubits))
+                throw std::runtime_error("bad batch run");
+            auto t0 = std::chrono::steady_clock::now();
+            size_t off = 0;
+            for (const auto &c : b->circuits)
+            {
+                oracle::Sim sim;
+                sim.num_qubits_cfg = c.n;
+                sim.allocate(c.n);
+                for (const auto &g : c.gates)
+                {
+                    std::vector<int> pos;
+                    if (g.control >= 0)
+                        pos.push_back(g.control);
+                    sim.operate(oracle::make_gate(g.name, g.params), {g.target}, {}, pos);
+                }
+                auto p = sim.probs_physical();
+                for (int q = 0; q < c.n; ++q)
+                {
+                    std::vector<double> v(p.size());
+                    for (size_t i = 0; i < p.size(); ++i)
+                        v[i] = ((i >> q) & 1) ? -p[i] : p[i];
+                    if (expz)
+                        expz[off + q] = oracle::canon_sum(std::move(v));
+                }
+                off += c.n;
+            }
+            if (seconds)
+                *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        });
+    }
+    int qcs_ext_batch_info(qcs_ext_batch *b, int *nc, int *tq, uint64_t *nodes)
+    {
+        return guard([&] {
+            if (!b)
+                throw std::runtime_error("batch is NULL");
+            put(nc, (int)b->circuits.size());
+            put(tq, b->total_qubits);
+            put(nodes, 0);
         });
     }
     int qcs_ext_profile_reset(qcs_simulator *s)

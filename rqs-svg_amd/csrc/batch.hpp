@@ -1,0 +1,80 @@
+// Batching form: many small variational circuits, one hipGraph launch.
+#pragma once
+#include "batch_kernels.hpp"
+#include "fused.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace qcore
+{
+    // One gate of a batched circuit: a 1-target gate with at most one positive control.
+    struct BatchGate
+    {
+        std::string name;
+        int target;
+        int control; // -1: none
+        double params[4];
+    };
+
+    struct BatchTuning
+    {
+        int lds_max_qubits = 14; // circuits up to this size run in the LDS-resident kernel
+        int block_size = 512;    // threads per workgroup of that kernel
+        int block_qubits = 5;    // fused block size for the circuits in HBM
+        int streams = 4;         // graph branches the HBM circuits are spread over
+    };
+
+    class Batch
+    {
+    public:
+        Batch();
+        ~Batch();
+        Batch(const Batch &) = delete;
+        Batch &operator=(const Batch &) = delete;
+
+        int add_circuit(int n, const std::vector<BatchGate> &gates);
+        int num_circuits() const { return (int)circuits_.size(); }
+        int total_qubits() const { return total_qubits_; }
+        void set_tuning(const BatchTuning &t);
+        const BatchTuning &tuning() const { return tuning_; }
+        // Runs every circuit from |0...0>, writes <Z_q> of circuit c at offset
+        // sum_{c' < c} n_c' + q; returns seconds between graph launch and completion.
+        double run(double *expz);
+        // Coordinate search over the tuning parameters on this batch; keeps the best.
+        BatchTuning autotune();
+        uint64_t graph_nodes() const { return graph_nodes_; }
+
+    private:
+        struct Circuit
+        {
+            int n;
+            std::vector<BatchGate> gates;
+            int out_offset;
+        };
+        void prepare();
+        void release();
+
+        std::vector<Circuit> circuits_;
+        int total_qubits_ = 0;
+        BatchTuning tuning_;
+        bool prepared_ = false;
+        int device_ = 0;
+        std::vector<hipStream_t> streams_;
+        std::vector<hipEvent_t> fork_join_;
+        hipEvent_t t0_ = nullptr, t1_ = nullptr;
+        hipGraph_t graph_ = nullptr;
+        hipGraphExec_t exec_ = nullptr;
+        uint64_t graph_nodes_ = 0;
+        // device buffers
+        qsv::BatchCircuit *d_circs_ = nullptr;
+        qsv::BatchOp *d_ops_ = nullptr;
+        double *d_out_ = nullptr;
+        unsigned *d_counter_ = nullptr;
+        qsv::c64 *d_states_ = nullptr;
+        double *d_partials_ = nullptr;
+        float *d_mats_ = nullptr;
+    };
+} // namespace qcore

@@ -9,6 +9,7 @@
 #include <qcs.h>
 #include <qcs_ext.h>
 
+#include "batch.hpp"
 #include "gates.hpp"
 #include "simulator.hpp"
 
@@ -20,6 +21,11 @@
 struct qcs_simulator
 {
     qcore::Simulator *core;
+};
+
+struct qcs_ext_batch
+{
+    qcore::Batch *core;
 };
 
 namespace
@@ -431,4 +437,91 @@ extern "C"
         return guard([&] { core(s).profile_get(cls, launches, seconds, bytes); });
     }
     int qcs_ext_profile_reset(qcs_simulator *s) { return guard([&] { core(s).profile_reset(); }); }
+
+    /* ---------------- batching form ---------------- */
+    int qcs_ext_batch_create(qcs_ext_batch **out)
+    {
+        return guard([&] {
+            if (!out)
+                throw std::runtime_error("result pointer must not be NULL");
+            *out = new qcs_ext_batch{new qcore::Batch()};
+        });
+    }
+    int qcs_ext_batch_destroy(qcs_ext_batch *b)
+    {
+        return guard([&] {
+            if (!b)
+                return;
+            delete b->core;
+            delete b;
+        });
+    }
+    int qcs_ext_batch_add_circuit(qcs_ext_batch *b, bit_num_t nq, int32_t ng, const char *const *names, const bit_num_t *targets, const bit_num_t *controls, const double *params)
+    {
+        return guard([&] {
+            if (!b || !b->core)
+                throw std::runtime_error("batch is NULL");
+            if (ng < 0 || (ng > 0 && (!names || !targets || !controls || !params)))
+                throw std::runtime_error("gate arrays must not be NULL");
+            std::vector<qcore::BatchGate> gates(ng);
+            for (int i = 0; i < ng; ++i)
+            {
+                if (!names[i])
+                    throw std::runtime_error("gate name must not be NULL");
+                gates[i].name = names[i];
+                gates[i].target = targets[i];
+                gates[i].control = controls[i];
+                for (int j = 0; j < 4; ++j)
+                    gates[i].params[j] = params[4 * i + j];
+            }
+            b->core->add_circuit(nq, gates);
+        });
+    }
+    int qcs_ext_batch_set_tuning(qcs_ext_batch *b, int lds_max_qubits, int block_size, int block_qubits, int streams)
+    {
+        return guard([&] {
+            if (!b || !b->core)
+                throw std::runtime_error("batch is NULL");
+            qcore::BatchTuning t;
+            t.lds_max_qubits = lds_max_qubits;
+            t.block_size = block_size;
+            t.block_qubits = block_qubits;
+            t.streams = streams;
+            b->core->set_tuning(t);
+        });
+    }
+    int qcs_ext_batch_autotune(qcs_ext_batch *b, int *tuning)
+    {
+        return guard([&] {
+            if (!b || !b->core || !tuning)
+                throw std::runtime_error("batch or result pointer is NULL");
+            auto t = b->core->autotune();
+            tuning[0] = t.lds_max_qubits;
+            tuning[1] = t.block_size;
+            tuning[2] = t.block_qubits;
+            tuning[3] = t.streams;
+        });
+    }
+    int qcs_ext_batch_run(qcs_ext_batch *b, double *expz, uint64_t count, double *seconds)
+    {
+        return guard([&] {
+            if (!b || !b->core)
+                throw std::runtime_error("batch is NULL");
+            if (expz && count != (uint64_t)b->core->total_qubits())
+                throw std::runtime_error("output size must equal the total number of qubits");
+            double s = b->core->run(expz);
+            if (seconds)
+                *seconds = s;
+        });
+    }
+    int qcs_ext_batch_info(qcs_ext_batch *b, int *num_circuits, int *total_qubits, uint64_t *graph_nodes)
+    {
+        return guard([&] {
+            if (!b || !b->core)
+                throw std::runtime_error("batch is NULL");
+            put(num_circuits, b->core->num_circuits());
+            put(total_qubits, b->core->total_qubits());
+            put(graph_nodes, b->core->graph_nodes());
+        });
+    }
 }

@@ -298,20 +298,23 @@ namespace qcore
         apply_blocks(gates, plan_greedy(gates, k_));
     }
 
-    void FusedEngine::apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan)
+    std::vector<PreparedBlock> prepare_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan, int kq, int n, std::vector<float> &mats)
     {
-        const int n = sim_.n_;
-        const int D = 2 << kq_;
+        const int D = 2 << kq;
         const size_t per = size_t(D) * D;
-        reserve_matrices(per * plan.size());
-        std::vector<BlockLayout> lays;
-        lays.reserve(plan.size());
-        for (size_t bi = 0; bi < plan.size(); ++bi)
+        const int K = 1 << kq;
+        std::vector<PreparedBlock> out;
+        out.reserve(plan.size());
+        for (const auto &pb : plan)
         {
-            lays.push_back(layout_block(gates, plan[bi], kq_, n));
-            auto M = block_matrix(gates, plan[bi], lays.back());
-            const int K = 1 << kq_;
-            float *R = rhost_ + bi * per;
+            const BlockLayout lay = layout_block(gates, pb, kq, n);
+            const auto M = block_matrix(gates, pb, lay);
+            PreparedBlock b;
+            b.kq = kq;
+            b.gates = (int)pb.gates.size();
+            b.matrix_offset = mats.size();
+            mats.resize(mats.size() + per);
+            float *R = mats.data() + b.matrix_offset;
             for (int r = 0; r < K; ++r)
                 for (int c = 0; c < K; ++c)
                 {
@@ -322,44 +325,96 @@ namespace qcore
                     R[(2 * r + 1) * D + 2 * c] = im;
                     R[(2 * r + 1) * D + 2 * c + 1] = re;
                 }
-        }
-        qsv::check(hipMemcpyAsync(rdev_, rhost_, per * plan.size() * sizeof(float), hipMemcpyHostToDevice, sim_.stream_), "matrix upload");
-        qsv::check(hipEventRecord(copy_done_, sim_.stream_), "hipEventRecord");
-        copy_pending_ = true;
-        for (size_t bi = 0; bi < plan.size(); ++bi)
-        {
-            const BlockLayout &lay = lays[bi];
-            qsv::BlockOp op{};
-            op.kq = kq_;
-            for (int e = 0; e < (1 << kq_); ++e)
-            {
-                uint64_t o = 0;
-                for (int j = 0; j < kq_; ++j)
-                    if ((e >> j) & 1)
-                        o |= uint64_t(1) << lay.qubits[j];
-                op.off[e] = o;
-            }
-            op.R = rdev_ + bi * per;
-            std::vector<int> ops = lay.qubits;
+            std::vector<int> outside = lay.pos_out;
+            outside.insert(outside.end(), lay.neg_out.begin(), lay.neg_out.end());
             uint64_t setb = 0;
             for (int q : lay.pos_out)
-            {
-                ops.push_back(q);
                 setb |= uint64_t(1) << q;
+            // the 5 lowest qubits outside the block and its controls become the tile's column
+            // bits when there are that many (coalesced LDS-staged kernel)
+            std::vector<int> cols;
+            for (int q = 0; q < n && cols.size() < 5; ++q)
+                if (local_pos(lay.qubits, q) < 0 && local_pos(outside, q) < 0)
+                    cols.push_back(q);
+            b.bytes = 2.0 * 8.0 * double(uint64_t(1) << (n - (int)outside.size()));
+            b.sub = qsv::Subspace{};
+            b.sub.set_bits = setb;
+            std::vector<int> ops;
+            if (cols.size() == 5)
+            {
+                b.lds = true;
+                std::vector<int> tb = lay.qubits;
+                tb.insert(tb.end(), cols.begin(), cols.end());
+                std::sort(tb.begin(), tb.end());
+                b.lop = qsv::BlockLdsOp{};
+                b.lop.kq = kq;
+                const int row = K + 2;
+                for (size_t i = 0; i < tb.size(); ++i)
+                {
+                    b.lop.gw[i] = uint64_t(1) << tb[i];
+                    int j = local_pos(lay.qubits, tb[i]);
+                    b.lop.lw[i] = j >= 0 ? (1 << j) : (row << local_pos(cols, tb[i]));
+                }
+                ops = tb;
             }
-            ops.insert(ops.end(), lay.neg_out.begin(), lay.neg_out.end());
+            else
+            {
+                b.lds = false;
+                b.op = qsv::BlockOp{};
+                b.op.kq = kq;
+                for (int e = 0; e < K; ++e)
+                {
+                    uint64_t o = 0;
+                    for (int j = 0; j < kq; ++j)
+                        if ((e >> j) & 1)
+                            o |= uint64_t(1) << lay.qubits[j];
+                    b.op.off[e] = o;
+                }
+                ops = lay.qubits;
+            }
+            ops.insert(ops.end(), outside.begin(), outside.end());
             std::sort(ops.begin(), ops.end());
-            qsv::Subspace sub{};
-            sub.set_bits = setb;
-            sub.nops = (int)ops.size();
-            for (int i = 0; i < sub.nops; ++i)
-                sub.ops[i] = ops[i];
-            const double bytes = 2.0 * 8.0 * double(uint64_t(1) << (n - sub.nops + kq_));
-            sim_.prof_begin(QCS_EXT_K_FUSED, bytes);
-            qsv::launch_block(sim_.stream_, state_, n, sub, op);
+            b.sub.nops = (int)ops.size();
+            for (int i = 0; i < b.sub.nops; ++i)
+                b.sub.ops[i] = ops[i];
+            out.push_back(b);
+        }
+        return out;
+    }
+
+    void launch_prepared(hipStream_t s, qsv::c64 *state, int n, PreparedBlock &b, const float *mats_dev)
+    {
+        if (b.lds)
+        {
+            b.lop.R = mats_dev + b.matrix_offset;
+            qsv::launch_block_lds(s, state, n, b.sub, b.lop);
+        }
+        else
+        {
+            b.op.R = mats_dev + b.matrix_offset;
+            qsv::launch_block(s, state, n, b.sub, b.op);
+        }
+    }
+
+    void FusedEngine::apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan)
+    {
+        if (plan.empty())
+            return;
+        const int n = sim_.n_;
+        std::vector<float> mats;
+        auto blocks = prepare_blocks(gates, plan, kq_, n, mats);
+        reserve_matrices(mats.size());
+        std::memcpy(rhost_, mats.data(), mats.size() * sizeof(float));
+        qsv::check(hipMemcpyAsync(rdev_, rhost_, mats.size() * sizeof(float), hipMemcpyHostToDevice, sim_.stream_), "matrix upload");
+        qsv::check(hipEventRecord(copy_done_, sim_.stream_), "hipEventRecord");
+        copy_pending_ = true;
+        for (auto &b : blocks)
+        {
+            sim_.prof_begin(QCS_EXT_K_FUSED, b.bytes);
+            launch_prepared(sim_.stream_, state_, n, b, rdev_);
             sim_.prof_end();
             blocks_applied_ += 1;
-            gates_applied_ += plan[bi].gates.size();
+            gates_applied_ += b.gates;
         }
     }
 } // namespace qcore

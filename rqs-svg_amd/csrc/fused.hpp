@@ -35,6 +35,24 @@ namespace qcore
     // basis of `lay.qubits`; gates' controls that are in lay.qubits act inside the matrix.
     std::vector<cd> block_matrix(const std::vector<ResolvedGate> &gates, const PlannedBlock &b, const BlockLayout &lay);
 
+    // A composed block ready to launch: matrix at matrix_offset (floats) of a matrix buffer.
+    struct PreparedBlock
+    {
+        int kq = 5;
+        int gates = 0;
+        size_t matrix_offset = 0;
+        double bytes = 0; // algorithmic HBM bytes of one launch
+        bool lds = true;  // LDS-staged kernel (needs 5 qubits outside the block)
+        qsv::Subspace sub{};
+        qsv::BlockLdsOp lop{};
+        qsv::BlockOp op{};
+    };
+    // Lays out and composes every planned block; appends the real (2^(kq+1))^2 matrices to
+    // `mats` (host) in launch order.
+    std::vector<PreparedBlock> prepare_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan, int kq, int n, std::vector<float> &mats);
+    // Launches one prepared block whose matrices were uploaded to mats_dev.
+    void launch_prepared(hipStream_t s, qsv::c64 *state, int n, PreparedBlock &b, const float *mats_dev);
+
     class FusedEngine
     {
     public:

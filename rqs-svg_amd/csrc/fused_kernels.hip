@@ -110,7 +110,167 @@ namespace qsv
                     }
             }
         }
+        // Orders this wave's LDS accesses against each other (lanes exchange data through LDS).
+        __device__ __forceinline__ void wave_sync()
+        {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+
+        // LDS-staged variant.  A wave's tile is the 2^KQ * 32 amplitudes whose index bits
+        // outside the "tile bits" (the block qubits plus the 5 lowest other qubits) are
+        // fixed.  Position p of the tile (bit b of p <-> tile bit TB[b], ascending) is
+        // moved in address order -- lanes take consecutive positions, so each load/store
+        // instruction covers contiguous runs -- and transposed through LDS into
+        // [column][local amplitude] rows for the MFMA operands and back.
+        template <int KQ>
+        __global__ __launch_bounds__(kThreads) void k_block_lds(c64 *__restrict__ s, uint64_t ntiles, Subspace tsub, BlockLdsOp op)
+        {
+            constexpr int D = 2 << KQ;
+            constexpr int MT = D / 32;
+            constexpr int H = D / 2;
+            constexpr int AMPS = H / 2;
+            constexpr int K = 1 << KQ;
+            constexpr int ROW = K + 2;           // complex per LDS row (padding against bank conflicts)
+            constexpr int NPOS = 32 * K;         // tile positions
+            constexpr int NJ = NPOS / 64;        // load/store instructions per lane
+            __shared__ __attribute__((aligned(16))) v2f lds[kWaves][32 * ROW];
+
+            const int lane = threadIdx.x & 63;
+            const int wave = threadIdx.x >> 6;
+            const int c = lane & 31;
+            const int h = lane >> 5;
+            v2f *tl = lds[wave];
+
+            float a[MT][H];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+            {
+                const v4f *rp = reinterpret_cast<const v4f *>(op.R + (32 * mt + c) * D + h * H);
+#pragma unroll
+                for (int q = 0; q < H / 4; ++q)
+                {
+                    v4f v = rp[q];
+                    a[mt][4 * q + 0] = v.x;
+                    a[mt][4 * q + 1] = v.y;
+                    a[mt][4 * q + 2] = v.z;
+                    a[mt][4 * q + 3] = v.w;
+                }
+            }
+            // lane part of the position -> global offset and LDS index (tile bits 0..5)
+            uint64_t g_lane = 0;
+            int l_lane = 0;
+#pragma unroll
+            for (int b = 0; b < 6; ++b)
+                if ((lane >> b) & 1)
+                {
+                    g_lane += op.gw[b];
+                    l_lane += op.lw[b];
+                }
+
+            const uint64_t nwaves = uint64_t(gridDim.x) * kWaves;
+            for (uint64_t tile = uint64_t(blockIdx.x) * kWaves + wave; tile < ntiles; tile += nwaves)
+            {
+                const uint64_t base = expand(tile, tsub) + g_lane;
+                v2f v[NJ];
+#pragma unroll
+                for (int j = 0; j < NJ; ++j)
+                {
+                    uint64_t gj = 0;
+#pragma unroll
+                    for (int b = 6; b < 6 + 4; ++b)
+                        if ((j >> (b - 6)) & 1)
+                            gj += op.gw[b];
+                    v[j] = __builtin_nontemporal_load(reinterpret_cast<const v2f *>(s + (base + gj)));
+                }
+#pragma unroll
+                for (int j = 0; j < NJ; ++j)
+                {
+                    int lj = 0;
+#pragma unroll
+                    for (int b = 6; b < 6 + 4; ++b)
+                        if ((j >> (b - 6)) & 1)
+                            lj += op.lw[b];
+                    tl[l_lane + lj] = v[j];
+                }
+                wave_sync();
+                float b_[H];
+#pragma unroll
+                for (int r = 0; r < AMPS / 2; ++r)
+                {
+                    v4f w = *reinterpret_cast<const v4f *>(&tl[c * ROW + h * AMPS + 2 * r]);
+                    b_[4 * r + 0] = w.x;
+                    b_[4 * r + 1] = w.y;
+                    b_[4 * r + 2] = w.z;
+                    b_[4 * r + 3] = w.w;
+                }
+                v16f d[MT];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    d[mt] = v16f{0.f};
+#pragma unroll
+                for (int st = 0; st < H; ++st)
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                        d[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt][st], b_[st], d[mt], 0, 0, 0);
+                wave_sync();
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                    {
+                        const int e0 = 16 * mt + 4 * q + 2 * h;
+                        *reinterpret_cast<v4f *>(&tl[c * ROW + e0]) = v4f{d[mt][4 * q + 0], d[mt][4 * q + 1], d[mt][4 * q + 2], d[mt][4 * q + 3]};
+                    }
+                wave_sync();
+#pragma unroll
+                for (int j = 0; j < NJ; ++j)
+                {
+                    uint64_t gj = 0;
+                    int lj = 0;
+#pragma unroll
+                    for (int b = 6; b < 6 + 4; ++b)
+                        if ((j >> (b - 6)) & 1)
+                        {
+                            gj += op.gw[b];
+                            lj += op.lw[b];
+                        }
+                    __builtin_nontemporal_store(tl[l_lane + lj], reinterpret_cast<v2f *>(s + (base + gj)));
+                }
+                wave_sync();
+            }
+        }
     } // namespace
+
+    void launch_block_lds(hipStream_t st, c64 *state, int n, const Subspace &tsub, const BlockLdsOp &op)
+    {
+        if (op.kq != 4 && op.kq != 5)
+            throw std::runtime_error("block: kq must be 4 or 5");
+        const int tile_bits = op.kq + 5;
+        if (tsub.nops < tile_bits || tsub.nops > n)
+            throw std::runtime_error("block: operand count out of range");
+        for (int k = 0; k < tsub.nops; ++k)
+            if (tsub.ops[k] < 0 || tsub.ops[k] >= n || (k > 0 && tsub.ops[k] <= tsub.ops[k - 1]))
+                throw std::runtime_error("block: operands must be ascending and < n");
+        for (int b = 0; b < tile_bits; ++b)
+            if (n < 64 && (op.gw[b] >> n) != 0)
+                throw std::runtime_error("block: tile bit outside the state");
+        for (int b = tile_bits; b < 10; ++b)
+            if (op.gw[b] != 0 || op.lw[b] != 0)
+                throw std::runtime_error("block: unused tile bits must have zero weight");
+        if (op.R == nullptr)
+            throw std::runtime_error("block: matrix pointer is NULL");
+        const uint64_t ntiles = uint64_t(1) << (n - tsub.nops);
+        uint64_t grid = (ntiles + kWaves - 1) / kWaves;
+        if (grid > 4096)
+            grid = 4096;
+        if (op.kq == 5)
+            hipLaunchKernelGGL(k_block_lds<5>, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+        else
+            hipLaunchKernelGGL(k_block_lds<4>, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+        check(hipGetLastError(), "block kernel");
+    }
 
     void launch_block(hipStream_t st, c64 *state, int n, const Subspace &sub, const BlockOp &op)
     {

@@ -15,6 +15,21 @@ namespace qsv
     };
 
     // Applies the block to every column of the subspace `sub` (block qubits plus any
-    // controls, ascending; set_bits = positive controls).
+    // controls, ascending; set_bits = positive controls).  Lanes read their own columns;
+    // used when fewer than 5 qubits lie outside the block.
     void launch_block(hipStream_t s, c64 *state, int n, const Subspace &sub, const BlockOp &op);
+
+    // LDS-staged form.  Tile bits TB[0..kq+4] = the block qubits plus the 5 lowest other
+    // non-control qubits, ascending.  gw[b] = 1 << TB[b] (amplitude offset of tile bit b);
+    // lw[b] = LDS index weight of tile bit b: 1 << j if TB[b] is the block's j-th qubit,
+    // ((1 << kq) + 2) << i if TB[b] is the i-th of the 5 column
+    // qubits.  tsub = tile bits plus controls (ascending), set_bits = positive controls.
+    struct BlockLdsOp
+    {
+        int kq;
+        uint64_t gw[10];
+        int lw[10];
+        const float *R;
+    };
+    void launch_block_lds(hipStream_t s, c64 *state, int n, const Subspace &tsub, const BlockLdsOp &op);
 } // namespace qsv

@@ -129,6 +129,13 @@ class Library:
         lib.qcs_ext_profile.argtypes = [vp, ctypes.c_int]
         lib.qcs_ext_profile_get.argtypes = [vp, ctypes.c_int, P(u64), P(ctypes.c_double), P(ctypes.c_double)]
         lib.qcs_ext_profile_reset.argtypes = [vp]
+        lib.qcs_ext_batch_create.argtypes = [P(vp)]
+        lib.qcs_ext_batch_destroy.argtypes = [vp]
+        lib.qcs_ext_batch_add_circuit.argtypes = [vp, bit_num_t, ctypes.c_int32, P(ctypes.c_char_p), P(bit_num_t), P(bit_num_t), P(ctypes.c_double)]
+        lib.qcs_ext_batch_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.qcs_ext_batch_autotune.argtypes = [vp, P(ctypes.c_int)]
+        lib.qcs_ext_batch_run.argtypes = [vp, vp, u64, P(ctypes.c_double)]
+        lib.qcs_ext_batch_info.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int), P(u64)]
 
     def check(self, rc, what):
         if rc != 1:
@@ -348,6 +355,68 @@ class Simulator:
             if n.value:
                 res[name] = {"launches": n.value, "seconds": s.value, "bytes": b.value}
         return res
+
+
+class Batch:
+    """Batching form: independent circuits of 1-target gates (at most one control each).
+
+    A circuit is a list of (name, target, params, control) with control -1 for none."""
+
+    def __init__(self, library: Library):
+        self.L = library
+        self.lib = library.lib
+        h = ctypes.c_void_p()
+        self.L.check(self.lib.qcs_ext_batch_create(ctypes.byref(h)), "batch_create")
+        self.h = h
+        self.sizes = []
+
+    def add(self, n, gates):
+        k = len(gates)
+        names = (ctypes.c_char_p * max(k, 1))(*[g[0].encode() for g in gates])
+        tg = (bit_num_t * max(k, 1))(*[g[1] for g in gates])
+        ct = (bit_num_t * max(k, 1))(*[g[3] for g in gates])
+        flat = []
+        for g in gates:
+            p = list(g[2]) + [0.0] * (4 - len(g[2]))
+            flat += p
+        pa = (ctypes.c_double * max(4 * k, 1))(*flat)
+        self.L.check(self.lib.qcs_ext_batch_add_circuit(self.h, n, k, names, tg, ct, pa), "batch_add_circuit")
+        self.sizes.append(n)
+
+    def set_tuning(self, lds_max_qubits=14, block_size=512, block_qubits=5, streams=4):
+        self.L.check(self.lib.qcs_ext_batch_set_tuning(self.h, lds_max_qubits, block_size, block_qubits, streams), "batch_set_tuning")
+
+    def autotune(self):
+        t = (ctypes.c_int * 4)()
+        self.L.check(self.lib.qcs_ext_batch_autotune(self.h, t), "batch_autotune")
+        return {"lds_max_qubits": t[0], "block_size": t[1], "block_qubits": t[2], "streams": t[3]}
+
+    def run(self):
+        """Returns (list of <Z> arrays per circuit, device seconds)."""
+        out = np.empty(sum(self.sizes), dtype=np.float64)
+        sec = ctypes.c_double()
+        self.L.check(self.lib.qcs_ext_batch_run(self.h, out.ctypes.data, out.size, ctypes.byref(sec)), "batch_run")
+        res, off = [], 0
+        for n in self.sizes:
+            res.append(out[off:off + n].copy())
+            off += n
+        return res, sec.value
+
+    def info(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
+        self.L.check(self.lib.qcs_ext_batch_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "batch_info")
+        return {"circuits": a.value, "total_qubits": b.value, "graph_nodes": c.value}
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value is not None:
+            self.lib.qcs_ext_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def load_statevector_file(path, n):

@@ -114,6 +114,26 @@ def variational(n: int, depth: int, seed: int):
     return circ
 
 
+def variational_batch(count: int, seed: int, nmin: int = 12, nmax: int = 22, dmin: int = 4, dmax: int = 16):
+    """Batch workload: `count` variational circuits, qubits ~ U{nmin..nmax}, depth ~
+    U{dmin..dmax}; each gate as (name, target, params, control) for pyqcs.Batch."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        n = nmin + rng.randbelow(nmax - nmin + 1)
+        depth = dmin + rng.randbelow(dmax - dmin + 1)
+        gates = []
+        for name, t, prm, c, _ng in variational(n, depth, rng.next_u64()):
+            gates.append((name, t[0], prm, c[0] if c else -1))
+        out.append((n, gates))
+    return out
+
+
+def batch_to_circuit(gates):
+    """(name, target, params, control) gates -> qcs circuit tuples."""
+    return [(g[0], (g[1],), tuple(g[2]), (g[3],) if g[3] >= 0 else (), ()) for g in gates]
+
+
 def qft(n: int):
     circ = []
     for j in range(n - 1, -1, -1):

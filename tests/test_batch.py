@@ -1,0 +1,76 @@
+"""Batching form: many circuits, <Z_q> outputs; oracle = plain method per circuit."""
+import math
+
+import numpy as np
+import pytest
+
+import pyqcs
+from pyqcs import circuits
+import textbook
+
+
+def expz_textbook(n, gates):
+    psi = textbook.run(n, circuits.batch_to_circuit(gates))
+    p = np.abs(psi) ** 2
+    idx = np.arange(1 << n)
+    return np.array([np.sum(np.where((idx >> q) & 1, -p, p)) for q in range(n)])
+
+
+def test_oracle_batch_matches_textbook(oracle):
+    work = circuits.variational_batch(6, seed=3, nmin=3, nmax=7, dmin=1, dmax=3)
+    work.append((4, [("h", 0, (), -1), ("x", 1, (), 0), ("u3", 3, (0.3, 0.2, 0.1), 2), ("rx", 2, (1.1,), -1)]))
+    b = pyqcs.Batch(oracle)
+    for n, g in work:
+        b.add(n, g)
+    res, _ = b.run()
+    for (n, g), z in zip(work, res):
+        assert np.max(np.abs(z - expz_textbook(n, g))) < 1e-13
+    assert b.info()["circuits"] == len(work)
+
+
+def test_batch_rejects_bad_input(oracle):
+    b = pyqcs.Batch(oracle)
+    with pytest.raises(pyqcs.QcsError):
+        b.add(3, [("swap", 0, (), -1)])  # two-target gate
+    with pytest.raises(pyqcs.QcsError):
+        b.add(3, [("h", 3, (), -1)])  # target out of range
+    with pytest.raises(pyqcs.QcsError):
+        b.add(3, [("x", 1, (), 1)])  # control == target
+
+
+@pytest.mark.gpu
+def test_gpu_batch_vs_oracle(gpu, oracle):
+    work = circuits.variational_batch(40, seed=5, nmin=4, nmax=17, dmin=2, dmax=6)
+    work.append((9, [("h", q, (), -1) for q in range(9)] + [("z", 4, (), 3), ("s", 2, (), -1), ("t", 7, (), 6)]))
+    ref = pyqcs.Batch(oracle)
+    for n, g in work:
+        ref.add(n, g)
+    want, _ = ref.run()
+    for tuning in [(14, 512, 5, 4), (12, 256, 4, 1), (13, 1024, 5, 2), (0, 256, 5, 4)]:
+        b = pyqcs.Batch(gpu)
+        for n, g in work:
+            b.add(n, g)
+        b.set_tuning(*tuning)
+        got, secs = b.run()
+        err = max(np.max(np.abs(x - y)) for x, y in zip(got, want))
+        print(f"tuning {tuning}: max |d<Z>| {err:.2e} device {secs * 1e3:.2f} ms nodes {b.info()['graph_nodes']}", flush=True)
+        assert err < 2e-5, tuning
+        # graph replays give identical bits
+        for _ in range(2):
+            again, _ = b.run()
+            assert all(np.array_equal(x, y) for x, y in zip(got, again))
+        b.close()
+
+
+@pytest.mark.gpu
+def test_gpu_batch_autotune(gpu):
+    work = circuits.variational_batch(24, seed=8, nmin=10, nmax=18, dmin=2, dmax=5)
+    b = pyqcs.Batch(gpu)
+    for n, g in work:
+        b.add(n, g)
+    before, _ = b.run()
+    t = b.autotune()
+    assert t["lds_max_qubits"] in (12, 13, 14) and t["block_size"] in (256, 512, 1024)
+    assert t["block_qubits"] in (4, 5) and t["streams"] in (1, 2, 4)
+    after, _ = b.run()
+    assert max(np.max(np.abs(x - y)) for x, y in zip(before, after)) < 2e-5
