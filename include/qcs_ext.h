@@ -93,9 +93,10 @@ int qcs_ext_get_mapping(qcs_simulator *sim, bit_num_t *perm_p2l, bit_num_t count
 int qcs_ext_permute(qcs_simulator *sim, const bit_num_t *new_perm_p2l, bit_num_t count);
 
 /* Precision form.  mode 0: complex128, one kernel per gate (default).  mode 1: complex64
- * state, gates queued and fused into blocks of at most max_block_qubits qubits applied as
- * dense products (FP32 accumulation); FP64 only in reductions and renormalisation.  Must
- * be called before allocate_memory. */
+ * state, gates queued and fused into blocks of at most max_block_qubits (1..5) qubits
+ * applied as dense products on FP32 MFMA (FP32 accumulation); FP64 only in reductions and
+ * renormalisation.  mode 2: complex128 state, gates fused into blocks of at most 4 qubits
+ * applied in FP64.  Must be called before allocate_memory. */
 int qcs_ext_set_precision(qcs_simulator *sim, int mode, int max_block_qubits);
 
 /* Apply all queued gates now (no-op in mode 0). */
@@ -126,6 +127,35 @@ int qcs_ext_batch_autotune(qcs_ext_batch *batch, int *tuning);
 /* expz receives sum(num_qubits) doubles, circuit by circuit; seconds = device time. */
 int qcs_ext_batch_run(qcs_ext_batch *batch, double *expz, uint64_t count, double *seconds);
 int qcs_ext_batch_info(qcs_ext_batch *batch, int *num_circuits, int *total_qubits, uint64_t *graph_nodes);
+
+/* Learned fusion planner.  A cost model is a 3-layer MLP, 64 units wide (F -> 64 -> 64 ->
+ * 1, ReLU, FP32) with parameters W1[64][F], b1[64], W2[64][64], b2[64], w3[64], b3.
+ * libqcs.so trains and evaluates it with its own GPU kernels; the oracle library computes
+ * the same functions on the CPU.  With a model attached (precision mode 1 or 2), every
+ * flush predicts the time of candidate fusion plans (features per block, see
+ * qcs_ext_planner_record) and takes the cheapest unless it does not beat the greedy plan
+ * by 3 %; without a model the greedy plan is used.  The caller keeps the model alive while
+ * it is attached. */
+typedef struct qcs_ext_mlp qcs_ext_mlp;
+int qcs_ext_mlp_create(qcs_ext_mlp **out, int num_features, uint64_t seed);
+int qcs_ext_mlp_destroy(qcs_ext_mlp *model);
+int qcs_ext_mlp_num_params(qcs_ext_mlp *model, int *count);
+int qcs_ext_mlp_get_params(qcs_ext_mlp *model, float *params, int count);
+int qcs_ext_mlp_set_params(qcs_ext_mlp *model, const float *params, int count);
+int qcs_ext_mlp_predict(qcs_ext_mlp *model, const float *x, int rows, float *y);
+/* Gradient of the mean squared error (1/rows) sum (y - t)^2 with respect to every parameter. */
+int qcs_ext_mlp_gradients(qcs_ext_mlp *model, const float *x, const float *t, int rows, float *grad, float *loss);
+/* Adam (0.9, 0.999, 1e-8) on shuffled mini-batches; loss = final mean squared error. */
+int qcs_ext_mlp_train(qcs_ext_mlp *model, const float *x, const float *t, int rows, int epochs, int batch, float lr, uint64_t seed, float *loss);
+/* Number of features per block (16). */
+int qcs_ext_planner_num_features(int *count);
+/* Times `count` random candidate blocks on this simulator's state (precision mode 1 or 2):
+ * features (count x 16 floats) and log2 of the kernel time in microseconds. */
+int qcs_ext_planner_record(qcs_simulator *sim, int count, uint64_t seed, float *features, float *log2_us);
+int qcs_ext_set_planner(qcs_simulator *sim, qcs_ext_mlp *model);
+/* out[0..4]: flushes planned, flushes that took the learned choice, predicted greedy
+ * microseconds, predicted microseconds of the executed plans, blocks applied. */
+int qcs_ext_planner_stats(qcs_simulator *sim, double *out);
 
 /* Per-kernel-class timing with HIP events around every launch (oracle: host clock). */
 int qcs_ext_profile(qcs_simulator *sim, int enable);

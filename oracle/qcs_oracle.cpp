@@ -702,6 +702,80 @@ struct qcs_simulator
     oracle::Sim *core;
 };
 
+/* Cost-model MLP on the CPU (F -> 64 -> 64 -> 1, ReLU, FP32): the same functions the GPU
+ * library computes, written as plain loops; used to check the GPU forward and gradients. */
+struct qcs_ext_mlp
+{
+    static constexpr int H = 64;
+    int F;
+    std::vector<float> p, m, v;
+    int step = 0;
+    int w1() const { return 0; }
+    int b1() const { return H * F; }
+    int w2() const { return H * F + H; }
+    int b2() const { return H * F + H + H * H; }
+    int w3() const { return H * F + 2 * H + H * H; }
+    int b3() const { return H * F + 3 * H + H * H; }
+    int P() const { return b3() + 1; }
+    float forward(const float *x, float *h1, float *h2) const
+    {
+        for (int i = 0; i < H; ++i)
+        {
+            float a = p[b1() + i];
+            for (int k = 0; k < F; ++k)
+                a = std::fma(p[w1() + i * F + k], x[k], a);
+            h1[i] = a > 0.f ? a : 0.f;
+        }
+        for (int i = 0; i < H; ++i)
+        {
+            float a = p[b2() + i];
+            for (int j = 0; j < H; ++j)
+                a = std::fma(p[w2() + i * H + j], h1[j], a);
+            h2[i] = a > 0.f ? a : 0.f;
+        }
+        float y = p[b3()];
+        for (int j = 0; j < H; ++j)
+            y = std::fma(p[w3() + j], h2[j], y);
+        return y;
+    }
+    float gradients(const float *X, const float *T, int B, float *G) const
+    {
+        std::vector<float> g(P(), 0.f);
+        double loss = 0;
+        for (int b = 0; b < B; ++b)
+        {
+            float h1[H], h2[H];
+            const float y = forward(X + size_t(b) * F, h1, h2);
+            loss += double(y - T[b]) * double(y - T[b]);
+            const float dy = 2.f * (y - T[b]) / float(B);
+            float d2[H], d1[H];
+            for (int i = 0; i < H; ++i)
+                d2[i] = h2[i] > 0.f ? dy * p[w3() + i] : 0.f;
+            for (int j = 0; j < H; ++j)
+            {
+                float a = 0.f;
+                for (int i = 0; i < H; ++i)
+                    a = std::fma(p[w2() + i * H + j], d2[i], a);
+                d1[j] = h1[j] > 0.f ? a : 0.f;
+            }
+            for (int i = 0; i < H; ++i)
+            {
+                for (int k = 0; k < F; ++k)
+                    g[w1() + i * F + k] = std::fma(d1[i], X[size_t(b) * F + k], g[w1() + i * F + k]);
+                g[b1() + i] += d1[i];
+                for (int j = 0; j < H; ++j)
+                    g[w2() + i * H + j] = std::fma(d2[i], h1[j], g[w2() + i * H + j]);
+                g[b2() + i] += d2[i];
+                g[w3() + i] = std::fma(dy, h2[i], g[w3() + i]);
+            }
+            g[b3()] += dy;
+        }
+        if (G)
+            std::copy(g.begin(), g.end(), G);
+        return float(loss / B);
+    }
+};
+
 /* Batching form in the oracle: the plain method, circuit after circuit. */
 struct qcs_ext_batch
 {
@@ -1416,6 +1490,133 @@ extern "C"
             put(nc, (int)b->circuits.size());
             put(tq, b->total_qubits);
             put(nodes, 0);
+        });
+    }
+    int qcs_ext_mlp_create(qcs_ext_mlp **out, int F, uint64_t seed)
+    {
+        return guard([&] {
+            if (!out || F < 1 || F > 32)
+                throw std::runtime_error("bad model arguments");
+            auto *m = new qcs_ext_mlp();
+            m->F = F;
+            m->p.assign(m->P(), 0.f);
+            auto fill = [&](int off, int count, int fan_in) {
+                const double a = std::sqrt(6.0 / fan_in);
+                for (int i = 0; i < count; ++i)
+                {
+                    uint64_t z = (seed += UINT64_C(0x9E3779B97F4A7C15));
+                    double u = double(oracle::splitmix64(z) >> 11) * 0x1.0p-53;
+                    m->p[off + i] = float((2 * u - 1) * a);
+                }
+            };
+            fill(m->w1(), 64 * F, F);
+            fill(m->w2(), 64 * 64, 64);
+            fill(m->w3(), 64, 64);
+            m->m.assign(m->P(), 0.f);
+            m->v.assign(m->P(), 0.f);
+            *out = m;
+        });
+    }
+    int qcs_ext_mlp_destroy(qcs_ext_mlp *m) { return guard([&] { delete m; }); }
+    int qcs_ext_mlp_num_params(qcs_ext_mlp *m, int *count)
+    {
+        return guard([&] {
+            if (!m)
+                throw std::runtime_error("model is NULL");
+            put(count, m->P());
+        });
+    }
+    int qcs_ext_mlp_get_params(qcs_ext_mlp *m, float *p, int count)
+    {
+        return guard([&] {
+            if (!m || !p || count != m->P())
+                throw std::runtime_error("parameter buffer size mismatch");
+            std::copy(m->p.begin(), m->p.end(), p);
+        });
+    }
+    int qcs_ext_mlp_set_params(qcs_ext_mlp *m, const float *p, int count)
+    {
+        return guard([&] {
+            if (!m || !p || count != m->P())
+                throw std::runtime_error("parameter buffer size mismatch");
+            m->p.assign(p, p + count);
+            m->step = 0;
+        });
+    }
+    int qcs_ext_mlp_predict(qcs_ext_mlp *m, const float *x, int rows, float *y)
+    {
+        return guard([&] {
+            if (!m || rows < 0 || (rows > 0 && (!x || !y)))
+                throw std::runtime_error("bad prediction buffers");
+            float h1[64], h2[64];
+            for (int b = 0; b < rows; ++b)
+                y[b] = m->forward(x + size_t(b) * m->F, h1, h2);
+        });
+    }
+    int qcs_ext_mlp_gradients(qcs_ext_mlp *m, const float *x, const float *t, int rows, float *grad, float *loss)
+    {
+        return guard([&] {
+            if (!m || rows < 1 || !x || !t)
+                throw std::runtime_error("bad gradient buffers");
+            float l = m->gradients(x, t, rows, grad);
+            if (loss)
+                *loss = l;
+        });
+    }
+    int qcs_ext_mlp_train(qcs_ext_mlp *m, const float *x, const float *t, int rows, int epochs, int batch, float lr, uint64_t seed, float *loss)
+    {
+        return guard([&] {
+            if (!m || rows < 1 || !x || !t || batch < 1 || epochs < 0)
+                throw std::runtime_error("bad training arguments");
+            batch = std::min(batch, rows);
+            std::vector<int> idx(rows);
+            for (int i = 0; i < rows; ++i)
+                idx[i] = i;
+            std::vector<float> xs(size_t(batch) * m->F), ts(batch), g(m->P());
+            for (int e = 0; e < epochs; ++e)
+            {
+                for (int i = rows - 1; i > 0; --i)
+                {
+                    uint64_t z = (seed += UINT64_C(0x9E3779B97F4A7C15));
+                    std::swap(idx[i], idx[oracle::splitmix64(z) % uint64_t(i + 1)]);
+                }
+                for (int b0 = 0; b0 + batch <= rows; b0 += batch)
+                {
+                    for (int i = 0; i < batch; ++i)
+                    {
+                        std::copy(x + size_t(idx[b0 + i]) * m->F, x + size_t(idx[b0 + i] + 1) * m->F, xs.begin() + size_t(i) * m->F);
+                        ts[i] = t[idx[b0 + i]];
+                    }
+                    m->gradients(xs.data(), ts.data(), batch, g.data());
+                    ++m->step;
+                    const float c1 = 1.f - std::pow(0.9f, float(m->step)), c2 = 1.f - std::pow(0.999f, float(m->step));
+                    for (int i = 0; i < m->P(); ++i)
+                    {
+                        m->m[i] = 0.9f * m->m[i] + 0.1f * g[i];
+                        m->v[i] = 0.999f * m->v[i] + 0.001f * g[i] * g[i];
+                        m->p[i] -= lr * (m->m[i] / c1) / (std::sqrt(m->v[i] / c2) + 1e-8f);
+                    }
+                }
+            }
+            float l = m->gradients(x, t, rows, nullptr);
+            if (loss)
+                *loss = l;
+        });
+    }
+    int qcs_ext_planner_num_features(int *count) { return guard([&] { put(count, 16); }); }
+    int qcs_ext_planner_record(qcs_simulator *, int, uint64_t, float *, float *)
+    {
+        return guard([&] { throw std::runtime_error("the oracle has no kernels to time"); });
+    }
+    int qcs_ext_set_planner(qcs_simulator *s, qcs_ext_mlp *) { return guard([&] { (void)core(s); }); }
+    int qcs_ext_planner_stats(qcs_simulator *s, double *out)
+    {
+        return guard([&] {
+            (void)core(s);
+            if (!out)
+                throw std::runtime_error("result pointer must not be NULL");
+            for (int i = 0; i < 5; ++i)
+                out[i] = 0;
         });
     }
     int qcs_ext_profile_reset(qcs_simulator *s)

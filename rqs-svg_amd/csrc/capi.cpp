@@ -11,6 +11,7 @@
 
 #include "batch.hpp"
 #include "gates.hpp"
+#include "planner.hpp"
 #include "simulator.hpp"
 
 #include <cstring>
@@ -26,6 +27,11 @@ struct qcs_simulator
 struct qcs_ext_batch
 {
     qcore::Batch *core;
+};
+
+struct qcs_ext_mlp
+{
+    qcore::CostModel *core;
 };
 
 namespace
@@ -437,6 +443,97 @@ extern "C"
         return guard([&] { core(s).profile_get(cls, launches, seconds, bytes); });
     }
     int qcs_ext_profile_reset(qcs_simulator *s) { return guard([&] { core(s).profile_reset(); }); }
+
+    /* ---------------- learned planner ---------------- */
+    int qcs_ext_mlp_create(qcs_ext_mlp **out, int F, uint64_t seed)
+    {
+        return guard([&] {
+            if (!out)
+                throw std::runtime_error("result pointer must not be NULL");
+            *out = new qcs_ext_mlp{new qcore::CostModel(F, seed)};
+        });
+    }
+    int qcs_ext_mlp_destroy(qcs_ext_mlp *m)
+    {
+        return guard([&] {
+            if (!m)
+                return;
+            delete m->core;
+            delete m;
+        });
+    }
+    static qcore::CostModel &model(qcs_ext_mlp *m)
+    {
+        if (!m || !m->core)
+            throw std::runtime_error("model is NULL");
+        return *m->core;
+    }
+    int qcs_ext_mlp_num_params(qcs_ext_mlp *m, int *count) { return guard([&] { put(count, model(m).num_params()); }); }
+    int qcs_ext_mlp_get_params(qcs_ext_mlp *m, float *p, int count)
+    {
+        return guard([&] {
+            if (!p || count != model(m).num_params())
+                throw std::runtime_error("parameter buffer size mismatch");
+            model(m).get_params(p);
+        });
+    }
+    int qcs_ext_mlp_set_params(qcs_ext_mlp *m, const float *p, int count)
+    {
+        return guard([&] {
+            if (!p || count != model(m).num_params())
+                throw std::runtime_error("parameter buffer size mismatch");
+            model(m).set_params(p);
+        });
+    }
+    int qcs_ext_mlp_predict(qcs_ext_mlp *m, const float *x, int rows, float *y)
+    {
+        return guard([&] {
+            if (rows < 0 || (rows > 0 && (!x || !y)))
+                throw std::runtime_error("bad prediction buffers");
+            model(m).predict(x, rows, y);
+        });
+    }
+    int qcs_ext_mlp_gradients(qcs_ext_mlp *m, const float *x, const float *t, int rows, float *grad, float *loss)
+    {
+        return guard([&] {
+            if (rows < 1 || !x || !t)
+                throw std::runtime_error("bad gradient buffers");
+            float l = model(m).gradients(x, t, rows, grad);
+            if (loss)
+                *loss = l;
+        });
+    }
+    int qcs_ext_mlp_train(qcs_ext_mlp *m, const float *x, const float *t, int rows, int epochs, int batch, float lr, uint64_t seed, float *loss)
+    {
+        return guard([&] {
+            if (rows < 1 || !x || !t)
+                throw std::runtime_error("bad training buffers");
+            float l = model(m).train(x, t, rows, epochs, batch, lr, seed);
+            if (loss)
+                *loss = l;
+        });
+    }
+    int qcs_ext_planner_num_features(int *count) { return guard([&] { put(count, qcore::kPlanFeatures); }); }
+    int qcs_ext_planner_record(qcs_simulator *s, int count, uint64_t seed, float *features, float *log2_us)
+    {
+        return guard([&] { core(s).planner_record(count, seed, features, log2_us); });
+    }
+    int qcs_ext_set_planner(qcs_simulator *s, qcs_ext_mlp *m)
+    {
+        return guard([&] {
+            if (m && m->core && m->core->num_features() != qcore::kPlanFeatures)
+                throw std::runtime_error("the planner model must take 16 features");
+            core(s).set_planner_model(m ? m->core : nullptr);
+        });
+    }
+    int qcs_ext_planner_stats(qcs_simulator *s, double *out)
+    {
+        return guard([&] {
+            if (!out)
+                throw std::runtime_error("result pointer must not be NULL");
+            core(s).planner_stats(out);
+        });
+    }
 
     /* ---------------- batching form ---------------- */
     int qcs_ext_batch_create(qcs_ext_batch **out)

@@ -3,10 +3,12 @@
 // shared support kernels, for every reduction, prefix sum and renormalisation; the state
 // itself and the block products are FP32.
 #include "fused.hpp"
+#include "planner.hpp"
 
 #include <qcs_ext.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 
@@ -38,11 +40,15 @@ namespace qcore
         }
     } // namespace
 
-    std::vector<PlannedBlock> plan_greedy(const std::vector<ResolvedGate> &gates, int k)
+    // Dependency-aware greedy fusion.  A gate may join any block at or after the last
+    // block touching one of its qubits (blocks in between touch none of them, so the gate
+    // commutes past them); among the blocks within `window` of that frontier that stay
+    // within k qubits, rule 0 takes the one that grows least, rule 1 the fullest one,
+    // rule 2 the earliest one; with none, the gate opens a new block.
+    std::vector<PlannedBlock> plan_rule(const std::vector<ResolvedGate> &gates, int k, int window, int rule)
     {
         std::vector<PlannedBlock> blocks;
         std::vector<int> last(qsv::kMaxQubits, -1);
-        const int kWindow = 64; // blocks examined after the dependency frontier
         for (int gi = 0; gi < (int)gates.size(); ++gi)
         {
             auto Q = gate_qubits(gates[gi]);
@@ -54,7 +60,7 @@ namespace qcore
             {
                 int best_cost = 1 << 30;
                 int start = std::max(lo, 0);
-                int stop = std::min((int)blocks.size(), start + kWindow);
+                int stop = std::min((int)blocks.size(), start + window);
                 for (int j = start; j < stop; ++j)
                 {
                     if (blocks[j].wide)
@@ -62,7 +68,7 @@ namespace qcore
                     int u = union_size(blocks[j].qubits, Q);
                     if (u > k)
                         continue;
-                    int cost = u - (int)blocks[j].qubits.size();
+                    int cost = rule == 0 ? u - (int)blocks[j].qubits.size() : rule == 1 ? -u : j;
                     if (cost < best_cost)
                     {
                         best_cost = cost;
@@ -86,6 +92,11 @@ namespace qcore
                 last[q] = chosen;
         }
         return blocks;
+    }
+
+    std::vector<PlannedBlock> plan_greedy(const std::vector<ResolvedGate> &gates, int k)
+    {
+        return plan_rule(gates, k, 64, 0);
     }
 
     BlockLayout layout_block(const std::vector<ResolvedGate> &gates, const PlannedBlock &b, int kq, int n)
@@ -207,15 +218,18 @@ namespace qcore
         return M;
     }
 
-    FusedEngine::FusedEngine(Simulator &sim, int k) : sim_(sim), k_(k)
+    FusedEngine::FusedEngine(Simulator &sim, int k, bool f64) : sim_(sim), k_(k), f64_(f64)
     {
         const int n = sim_.n_;
         if (n < 4)
             throw std::runtime_error("precision mode needs at least 4 qubits");
-        kq_ = (k <= 4 || n < 5) ? 4 : 5;
+        kq_ = (f64 || k <= 4 || n < 5) ? 4 : 5;
         if (k_ > kq_)
             k_ = kq_;
-        qsv::check(hipMalloc((void **)&state_, (uint64_t(1) << n) * sizeof(qsv::c64)), "hipMalloc c64 state");
+        if (f64_)
+            qsv::check(hipMalloc((void **)&state128_, (uint64_t(1) << n) * sizeof(qsv::c128)), "hipMalloc c128 state");
+        else
+            qsv::check(hipMalloc((void **)&state_, (uint64_t(1) << n) * sizeof(qsv::c64)), "hipMalloc c64 state");
         qsv::check(hipEventCreateWithFlags(&copy_done_, hipEventDisableTiming), "hipEventCreate");
     }
 
@@ -224,6 +238,8 @@ namespace qcore
         (void)hipStreamSynchronize(sim_.stream_);
         if (state_)
             (void)hipFree(state_);
+        if (state128_)
+            (void)hipFree(state128_);
         if (rdev_)
             (void)hipFree(rdev_);
         if (rhost_)
@@ -232,14 +248,14 @@ namespace qcore
             (void)hipEventDestroy(copy_done_);
     }
 
-    void FusedEngine::reserve_matrices(size_t floats)
+    void FusedEngine::reserve_matrices(size_t bytes)
     {
         if (copy_pending_)
         {
             qsv::check(hipEventSynchronize(copy_done_), "hipEventSynchronize");
             copy_pending_ = false;
         }
-        if (floats <= rcap_)
+        if (bytes <= rcap_)
             return;
         // the device buffer may still be read by queued kernels: wait before freeing it
         qsv::check(hipStreamSynchronize(sim_.stream_), "hipStreamSynchronize");
@@ -249,9 +265,9 @@ namespace qcore
             (void)hipHostFree(rhost_);
         rdev_ = nullptr;
         rhost_ = nullptr;
-        size_t cap = std::max<size_t>(floats, 64 * 64 * 64);
-        qsv::check(hipMalloc((void **)&rdev_, cap * sizeof(float)), "hipMalloc block matrices");
-        qsv::check(hipHostMalloc((void **)&rhost_, cap * sizeof(float), hipHostMallocDefault), "hipHostMalloc block matrices");
+        size_t cap = std::max<size_t>(bytes, size_t(1) << 20);
+        qsv::check(hipMalloc(&rdev_, cap), "hipMalloc block matrices");
+        qsv::check(hipHostMalloc(&rhost_, cap, hipHostMallocDefault), "hipHostMalloc block matrices");
         rcap_ = cap;
     }
 
@@ -264,7 +280,12 @@ namespace qcore
             flush();
             for (int j = 0; j < g.spec.ntargets; ++j)
                 if ((g.mmask >> j) & 1)
-                    qsv::launch_collapse(sim_.stream_, state_, sim_.n_, g.tphys[j], (g.mval >> j) & 1, 1.0);
+                {
+                    if (f64_)
+                        qsv::launch_collapse(sim_.stream_, state128_, sim_.n_, g.tphys[j], (g.mval >> j) & 1, 1.0);
+                    else
+                        qsv::launch_collapse(sim_.stream_, state_, sim_.n_, g.tphys[j], (g.mval >> j) & 1, 1.0);
+                }
             ResolvedGate h = g;
             h.mmask = h.mval = 0;
             queue_.push_back(h);
@@ -284,7 +305,171 @@ namespace qcore
         std::vector<ResolvedGate> gates;
         gates.swap(queue_);
         auto plan = plan_greedy(gates, k_);
+        if (model_)
+        {
+            // predicted cost of each candidate plan = sum over its blocks of 2^prediction
+            const int n = sim_.n_;
+            auto opts = candidate_options(k_);
+            std::vector<std::vector<PlannedBlock>> plans;
+            plans.push_back(plan);
+            for (const auto &o : opts)
+                if (o.k <= k_)
+                    plans.push_back(plan_with(gates, o));
+            std::vector<float> X;
+            std::vector<size_t> first(plans.size() + 1, 0);
+            for (size_t pi = 0; pi < plans.size(); ++pi)
+            {
+                first[pi] = X.size() / kPlanFeatures;
+                for (const auto &b : plans[pi])
+                {
+                    float f[kPlanFeatures];
+                    block_features(n, f64_, kq_, layout_block(gates, b, kq_, n), (int)b.gates.size(), f);
+                    X.insert(X.end(), f, f + kPlanFeatures);
+                }
+            }
+            first[plans.size()] = X.size() / kPlanFeatures;
+            std::vector<float> Y(first.back());
+            model_->predict(X.data(), (int)Y.size(), Y.data());
+            std::vector<double> cost(plans.size(), 0.0);
+            for (size_t pi = 0; pi < plans.size(); ++pi)
+                for (size_t r = first[pi]; r < first[pi + 1]; ++r)
+                    cost[pi] += std::exp2(double(Y[r]));
+            size_t best = 0;
+            for (size_t pi = 1; pi < plans.size(); ++pi)
+                if (std::isfinite(cost[pi]) && cost[pi] < cost[best])
+                    best = pi;
+            stat_flushes_ += 1;
+            stat_pred_greedy_ += cost[0];
+            // the greedy plan judges the prediction: keep it unless clearly beaten
+            if (best != 0 && std::isfinite(cost[0]) && cost[best] < 0.97 * cost[0])
+            {
+                plan = plans[best];
+                stat_learned_ += 1;
+                stat_pred_chosen_ += cost[best];
+            }
+            else
+            {
+                stat_pred_chosen_ += cost[0];
+            }
+        }
         apply_blocks(gates, plan);
+    }
+
+    void FusedEngine::planner_stats(double out[5]) const
+    {
+        out[0] = stat_flushes_;
+        out[1] = stat_learned_;
+        out[2] = stat_pred_greedy_;
+        out[3] = stat_pred_chosen_;
+        out[4] = double(blocks_applied_);
+    }
+
+    void FusedEngine::record(int count, uint64_t seed, float *features, float *log2_us)
+    {
+        flush();
+        const int n = sim_.n_;
+        auto rnd = [&seed]() {
+            uint64_t z = (seed += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        auto unif = [&]() { return double(rnd() >> 11) * 0x1.0p-53; };
+        const int chunk = 128;
+        std::vector<hipEvent_t> ev(2 * chunk);
+        for (auto &e : ev)
+            qsv::check(hipEventCreate(&e), "hipEventCreate");
+        try
+        {
+            for (int start = -3; start < count; start += chunk)
+            {
+                const int m = start < 0 ? 3 : std::min(chunk, count - start); // 3 warm-up launches first
+                std::vector<std::vector<ResolvedGate>> gate_sets(m);
+                std::vector<PlannedBlock> blocks(m);
+                std::vector<float> mats;
+                std::vector<double> mats64;
+                std::vector<PreparedBlock> pb;
+                std::vector<PreparedBlock128> pb128;
+                for (int i = 0; i < m; ++i)
+                {
+                    const int kb = 1 + int(rnd() % uint64_t(k_));
+                    std::vector<int> qs;
+                    while ((int)qs.size() < kb)
+                    {
+                        int q = int(rnd() % uint64_t(n));
+                        if (std::find(qs.begin(), qs.end(), q) == qs.end())
+                            qs.push_back(q);
+                    }
+                    const int ng = 1 + int(rnd() % 24);
+                    for (int g = 0; g < ng; ++g)
+                    {
+                        ResolvedGate r;
+                        if (kb >= 2 && unif() < 0.3)
+                        {
+                            double none[1] = {0.0};
+                            r.spec = make_gate(unif() < 0.5 ? "z" : "x", none);
+                            int a = qs[rnd() % qs.size()], b = a;
+                            while (b == a)
+                                b = qs[rnd() % qs.size()];
+                            r.tphys = {a};
+                            r.pphys = {b};
+                        }
+                        else
+                        {
+                            double prm[3] = {6.283 * unif(), 6.283 * unif(), 6.283 * unif()};
+                            r.spec = make_gate("u3", prm);
+                            r.tphys = {qs[rnd() % qs.size()]};
+                        }
+                        gate_sets[i].push_back(r);
+                    }
+                    blocks[i].qubits = qs;
+                    for (int g = 0; g < ng; ++g)
+                        blocks[i].gates.push_back(g);
+                    if (f64_)
+                    {
+                        auto v = prepare_blocks128(gate_sets[i], {blocks[i]}, n, mats64);
+                        pb128.push_back(v[0]);
+                    }
+                    else
+                    {
+                        auto v = prepare_blocks(gate_sets[i], {blocks[i]}, kq_, n, mats);
+                        pb.push_back(v[0]);
+                    }
+                }
+                const size_t bytes = f64_ ? mats64.size() * sizeof(double) : mats.size() * sizeof(float);
+                reserve_matrices(bytes);
+                qsv::check(hipStreamSynchronize(sim_.stream_), "sync");
+                qsv::check(hipMemcpy(rdev_, f64_ ? (const void *)mats64.data() : (const void *)mats.data(), bytes, hipMemcpyHostToDevice), "matrix upload");
+                for (int i = 0; i < m; ++i)
+                {
+                    qsv::check(hipEventRecord(ev[2 * i], sim_.stream_), "hipEventRecord");
+                    if (f64_)
+                        launch_prepared128(sim_.stream_, state128_, n, pb128[i], static_cast<const double *>(rdev_));
+                    else
+                        launch_prepared(sim_.stream_, state_, n, pb[i], static_cast<const float *>(rdev_));
+                    qsv::check(hipEventRecord(ev[2 * i + 1], sim_.stream_), "hipEventRecord");
+                }
+                qsv::check(hipStreamSynchronize(sim_.stream_), "sync");
+                if (start < 0)
+                    continue;
+                for (int i = 0; i < m; ++i)
+                {
+                    float ms = 0.f;
+                    qsv::check(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]), "hipEventElapsedTime");
+                    const int row = start + i;
+                    block_features(n, f64_, kq_, layout_block(gate_sets[i], blocks[i], kq_, n), (int)blocks[i].gates.size(), features + size_t(row) * kPlanFeatures);
+                    log2_us[row] = float(std::log2(std::max(1e-3, double(ms) * 1e3)));
+                }
+            }
+        }
+        catch (...)
+        {
+            for (auto e : ev)
+                (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto e : ev)
+            (void)hipEventDestroy(e);
     }
 
     void FusedEngine::swap_physical(int a, int b)
@@ -396,22 +581,112 @@ namespace qcore
         }
     }
 
+    std::vector<PreparedBlock128> prepare_blocks128(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan, int n, std::vector<double> &mats)
+    {
+        std::vector<PreparedBlock128> out;
+        out.reserve(plan.size());
+        for (const auto &pb : plan)
+        {
+            const BlockLayout lay = layout_block(gates, pb, 4, n);
+            const auto M = block_matrix(gates, pb, lay);
+            PreparedBlock128 b;
+            b.gates = (int)pb.gates.size();
+            b.matrix_offset = mats.size();
+            for (int i = 0; i < 256; ++i)
+            {
+                mats.push_back(M[i].real());
+                mats.push_back(M[i].imag());
+            }
+            std::vector<int> outside = lay.pos_out;
+            outside.insert(outside.end(), lay.neg_out.begin(), lay.neg_out.end());
+            b.sub = qsv::Subspace{};
+            for (int q : lay.pos_out)
+                b.sub.set_bits |= uint64_t(1) << q;
+            b.bytes = 2.0 * 16.0 * double(uint64_t(1) << (n - (int)outside.size()));
+            std::vector<int> cols;
+            for (int q = 0; q < n && cols.size() < 6; ++q)
+                if (local_pos(lay.qubits, q) < 0 && local_pos(outside, q) < 0)
+                    cols.push_back(q);
+            for (int e = 0; e < 16; ++e)
+            {
+                uint64_t o = 0;
+                for (int j = 0; j < 4; ++j)
+                    if ((e >> j) & 1)
+                        o |= uint64_t(1) << lay.qubits[j];
+                b.op.off[e] = o;
+            }
+            std::vector<int> ops;
+            b.staged = cols.size() == 6;
+            if (b.staged)
+            {
+                std::vector<int> tb = lay.qubits;
+                tb.insert(tb.end(), cols.begin(), cols.end());
+                std::sort(tb.begin(), tb.end());
+                for (size_t i = 0; i < tb.size(); ++i)
+                {
+                    b.op.gw[i] = uint64_t(1) << tb[i];
+                    int j = local_pos(lay.qubits, tb[i]);
+                    b.op.lw[i] = j >= 0 ? (1 << j) : (17 << local_pos(cols, tb[i]));
+                }
+                ops = tb;
+            }
+            else
+            {
+                ops = lay.qubits;
+            }
+            ops.insert(ops.end(), outside.begin(), outside.end());
+            std::sort(ops.begin(), ops.end());
+            b.sub.nops = (int)ops.size();
+            for (int i = 0; i < b.sub.nops; ++i)
+                b.sub.ops[i] = ops[i];
+            out.push_back(b);
+        }
+        return out;
+    }
+
+    void launch_prepared128(hipStream_t s, qsv::c128 *state, int n, PreparedBlock128 &b, const double *mats_dev)
+    {
+        b.op.M = mats_dev + b.matrix_offset;
+        qsv::launch_block128(s, state, n, b.sub, b.op, b.staged);
+    }
+
     void FusedEngine::apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan)
     {
         if (plan.empty())
             return;
         const int n = sim_.n_;
+        if (f64_)
+        {
+            std::vector<double> mats;
+            auto blocks = prepare_blocks128(gates, plan, n, mats);
+            const size_t bytes = mats.size() * sizeof(double);
+            reserve_matrices(bytes);
+            std::memcpy(rhost_, mats.data(), bytes);
+            qsv::check(hipMemcpyAsync(rdev_, rhost_, bytes, hipMemcpyHostToDevice, sim_.stream_), "matrix upload");
+            qsv::check(hipEventRecord(copy_done_, sim_.stream_), "hipEventRecord");
+            copy_pending_ = true;
+            for (auto &b : blocks)
+            {
+                sim_.prof_begin(QCS_EXT_K_FUSED, b.bytes);
+                launch_prepared128(sim_.stream_, state128_, n, b, static_cast<const double *>(rdev_));
+                sim_.prof_end();
+                blocks_applied_ += 1;
+                gates_applied_ += b.gates;
+            }
+            return;
+        }
         std::vector<float> mats;
         auto blocks = prepare_blocks(gates, plan, kq_, n, mats);
-        reserve_matrices(mats.size());
-        std::memcpy(rhost_, mats.data(), mats.size() * sizeof(float));
-        qsv::check(hipMemcpyAsync(rdev_, rhost_, mats.size() * sizeof(float), hipMemcpyHostToDevice, sim_.stream_), "matrix upload");
+        const size_t bytes = mats.size() * sizeof(float);
+        reserve_matrices(bytes);
+        std::memcpy(rhost_, mats.data(), bytes);
+        qsv::check(hipMemcpyAsync(rdev_, rhost_, bytes, hipMemcpyHostToDevice, sim_.stream_), "matrix upload");
         qsv::check(hipEventRecord(copy_done_, sim_.stream_), "hipEventRecord");
         copy_pending_ = true;
         for (auto &b : blocks)
         {
             sim_.prof_begin(QCS_EXT_K_FUSED, b.bytes);
-            launch_prepared(sim_.stream_, state_, n, b, rdev_);
+            launch_prepared(sim_.stream_, state_, n, b, static_cast<const float *>(rdev_));
             sim_.prof_end();
             blocks_applied_ += 1;
             gates_applied_ += b.gates;

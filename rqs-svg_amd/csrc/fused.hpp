@@ -20,6 +20,8 @@ namespace qcore
     // of its qubits) that grows least while staying within k qubits; otherwise it opens a
     // new block.  Gates on disjoint qubits commute, so a gate may join a later block.
     std::vector<PlannedBlock> plan_greedy(const std::vector<ResolvedGate> &gates, int k);
+    // The same with the search window and the choice rule as parameters (see fused.cpp).
+    std::vector<PlannedBlock> plan_rule(const std::vector<ResolvedGate> &gates, int k, int window, int rule);
 
     // How one planned block is launched: local bit j of the block matrix is physical qubit
     // qubits[j] (block qubits, then padding); out_ctrls stay outside as subspace controls.
@@ -53,15 +55,34 @@ namespace qcore
     // Launches one prepared block whose matrices were uploaded to mats_dev.
     void launch_prepared(hipStream_t s, qsv::c64 *state, int n, PreparedBlock &b, const float *mats_dev);
 
+    // complex128 blocks (4 qubits, FP64): the 16x16 complex matrices are appended to `mats`
+    // as interleaved doubles.
+    struct PreparedBlock128
+    {
+        int gates = 0;
+        size_t matrix_offset = 0;
+        double bytes = 0;
+        bool staged = true;
+        qsv::Subspace sub{};
+        qsv::Block128Op op{};
+    };
+    std::vector<PreparedBlock128> prepare_blocks128(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan, int n, std::vector<double> &mats);
+    void launch_prepared128(hipStream_t s, qsv::c128 *state, int n, PreparedBlock128 &b, const double *mats_dev);
+
+    class CostModel;
+
     class FusedEngine
     {
     public:
-        FusedEngine(Simulator &sim, int max_block_qubits);
+        // f64: complex128 state and FP64 blocks of 4 qubits; otherwise complex64 and FP32 MFMA.
+        FusedEngine(Simulator &sim, int max_block_qubits, bool f64);
         ~FusedEngine();
         FusedEngine(const FusedEngine &) = delete;
         FusedEngine &operator=(const FusedEngine &) = delete;
 
         qsv::c64 *state() const { return state_; }
+        qsv::c128 *state128() const { return state128_; }
+        bool f64() const { return f64_; }
         void enqueue(const ResolvedGate &g);
         void flush();
         void drop() { queue_.clear(); }
@@ -70,21 +91,33 @@ namespace qcore
         uint64_t blocks_applied() const { return blocks_applied_; }
         uint64_t gates_applied() const { return gates_applied_; }
 
+        // Learned planner: with a model attached, each flush picks the candidate plan with
+        // the lowest predicted time unless it does not beat the greedy plan by 3 %.
+        void set_model(CostModel *m) { model_ = m; }
+        // flushes, learned choices, predicted greedy us, predicted chosen us, blocks
+        void planner_stats(double out[5]) const;
+        // Time `count` random candidate blocks on this state; features + log2(microseconds).
+        void record(int count, uint64_t seed, float *features, float *log2_us);
+
     private:
         void apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan);
-        void reserve_matrices(size_t floats);
+        void reserve_matrices(size_t bytes);
 
         Simulator &sim_;
         int k_;  // planner limit (qubits per block)
         int kq_; // kernel block size (4 or 5)
+        bool f64_ = false;
         qsv::c64 *state_ = nullptr;
+        qsv::c128 *state128_ = nullptr;
         std::vector<ResolvedGate> queue_;
-        float *rdev_ = nullptr;
-        float *rhost_ = nullptr;
-        size_t rcap_ = 0;
+        void *rdev_ = nullptr;  // block matrices on the device
+        void *rhost_ = nullptr; // pinned staging for their upload
+        size_t rcap_ = 0;       // bytes
         hipEvent_t copy_done_ = nullptr;
         bool copy_pending_ = false;
         uint64_t blocks_applied_ = 0;
         uint64_t gates_applied_ = 0;
+        CostModel *model_ = nullptr;
+        double stat_flushes_ = 0, stat_learned_ = 0, stat_pred_greedy_ = 0, stat_pred_chosen_ = 0;
     };
 } // namespace qcore

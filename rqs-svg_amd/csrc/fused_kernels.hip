@@ -241,7 +241,161 @@ namespace qsv
                 wave_sync();
             }
         }
+        // complex128 block on 4 qubits.  Tile = 16 amplitudes x 64 columns (tile bits = the 4
+        // block qubits + the 6 lowest other qubits): moved in address order through LDS,
+        // then each lane applies the 16x16 complex matrix to its column in FP64 with the
+        // matrix read through uniform (scalar) loads.
+        constexpr int kRow128 = 17; // complex per LDS row (one of padding)
+
+        __global__ __launch_bounds__(kThreads) void k_block128_lds(c128 *__restrict__ s, uint64_t ntiles, Subspace tsub, Block128Op op)
+        {
+            typedef double v2d __attribute__((ext_vector_type(2)));
+            __shared__ __attribute__((aligned(16))) v2d lds[kWaves][64 * kRow128];
+            const int lane = threadIdx.x & 63;
+            const int wave = threadIdx.x >> 6;
+            v2d *tl = lds[wave];
+            const double *__restrict__ M = op.M;
+            uint64_t g_lane = 0;
+            int l_lane = 0;
+#pragma unroll
+            for (int b = 0; b < 6; ++b)
+                if ((lane >> b) & 1)
+                {
+                    g_lane += op.gw[b];
+                    l_lane += op.lw[b];
+                }
+            const uint64_t nwaves = uint64_t(gridDim.x) * kWaves;
+            for (uint64_t tile = uint64_t(blockIdx.x) * kWaves + wave; tile < ntiles; tile += nwaves)
+            {
+                const uint64_t base = expand(tile, tsub) + g_lane;
+                v2d v[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                {
+                    uint64_t gj = 0;
+#pragma unroll
+                    for (int b = 6; b < 10; ++b)
+                        if ((j >> (b - 6)) & 1)
+                            gj += op.gw[b];
+                    v[j] = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(s + (base + gj)));
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                {
+                    int lj = 0;
+#pragma unroll
+                    for (int b = 6; b < 10; ++b)
+                        if ((j >> (b - 6)) & 1)
+                            lj += op.lw[b];
+                    tl[l_lane + lj] = v[j];
+                }
+                wave_sync();
+                v2d x[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                    x[k] = tl[lane * kRow128 + k];
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                {
+                    double yr = 0.0, yi = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 16; ++k)
+                    {
+                        const double mr = M[2 * (16 * r + k)], mi = M[2 * (16 * r + k) + 1];
+                        yr = __builtin_fma(mr, x[k].x, yr);
+                        yr = __builtin_fma(-mi, x[k].y, yr);
+                        yi = __builtin_fma(mr, x[k].y, yi);
+                        yi = __builtin_fma(mi, x[k].x, yi);
+                    }
+                    tl[lane * kRow128 + r] = v2d{yr, yi};
+                }
+                wave_sync();
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                {
+                    uint64_t gj = 0;
+                    int lj = 0;
+#pragma unroll
+                    for (int b = 6; b < 10; ++b)
+                        if ((j >> (b - 6)) & 1)
+                        {
+                            gj += op.gw[b];
+                            lj += op.lw[b];
+                        }
+                    __builtin_nontemporal_store(tl[l_lane + lj], reinterpret_cast<v2d *>(s + (base + gj)));
+                }
+                wave_sync();
+            }
+        }
+
+        // complex128 block without LDS staging (fewer than 6 qubits outside the block): one
+        // lane per column, gathered directly.
+        __global__ __launch_bounds__(kThreads) void k_block128(c128 *__restrict__ s, uint64_t ncols, Subspace sub, Block128Op op)
+        {
+            typedef double v2d __attribute__((ext_vector_type(2)));
+            const double *__restrict__ M = op.M;
+            for (uint64_t col = uint64_t(blockIdx.x) * kThreads + threadIdx.x; col < ncols; col += uint64_t(gridDim.x) * kThreads)
+            {
+                const uint64_t base = expand(col, sub);
+                v2d x[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                    x[k] = *reinterpret_cast<const v2d *>(s + (base | op.off[k]));
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                {
+                    double yr = 0.0, yi = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 16; ++k)
+                    {
+                        const double mr = M[2 * (16 * r + k)], mi = M[2 * (16 * r + k) + 1];
+                        yr = __builtin_fma(mr, x[k].x, yr);
+                        yr = __builtin_fma(-mi, x[k].y, yr);
+                        yi = __builtin_fma(mr, x[k].y, yi);
+                        yi = __builtin_fma(mi, x[k].x, yi);
+                    }
+                    *reinterpret_cast<v2d *>(s + (base | op.off[r])) = v2d{yr, yi};
+                }
+            }
+        }
     } // namespace
+
+    void launch_block128(hipStream_t st, c128 *state, int n, const Subspace &sub, const Block128Op &op, bool staged)
+    {
+        if (sub.nops > n || n > kMaxQubits)
+            throw std::runtime_error("block128: operand count out of range");
+        for (int k = 0; k < sub.nops; ++k)
+            if (sub.ops[k] < 0 || sub.ops[k] >= n || (k > 0 && sub.ops[k] <= sub.ops[k - 1]))
+                throw std::runtime_error("block128: operands must be ascending and < n");
+        if (op.M == nullptr)
+            throw std::runtime_error("block128: matrix pointer is NULL");
+        const uint64_t count = uint64_t(1) << (n - sub.nops);
+        if (staged)
+        {
+            if (sub.nops < 10)
+                throw std::runtime_error("block128: staged form needs 10 tile bits");
+            for (int b = 0; b < 10; ++b)
+                if (n < 64 && (op.gw[b] >> n) != 0)
+                    throw std::runtime_error("block128: tile bit outside the state");
+            uint64_t grid = (count + kWaves - 1) / kWaves;
+            if (grid > 4096)
+                grid = 4096;
+            hipLaunchKernelGGL(k_block128_lds, dim3(grid), dim3(kThreads), 0, st, state, count, sub, op);
+        }
+        else
+        {
+            if (sub.nops < 4)
+                throw std::runtime_error("block128: needs 4 block qubits");
+            for (int e = 0; e < 16; ++e)
+                if (n < 64 && (op.off[e] >> n) != 0)
+                    throw std::runtime_error("block128: offset outside the state");
+            uint64_t grid = (count + kThreads - 1) / kThreads;
+            if (grid > 4096)
+                grid = 4096;
+            hipLaunchKernelGGL(k_block128, dim3(grid), dim3(kThreads), 0, st, state, count, sub, op);
+        }
+        check(hipGetLastError(), "block128 kernel");
+    }
 
     void launch_block_lds(hipStream_t st, c64 *state, int n, const Subspace &tsub, const BlockLdsOp &op)
     {

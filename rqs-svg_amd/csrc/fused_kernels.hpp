@@ -32,4 +32,17 @@ namespace qsv
         const float *R;
     };
     void launch_block_lds(hipStream_t s, c64 *state, int n, const Subspace &tsub, const BlockLdsOp &op);
+
+    // complex128 block on 4 qubits in FP64.  M: 16x16 complex, row-major, interleaved
+    // doubles, in device memory.  Staged form: gw/lw as for BlockLdsOp with 4 block bits and
+    // the 6 lowest other qubits as column bits (LDS weight of column bit i = 17 << i);
+    // direct form: off[e] = offset of local basis state e.
+    struct Block128Op
+    {
+        uint64_t gw[10];
+        int lw[10];
+        uint64_t off[16];
+        const double *M;
+    };
+    void launch_block128(hipStream_t s, c128 *state, int n, const Subspace &sub, const Block128Op &op, bool staged);
 } // namespace qsv

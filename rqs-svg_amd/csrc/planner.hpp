@@ -1,0 +1,57 @@
+// Learned fusion planner: an MLP cost model of block-kernel time, trained on timings the
+// simulator records, choosing among candidate fusion plans; the greedy plan is the
+// fallback and the baseline every choice is judged against.
+#pragma once
+#include "fused.hpp"
+#include "mlp_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace qcore
+{
+    constexpr int kPlanFeatures = 16;
+
+    // Features of one block launch (all in [0, ~1]); see planner.cpp for the list.
+    void block_features(int n, bool f64, int kq, const BlockLayout &lay, int gates, float *f);
+
+    class CostModel
+    {
+    public:
+        CostModel(int num_features, uint64_t seed);
+        ~CostModel();
+        CostModel(const CostModel &) = delete;
+        CostModel &operator=(const CostModel &) = delete;
+
+        int num_features() const { return F_; }
+        int num_params() const { return P_; }
+        void get_params(float *out) const;
+        void set_params(const float *in);
+        // predictions for B rows of features (host in/out), one GPU forward pass
+        void predict(const float *X, int B, float *Y);
+        // gradient of the mean squared error over the batch; returns the loss
+        float gradients(const float *X, const float *T, int B, float *G);
+        // Adam on shuffled mini-batches; returns the final mean squared error on all data
+        float train(const float *X, const float *T, int N, int epochs, int batch, float lr, uint64_t seed);
+
+    private:
+        void reserve(int B);
+        int F_, P_;
+        hipStream_t stream_ = nullptr;
+        float *params_ = nullptr, *grad_ = nullptr, *m_ = nullptr, *v_ = nullptr;
+        float *X_ = nullptr, *T_ = nullptr, *H1_ = nullptr, *H2_ = nullptr, *Y_ = nullptr;
+        float *dH1_ = nullptr, *dH2_ = nullptr, *dY_ = nullptr;
+        int cap_ = 0;
+        int step_ = 0;
+    };
+
+    struct PlanOption
+    {
+        int k;      // qubits per block
+        int window; // blocks examined after the dependency frontier
+        int rule;   // 0: least growth, 1: fullest block, 2: earliest block that fits
+    };
+    std::vector<PlannedBlock> plan_with(const std::vector<ResolvedGate> &gates, const PlanOption &opt);
+    std::vector<PlanOption> candidate_options(int kq);
+} // namespace qcore

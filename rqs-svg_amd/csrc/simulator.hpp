@@ -13,6 +13,7 @@
 namespace qcore
 {
     class FusedEngine;
+    class CostModel;
 
     struct ResolvedGate
     {
@@ -64,6 +65,9 @@ namespace qcore
         void set_precision(int mode, int k);
         void flush();
         void overlap(Simulator &other, double out[3]);
+        void set_planner_model(CostModel *m);
+        void planner_stats(double out[5]);
+        void planner_record(int count, uint64_t seed, float *features, float *log2_us);
         void fusion_stats(uint64_t *gates, uint64_t *blocks);
 
         int event_create();

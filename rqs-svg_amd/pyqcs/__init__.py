@@ -129,6 +129,19 @@ class Library:
         lib.qcs_ext_profile.argtypes = [vp, ctypes.c_int]
         lib.qcs_ext_profile_get.argtypes = [vp, ctypes.c_int, P(u64), P(ctypes.c_double), P(ctypes.c_double)]
         lib.qcs_ext_profile_reset.argtypes = [vp]
+        f32p = P(ctypes.c_float)
+        lib.qcs_ext_mlp_create.argtypes = [P(vp), ctypes.c_int, u64]
+        lib.qcs_ext_mlp_destroy.argtypes = [vp]
+        lib.qcs_ext_mlp_num_params.argtypes = [vp, P(ctypes.c_int)]
+        lib.qcs_ext_mlp_get_params.argtypes = [vp, vp, ctypes.c_int]
+        lib.qcs_ext_mlp_set_params.argtypes = [vp, vp, ctypes.c_int]
+        lib.qcs_ext_mlp_predict.argtypes = [vp, vp, ctypes.c_int, vp]
+        lib.qcs_ext_mlp_gradients.argtypes = [vp, vp, vp, ctypes.c_int, vp, f32p]
+        lib.qcs_ext_mlp_train.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, u64, f32p]
+        lib.qcs_ext_planner_num_features.argtypes = [P(ctypes.c_int)]
+        lib.qcs_ext_planner_record.argtypes = [vp, ctypes.c_int, u64, vp, vp]
+        lib.qcs_ext_set_planner.argtypes = [vp, vp]
+        lib.qcs_ext_planner_stats.argtypes = [vp, vp]
         lib.qcs_ext_batch_create.argtypes = [P(vp)]
         lib.qcs_ext_batch_destroy.argtypes = [vp]
         lib.qcs_ext_batch_add_circuit.argtypes = [vp, bit_num_t, ctypes.c_int32, P(ctypes.c_char_p), P(bit_num_t), P(bit_num_t), P(ctypes.c_double)]
@@ -355,6 +368,81 @@ class Simulator:
             if n.value:
                 res[name] = {"launches": n.value, "seconds": s.value, "bytes": b.value}
         return res
+
+
+class Mlp:
+    """Cost-model MLP (F -> 64 -> 64 -> 1): GPU kernels in libqcs.so, CPU loops in the oracle."""
+
+    def __init__(self, library: Library, num_features: int = 16, seed: int = 1):
+        self.L = library
+        self.lib = library.lib
+        h = ctypes.c_void_p()
+        self.L.check(self.lib.qcs_ext_mlp_create(ctypes.byref(h), num_features, seed), "mlp_create")
+        self.h = h
+        self.F = num_features
+        c = ctypes.c_int()
+        self.L.check(self.lib.qcs_ext_mlp_num_params(self.h, ctypes.byref(c)), "mlp_num_params")
+        self.P = c.value
+
+    def params(self):
+        out = np.empty(self.P, dtype=np.float32)
+        self.L.check(self.lib.qcs_ext_mlp_get_params(self.h, out.ctypes.data, self.P), "mlp_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self.L.check(self.lib.qcs_ext_mlp_set_params(self.h, p.ctypes.data, p.size), "mlp_set_params")
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.empty(X.shape[0], dtype=np.float32)
+        self.L.check(self.lib.qcs_ext_mlp_predict(self.h, X.ctypes.data, X.shape[0], y.ctypes.data), "mlp_predict")
+        return y
+
+    def gradients(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(self.P, dtype=np.float32)
+        loss = ctypes.c_float()
+        self.L.check(self.lib.qcs_ext_mlp_gradients(self.h, X.ctypes.data, T.ctypes.data, X.shape[0], g.ctypes.data, ctypes.byref(loss)), "mlp_gradients")
+        return g, loss.value
+
+    def train(self, X, T, epochs=50, batch=256, lr=1e-3, seed=1):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        loss = ctypes.c_float()
+        self.L.check(self.lib.qcs_ext_mlp_train(self.h, X.ctypes.data, T.ctypes.data, X.shape[0], epochs, batch, lr, seed, ctypes.byref(loss)), "mlp_train")
+        return loss.value
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value is not None:
+            self.lib.qcs_ext_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def planner_record(sim, count, seed):
+    F = 16
+    X = np.empty((count, F), dtype=np.float32)
+    y = np.empty(count, dtype=np.float32)
+    sim._call(sim.lib.qcs_ext_planner_record(sim.h, count, seed, X.ctypes.data, y.ctypes.data), "planner_record")
+    return X, y
+
+
+def set_planner(sim, model):
+    sim._call(sim.lib.qcs_ext_set_planner(sim.h, model.h if model is not None else None), "set_planner")
+
+
+def planner_stats(sim):
+    out = np.zeros(5, dtype=np.float64)
+    sim._call(sim.lib.qcs_ext_planner_stats(sim.h, out.ctypes.data), "planner_stats")
+    return {"flushes": out[0], "learned_choices": out[1], "pred_greedy_us": out[2],
+            "pred_chosen_us": out[3], "blocks": out[4]}
 
 
 class Batch:

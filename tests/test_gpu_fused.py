@@ -151,3 +151,35 @@ def test_precision_mode_errors(gpu):
     sim.close()
     with pyqcs.Simulator(gpu, 6) as sim:
         assert sim.lib.qcs_ext_set_precision(sim.h, 1, 5) == 0  # only before allocate_memory
+
+
+def test_fused_complex128_matches_oracle(gpu, oracle):
+    # precision mode 2: complex128 state, FP64 blocks of 4 qubits; FP64 bars apply
+    for n, seed in ((5, 1), (9, 2), (14, 3)):
+        circ = circuits.random_all_gates(n, 150, seed=seed, max_ctrls=3)
+        with pyqcs.Simulator(gpu, n, precision=2, block_qubits=4) as f:
+            f.run(circ)
+            got = f.statevector()
+            g, b = f.fusion_stats()
+        with pyqcs.Simulator(oracle, n) as o:
+            o.run(circ)
+            want = o.statevector()
+        say(f"c128 fused n={n}: max|d| {np.max(np.abs(got - want)):.2e} blocks {b} gates {g}")
+        assert np.max(np.abs(got - want)) <= 1e-13
+        assert fidelity(got, want) >= 1 - 1e-12
+
+
+def test_fused_complex128_qft_and_measure(gpu, oracle):
+    n = 12
+    circ = circuits.qft(n)
+    psi = random_state(n, 4)
+    res = []
+    for lib, prec in ((gpu, 2), (oracle, 0)):
+        with pyqcs.Simulator(lib, n, seed=3, precision=prec, block_qubits=4, num_clbits=n) as sim:
+            sim.set_statevector(psi)
+            sim.run(circ)
+            sv = sim.statevector()
+            bits = [sim.measure_to_clbit(q, q) for q in range(n)]
+            res.append((sv, bits))
+    assert np.max(np.abs(res[0][0] - res[1][0])) <= 1e-13
+    assert res[0][1] == res[1][1]

@@ -1,0 +1,130 @@
+"""Learned fusion planner: the cost-model MLP and its use by the fused engine.
+
+The MLP's forward pass and gradients are checked three ways: an independent numpy
+implementation (pinned by central finite differences in float64), the oracle library's CPU
+loops, and the GPU kernels of libqcs.so.
+"""
+import numpy as np
+import pytest
+
+import pyqcs
+from pyqcs import circuits
+from conftest import fidelity
+
+H = 64
+
+
+def unpack(p, F):
+    o = 0
+    W1 = p[o:o + H * F].reshape(H, F); o += H * F
+    b1 = p[o:o + H]; o += H
+    W2 = p[o:o + H * H].reshape(H, H); o += H * H
+    b2 = p[o:o + H]; o += H
+    w3 = p[o:o + H]; o += H
+    b3 = p[o]
+    return W1, b1, W2, b2, w3, b3
+
+
+def np_forward(p, X):
+    W1, b1, W2, b2, w3, b3 = unpack(p, X.shape[1])
+    h1 = np.maximum(X @ W1.T + b1, 0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0)
+    return h2 @ w3 + b3, h1, h2
+
+
+def np_loss(p, X, T):
+    y, _, _ = np_forward(p, X)
+    return np.mean((y - T) ** 2)
+
+
+def np_grad(p, X, T):
+    F = X.shape[1]
+    W1, b1, W2, b2, w3, b3 = unpack(p, F)
+    y, h1, h2 = np_forward(p, X)
+    B = X.shape[0]
+    dy = 2 * (y - T) / B
+    d2 = np.outer(dy, w3) * (h2 > 0)
+    d1 = (d2 @ W2) * (h1 > 0)
+    return np.concatenate([(d1.T @ X).ravel(), d1.sum(0), (d2.T @ h1).ravel(), d2.sum(0), h2.T @ dy, [dy.sum()]])
+
+
+def data(F=16, B=48, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(0, 1, size=(B, F)).astype(np.float32)
+    T = (np.sin(3 * X[:, 0]) + X[:, 1] * X[:, 2] + 0.5 * X[:, 3]).astype(np.float32)
+    return X, T
+
+
+def test_numpy_gradient_pinned_by_finite_differences():
+    F = 5
+    X, T = data(F, 20, 1)
+    rng = np.random.default_rng(2)
+    p = rng.normal(0, 0.4, size=H * F + 3 * H + H * H + 1)
+    X, T = X.astype(np.float64), T.astype(np.float64)
+    g = np_grad(p, X, T)
+    for i in rng.choice(p.size, 40, replace=False):
+        e = np.zeros_like(p)
+        e[i] = 1e-6
+        fd = (np_loss(p + e, X, T) - np_loss(p - e, X, T)) / 2e-6
+        assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(g[i])
+
+
+def test_oracle_mlp_matches_numpy(oracle):
+    X, T = data()
+    m = pyqcs.Mlp(oracle, 16, seed=7)
+    p = m.params()
+    assert p.size == H * 16 + 3 * H + H * H + 1
+    y = m.predict(X)
+    want, _, _ = np_forward(p.astype(np.float64), X.astype(np.float64))
+    assert np.max(np.abs(y - want)) < 1e-5
+    g, loss = m.gradients(X, T)
+    gw = np_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
+    assert np.max(np.abs(g - gw)) <= 1e-5 + 1e-4 * np.max(np.abs(gw))
+    assert abs(loss - np_loss(p.astype(np.float64), X, T)) < 1e-5
+    l0 = loss
+    l1 = m.train(X, T, epochs=200, batch=16, lr=3e-3, seed=3)
+    assert l1 < 0.3 * l0
+
+
+@pytest.mark.gpu
+def test_gpu_mlp_matches_oracle(gpu, oracle):
+    X, T = data(B=300, seed=4)
+    mg = pyqcs.Mlp(gpu, 16, seed=11)
+    mo = pyqcs.Mlp(oracle, 16, seed=11)
+    p = mg.params()
+    assert np.array_equal(p, mo.params())  # same SplitMix64 initialisation
+    assert np.max(np.abs(mg.predict(X) - mo.predict(X))) < 1e-5
+    gg, lg = mg.gradients(X, T)
+    go, lo = mo.gradients(X, T)
+    assert np.max(np.abs(gg - go)) <= 1e-6 + 1e-4 * np.max(np.abs(go))
+    assert abs(lg - lo) < 1e-5
+    l1 = mg.train(X, T, epochs=150, batch=32, lr=3e-3, seed=5)
+    assert l1 < 0.3 * lg
+
+
+@pytest.mark.gpu
+def test_planner_record_train_and_plan(gpu, oracle):
+    n = 20
+    with pyqcs.Simulator(gpu, n, precision=1, block_qubits=5) as sim:
+        X, y = pyqcs.planner_record(sim, 400, seed=9)
+    assert X.shape == (400, 16) and np.all(np.isfinite(y))
+    assert np.all(X[:, 15] == 1) and np.all(X[:, 0] == np.float32(n / 32))
+    model = pyqcs.Mlp(gpu, 16, seed=3)
+    loss = model.train(X, y, epochs=100, batch=64, lr=3e-3, seed=1)
+    assert loss < np.var(y)  # better than predicting the mean
+    circ = circuits.qft(n)
+    states = []
+    for use in (False, True):
+        with pyqcs.Simulator(gpu, n, precision=1, block_qubits=5) as sim:
+            if use:
+                pyqcs.set_planner(sim, model)
+            sim.run(circ)
+            states.append(sim.statevector())
+            st = pyqcs.planner_stats(sim)
+            if use:
+                assert st["flushes"] >= 1 and st["pred_chosen_us"] <= st["pred_greedy_us"] * 1.0001
+    with pyqcs.Simulator(oracle, n) as o:
+        o.run(circ)
+        want = o.statevector()
+    for s in states:
+        assert fidelity(s, want) >= 1 - 1e-6
