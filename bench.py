@@ -18,7 +18,13 @@ Also reported:
                    state, gates fused into k=5 blocks on FP32 MFMA) against the plain FP64
                    per-gate path on the same GPU: time of each, speedup, and fidelity /
                    max |d amp| of the fused state against the FP64 state (both resident in
-                   HBM, compared on the GPU in FP64).
+                   HBM, compared on the GPU in FP64);
+  batch         -- 512 variational circuits of 12-22 qubits in one autotuned hipGraph
+                   launch against a per-circuit plain-method loop, with max |d<Z>|;
+  planner       -- the learned fusion planner: block timings recorded at 26-32 qubits,
+                   cost-model MLP trained on the GPU, then a 33-qubit QFT and a 33-qubit
+                   brick circuit of depth 40 (complex128 fused) with the greedy plan and
+                   with the learned plan.
 """
 import argparse
 import json
@@ -26,6 +32,8 @@ import os
 import pathlib
 import sys
 import time
+
+import numpy as np
 
 REPO = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "rqs-svg_amd"))
@@ -122,6 +130,118 @@ def fused_leg(n, depth, k, shots):
     return res
 
 
+def batch_leg(count, seed=20250404):
+    """512 variational circuits (12-22 qubits, 4-16 layers of RY/RZ + CNOT ladder): one
+    autotuned hipGraph launch (complex64 states, FP64 <Z> reductions) against a per-circuit
+    plain-method loop (complex128, one kernel per gate, one simulator per qubit count)."""
+    lib = pyqcs.gpu_library()
+    work = circuits.variational_batch(count, seed=seed)
+    gates = sum(len(g) for _, g in work)
+    b = pyqcs.Batch(lib)
+    for n, g in work:
+        b.add(n, g)
+    t0 = time.perf_counter()
+    tuning = b.autotune()
+    t_tune = time.perf_counter() - t0
+    b.run()
+    runs = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        z_batch, dev = b.run()
+        runs.append((time.perf_counter() - t0, dev))
+    info = b.info()
+    b.close()
+    wall = min(r[0] for r in runs)
+    dev = min(r[1] for r in runs)
+    log(f"batch leg: {count} circuits, {gates} gates, graph {dev * 1e3:.2f} ms device "
+        f"({wall * 1e3:.2f} ms wall), tuning {tuning} found in {t_tune:.1f}s")
+    sims = {}
+    z_plain = []
+    t0 = time.perf_counter()
+    for n, g in work:
+        if n not in sims:
+            sims[n] = pyqcs.Simulator(lib, n, seed=1)
+        s = sims[n]
+        s.call("set_zero_state")
+        s.run(circuits.batch_to_circuit(g))
+        z_plain.append(s.expect_z())
+    t_plain = time.perf_counter() - t0
+    for s in sims.values():
+        s.close()
+    err = max(float(abs(x - y).max()) for x, y in zip(z_batch, z_plain))
+    log(f"batch leg: plain loop {t_plain:.3f}s, max |d<Z>| {err:.2e}")
+    return {"config": f"{count} variational circuits, 12-22 qubits, 4-16 layers RY+RZ on every "
+                      f"qubit + CNOT ladder, {gates} gates; <Z_i> of every qubit",
+            "circuits": count, "gates": gates, "graph_device_s": dev, "graph_wall_s": wall,
+            "plain_loop_s": t_plain, "speedup_wall": t_plain / wall,
+            "gates_per_s": gates / wall, "max_abs_diff_expz": err,
+            "tuning": tuning, "autotune_s": t_tune, "graph_nodes": info["graph_nodes"]}
+
+
+def planner_leg(n, record_qubits, record_count, depth, k):
+    """Learned fusion planner on complex128 fused blocks: record block timings on states of
+    `record_qubits` qubits, train the cost-model MLP on the GPU, then run a QFT and a random
+    brick circuit at n qubits with the greedy plan and with the learned plan. The two
+    complex128 results must agree (<Z_q> and probability total in FP64); agreement with the
+    FP64 oracle is covered by the tests at smaller n."""
+    lib = pyqcs.gpu_library()
+    Xs, ys = [], []
+    t0 = time.perf_counter()
+    per = max(1, record_count // len(record_qubits))
+    for i, m in enumerate(record_qubits):
+        with pyqcs.Simulator(lib, m, precision=2, block_qubits=k) as s:
+            X, y = pyqcs.planner_record(s, per, seed=77 + i)
+        Xs.append(X)
+        ys.append(y)
+    X = np.concatenate(Xs)
+    y = np.concatenate(ys)
+    t_rec = time.perf_counter() - t0
+    model = pyqcs.Mlp(lib, 16, seed=5)
+    t0 = time.perf_counter()
+    loss = model.train(X, y, epochs=300, batch=256, lr=2e-3, seed=2)
+    t_train = time.perf_counter() - t0
+    log(f"planner leg: recorded {len(y)} blocks at {list(record_qubits)} qubits in {t_rec:.1f}s; "
+        f"trained in {t_train:.2f}s, mse {loss:.4f} (variance {np.var(y):.4f})")
+    out = {"config": f"complex128 fused blocks (k<={k}), {n}-qubit QFT and {n}-qubit brick "
+                     f"depth {depth}; cost model trained on {len(y)} recorded blocks at "
+                     f"{min(record_qubits)}-{max(record_qubits)} qubits",
+           "recorded_blocks": int(len(y)), "record_s": t_rec, "train_s": t_train,
+           "train_mse_log2us": float(loss), "target_variance": float(np.var(y))}
+    sim = pyqcs.Simulator(lib, n, seed=1, precision=2, block_qubits=k)
+    try:
+        sim.run(circuits.brick(n, 1, seed=1)[: n + 2])
+        for name, circ in (("qft", circuits.qft(n)), ("brick", circuits.brick(n, depth, seed=CIRCUIT_SEED + 5))):
+            res = {"gates": len(circ)}
+            zs = {}
+            for mode in ("greedy", "learned"):
+                pyqcs.set_planner(sim, model if mode == "learned" else None)
+                sim.call("set_zero_state")
+                sim.norm()
+                _, b0 = sim.fusion_stats()
+                t0 = time.perf_counter()
+                sim.run(circ)
+                total = sim.probability_total()
+                dt = time.perf_counter() - t0
+                _, b1 = sim.fusion_stats()
+                zs[mode] = sim.expect_z()
+                res[f"{mode}_s"] = dt
+                res[f"{mode}_blocks"] = b1 - b0
+                res[f"{mode}_probability_total"] = total
+                if mode == "learned":
+                    st = pyqcs.planner_stats(sim)
+                    res["pred_greedy_us"] = st["pred_greedy_us"]
+                    res["pred_chosen_us"] = st["pred_chosen_us"]
+                log(f"planner leg: {name} {mode}: {dt:.3f}s, {b1 - b0} blocks")
+            res["speedup"] = res["greedy_s"] / res["learned_s"]
+            res["max_abs_diff_expz"] = float(np.max(np.abs(zs["greedy"] - zs["learned"])))
+            out[name] = res
+    finally:
+        pyqcs.set_planner(sim, None)
+        sim.close()
+        model.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +256,14 @@ def main():
     ap.add_argument("--fused-qubits", type=int, default=32)
     ap.add_argument("--fused-depth", type=int, default=32)
     ap.add_argument("--fused-k", type=int, default=5)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-circuits", type=int, default=512)
+    ap.add_argument("--no-planner", action="store_true")
+    ap.add_argument("--planner-qubits", type=int, default=33)
+    ap.add_argument("--planner-depth", type=int, default=40)
+    ap.add_argument("--planner-k", type=int, default=4)
+    ap.add_argument("--planner-record", type=int, default=10000,
+                    help="blocks timed for the cost model, spread over 26, 28, 30 and 32 qubits")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one write-only fill and one read-only norm pass of the "
                          "full state (known HBM bytes, for scaling PMC byte counters)")
@@ -231,6 +359,11 @@ def main():
     }
     if not args.no_fused:
         result["fused_c64"] = fused_leg(args.fused_qubits, args.fused_depth, args.fused_k, args.shots)
+    if not args.no_batch:
+        result["batch"] = batch_leg(args.batch_circuits)
+    if not args.no_planner:
+        result["planner"] = planner_leg(args.planner_qubits, (26, 28, 30, 32), args.planner_record,
+                                        args.planner_depth, args.planner_k)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(n, circ, args.cpu_budget)
     print(json.dumps(result), flush=True)

@@ -65,6 +65,12 @@ int qcs_ext_set_seed(qcs_simulator *sim, uint64_t seed);
 int qcs_ext_get_statevector(qcs_simulator *sim, double *out, uint64_t num_amplitudes);
 int qcs_ext_set_statevector(qcs_simulator *sim, const double *in, uint64_t num_amplitudes);
 
+/* Load a state saved by qcs_simulator_save_statevector (raw complex<double>, logical order,
+ * exactly 2^num_qubits amplitudes) into the allocated state, under any qubit mapping, and
+ * clear the measured lists. Replaces the reference's unreachable loader
+ * (src/qcs.cu:2121-2159, fixed file name statevector_input.bin, physical order). */
+int qcs_ext_load_statevector(qcs_simulator *sim, const char *path);
+
 /* Seeded sampling of num_shots full-register outcomes (logical indices). */
 int qcs_ext_sample(qcs_simulator *sim, uint64_t seed, uint64_t num_shots, uint64_t *outcomes);
 

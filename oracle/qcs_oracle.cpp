@@ -1263,6 +1263,27 @@ extern "C"
             }
         });
     }
+    int qcs_ext_load_statevector(qcs_simulator *s, const char *path)
+    { // src/qcs.cu:2121-2159 with the file format of save_statevector (logical order)
+        return guard([&] {
+            auto &c = core(s);
+            c.need_alloc();
+            if (path == nullptr)
+                throw std::runtime_error("input file name must not be NULL");
+            FILE *fp = std::fopen(path, "rb");
+            if (!fp)
+                throw std::runtime_error("open failed");
+            std::vector<oracle::cplx> in(c.state.size() + 1);
+            const size_t got = std::fread(in.data(), sizeof(oracle::cplx), in.size(), fp);
+            std::fclose(fp);
+            if (got != c.state.size())
+                throw std::runtime_error(oracle::fmt("statevector file holds %zu amplitudes, expected %zu", got, c.state.size()));
+            for (uint64_t i = 0; i < c.state.size(); ++i)
+                c.state[i] = in[c.logical_index(i)];
+            c.m0.clear();
+            c.m1.clear();
+        });
+    }
     int qcs_ext_sample(qcs_simulator *s, uint64_t sd, uint64_t shots, uint64_t *out)
     {
         return guard([&] {

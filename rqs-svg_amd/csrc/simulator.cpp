@@ -678,6 +678,48 @@ namespace qcore
         qsv::check(hipStreamSynchronize(stream_), "sync");
     }
 
+    void Simulator::load_statevector(const char *path)
+    { // src/qcs.cu:2121-2159 (reads physical order from a fixed name); here: any path, the
+      // logical-order format save_statevector writes, streamed through the pinned stage
+        need_alloc();
+        flush();
+        if (path == nullptr)
+            throw std::runtime_error("input file name must not be NULL");
+        FILE *fp = std::fopen(path, "rb");
+        if (!fp)
+            throw std::runtime_error("open failed");
+        try
+        {
+            const uint64_t count = uint64_t(1) << n_;
+            if (std::fseek(fp, 0, SEEK_END) != 0)
+                throw std::runtime_error("fseek failed");
+            const long long size = std::ftell(fp);
+            if (size < 0 || uint64_t(size) != count * sizeof(qsv::c128))
+                throw std::runtime_error("statevector file holds " + std::to_string(size) + " bytes, expected " + std::to_string(count * sizeof(qsv::c128)));
+            std::rewind(fp);
+            if (fused_)
+                fused_->drop();
+            ensure_stage();
+            const uint64_t chunk = logical_chunk();
+            for (uint64_t first = 0; first < count; first += chunk)
+            {
+                qsv::check(hipStreamSynchronize(stream_), "sync");
+                if (std::fread(stage_host_, sizeof(qsv::c128), chunk, fp) != chunk)
+                    throw std::runtime_error("fread failed");
+                qsv::check(hipMemcpyAsync(stage_dev_, stage_host_, chunk * sizeof(qsv::c128), hipMemcpyHostToDevice, stream_), "copy");
+                with_state([&](auto *st) { qsv::launch_scatter_logical(stream_, st, stage_dev_, first, chunk, l2p_.data(), n_); });
+            }
+            qsv::check(hipStreamSynchronize(stream_), "sync");
+        }
+        catch (...)
+        {
+            std::fclose(fp);
+            throw;
+        }
+        std::fclose(fp);
+        clear_measurement_state();
+    }
+
     void Simulator::ensure_levels()
     {
         auto sz = qsv::fold_level_sizes(uint64_t(1) << n_);

@@ -55,6 +55,7 @@ namespace qcore
         void save_statevector(const char *path);
         void get_statevector(double *out, uint64_t count);
         void set_statevector(const double *in, uint64_t count);
+        void load_statevector(const char *path);
         void sample(uint64_t seed, uint64_t shots, uint64_t *out);
         double probability_total();
         double norm();

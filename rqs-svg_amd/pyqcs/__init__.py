@@ -114,6 +114,7 @@ class Library:
         lib.qcs_ext_set_seed.argtypes = [vp, u64]
         lib.qcs_ext_get_statevector.argtypes = [vp, vp, u64]
         lib.qcs_ext_set_statevector.argtypes = [vp, vp, u64]
+        lib.qcs_ext_load_statevector.argtypes = [vp, ctypes.c_char_p]
         lib.qcs_ext_sample.argtypes = [vp, u64, u64, vp]
         lib.qcs_ext_probability_total.argtypes = [vp, P(ctypes.c_double)]
         lib.qcs_ext_norm.argtypes = [vp, P(ctypes.c_double)]
@@ -282,6 +283,9 @@ class Simulator:
     def set_statevector(self, psi):
         psi = np.ascontiguousarray(psi, dtype=np.complex128)
         self._call(self.lib.qcs_ext_set_statevector(self.h, psi.ctypes.data, psi.size), "set_statevector")
+
+    def load(self, path):
+        self._call(self.lib.qcs_ext_load_statevector(self.h, str(path).encode()), "load_statevector")
 
     def save(self, path):
         self._call(self.lib.qcs_simulator_save_statevector(self.h, str(path).encode()), "save_statevector")
