@@ -15,7 +15,8 @@ Also reported:
   cpu_baseline  -- the oracle (the reference's plain method restated in C++/OpenMP) on the
                    host cores, on a bounded sample of the same workload;
   fused_c64     -- the precision form on the 32-qubit brick circuit of depth 32 (complex64
-                   state, gates fused into k=5 blocks on FP32 MFMA) against the plain FP64
+                   state, gates fused into k=5 blocks on FP32 MFMA; fused_c64_bf16x3 the
+                   same on BF16x3 split-precision MFMA) against the plain FP64
                    per-gate path on the same GPU: time of each, speedup, and fidelity /
                    max |d amp| of the fused state against the FP64 state (both resident in
                    HBM, compared on the GPU in FP64);
@@ -88,14 +89,12 @@ def cpu_baseline(n, circ, budget_s=20.0):
     }
 
 
-def fused_leg(n, depth, k, shots):
-    """Config: 32 qubits, brick depth 32; fused complex64 vs plain complex128 on one GPU."""
+def fused_leg(n, depth, k, shots, modes=(1, 3)):
+    """Config: 32 qubits, brick depth 32; fused complex64 vs plain complex128 on one GPU.
+    Precision mode 1 runs the blocks on FP32 MFMA, mode 3 on BF16x3 split-precision MFMA."""
     lib = pyqcs.gpu_library()
     circ = circuits.brick(n, depth, seed=CIRCUIT_SEED + 3)
     warm = circuits.brick(n, 1, seed=1)[: n + 2]
-    res = {"config": f"brick{n}: {n} qubits, {depth} brick layers (U3 all + CZ even/odd), "
-                     f"{len(circ)} gates, {shots} samples",
-           "gates": len(circ), "block_qubits": k}
     plain = pyqcs.Simulator(lib, n, seed=1)
     plain.run(warm)
     plain.call("set_zero_state")
@@ -105,29 +104,51 @@ def fused_leg(n, depth, k, shots):
     plain.probability_total()
     t_plain = time.perf_counter() - t0
     log(f"fused leg: plain c128 {t_plain:.3f}s")
-    fused = pyqcs.Simulator(lib, n, seed=1, precision=1, block_qubits=k)
-    fused.run(warm)
-    fused.call("set_zero_state")
-    fused.norm()
-    g0, b0 = fused.fusion_stats()
-    t0 = time.perf_counter()
-    fused.run(circ)
-    total = fused.probability_total()
-    t_fused = time.perf_counter() - t0
-    g1, b1 = fused.fusion_stats()
-    ts = time.perf_counter()
-    fused.sample(shots, seed=SAMPLE_SEED)
-    t_sample = time.perf_counter() - ts
-    ov, maxdiff = fused.overlap(plain)
-    fid = abs(ov) ** 2 / (fused.norm() * plain.norm())
-    fused.close()
+    out = {}
+    for mode in modes:
+        math_name = {1: "FP32 MFMA", 3: "BF16x3 split-precision MFMA"}[mode]
+        res = {"config": f"brick{n}: {n} qubits, {depth} brick layers (U3 all + CZ even/odd), "
+                         f"{len(circ)} gates, {shots} samples; complex64 state, k={k} blocks on {math_name}",
+               "gates": len(circ), "block_qubits": k, "precision_mode": mode}
+        fused = pyqcs.Simulator(lib, n, seed=1, precision=mode, block_qubits=k)
+        fused.run(warm)
+        fused.call("set_zero_state")
+        fused.norm()
+        g0, b0 = fused.fusion_stats()
+        fused.profile_reset()
+        fused.profile(True)
+        t0 = time.perf_counter()
+        fused.run(circ)
+        total = fused.probability_total()
+        t_fused = time.perf_counter() - t0
+        fused.profile(False)
+        g1, b1 = fused.fusion_stats()
+        blk = fused.profile_get().get("fused", {"launches": 1, "seconds": 1.0, "bytes": 0.0})
+        ts = time.perf_counter()
+        fused.sample(shots, seed=SAMPLE_SEED)
+        t_sample = time.perf_counter() - ts
+        ov, maxdiff = fused.overlap(plain)
+        fid = abs(ov) ** 2 / (fused.norm() * plain.norm())
+        fused.close()
+        log(f"fused leg: mode {mode} {t_fused:.3f}s ({b1 - b0} blocks), fidelity {fid:.12f}, max|d| {maxdiff:.2e}")
+        res.update({"plain_c128_s": t_plain, "fused_c64_s": t_fused, "speedup": t_plain / t_fused,
+                    "blocks": b1 - b0,
+                    "fidelity": fid, "infidelity": 1 - fid, "max_abs_diff": maxdiff,
+                    "probability_total": total, "sample_s": t_sample})
+        # block kernel: algorithmic bytes 2 * 8 B * amplitudes per launch; the real-ified
+        # 2^(k+1) x 2^(k+1) product costs 8 * 2^k flops per amplitude
+        gbs = blk["bytes"] / blk["seconds"] / 1e9
+        tflops = blk["bytes"] / 16 * 8 * (1 << k) / blk["seconds"] / 1e12
+        res["roofline"] = {"bound": "hbm", "kernel": f"k_block_lds<{k}, {'true' if mode == 3 else 'false'}>",
+                           "achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                           "frac": gbs / (HBM_PEAK / 1e9), "launches": blk["launches"],
+                           "avg_ms": blk["seconds"] / blk["launches"] * 1e3,
+                           "mfma_tflops": tflops,
+                           "mfma_peak_tflops": 157.3 if mode == 1 else 2500.0 / 3}
+        log(f"fused leg: block kernel {gbs:.1f} GB/s, {tflops:.1f} TFLOP/s over {blk['launches']} launches")
+        out["fused_c64" if mode == 1 else "fused_c64_bf16x3"] = res
     plain.close()
-    log(f"fused leg: fused c64 {t_fused:.3f}s ({b1 - b0} blocks), fidelity {fid:.12f}, max|d| {maxdiff:.2e}")
-    res.update({"plain_c128_s": t_plain, "fused_c64_s": t_fused, "speedup": t_plain / t_fused,
-                "blocks": b1 - b0,
-                "fidelity": fid, "infidelity": 1 - fid, "max_abs_diff": maxdiff,
-                "probability_total": total, "sample_s": t_sample})
-    return res
+    return out
 
 
 def batch_leg(count, seed=20250404):
@@ -358,7 +379,7 @@ def main():
         "hbm_frac_whole_step": alg_bytes * args.steps / wall / HBM_PEAK,
     }
     if not args.no_fused:
-        result["fused_c64"] = fused_leg(args.fused_qubits, args.fused_depth, args.fused_k, args.shots)
+        result.update(fused_leg(args.fused_qubits, args.fused_depth, args.fused_k, args.shots))
     if not args.no_batch:
         result["batch"] = batch_leg(args.batch_circuits)
     if not args.no_planner:

@@ -102,7 +102,9 @@ int qcs_ext_permute(qcs_simulator *sim, const bit_num_t *new_perm_p2l, bit_num_t
  * state, gates queued and fused into blocks of at most max_block_qubits (1..5) qubits
  * applied as dense products on FP32 MFMA (FP32 accumulation); FP64 only in reductions and
  * renormalisation.  mode 2: complex128 state, gates fused into blocks of at most 4 qubits
- * applied in FP64.  Must be called before allocate_memory. */
+ * applied in FP64.  mode 3: as mode 1 with the block products on BF16x3 split-precision
+ * MFMA (matrix and state each split into two bfloat16 parts, three bf16 products summed in
+ * FP32).  Must be called before allocate_memory. */
 int qcs_ext_set_precision(qcs_simulator *sim, int mode, int max_block_qubits);
 
 /* Apply all queued gates now (no-op in mode 0). */

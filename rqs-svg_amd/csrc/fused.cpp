@@ -218,12 +218,14 @@ namespace qcore
         return M;
     }
 
-    FusedEngine::FusedEngine(Simulator &sim, int k, bool f64) : sim_(sim), k_(k), f64_(f64)
+    FusedEngine::FusedEngine(Simulator &sim, int k, int mode) : sim_(sim), k_(k), f64_(mode == 2), x3_(mode == 3)
     {
         const int n = sim_.n_;
+        if (mode < 1 || mode > 3)
+            throw std::runtime_error("fused engine: mode must be 1, 2 or 3");
         if (n < 4)
             throw std::runtime_error("precision mode needs at least 4 qubits");
-        kq_ = (f64 || k <= 4 || n < 5) ? 4 : 5;
+        kq_ = (f64_ || k <= 4 || n < 5) ? 4 : 5;
         if (k_ > kq_)
             k_ = kq_;
         if (f64_)
@@ -323,7 +325,7 @@ namespace qcore
                 for (const auto &b : plans[pi])
                 {
                     float f[kPlanFeatures];
-                    block_features(n, f64_, kq_, layout_block(gates, b, kq_, n), (int)b.gates.size(), f);
+                    block_features(n, math_feature(), kq_, layout_block(gates, b, kq_, n), (int)b.gates.size(), f);
                     X.insert(X.end(), f, f + kPlanFeatures);
                 }
             }
@@ -432,7 +434,7 @@ namespace qcore
                     }
                     else
                     {
-                        auto v = prepare_blocks(gate_sets[i], {blocks[i]}, kq_, n, mats);
+                        auto v = prepare_blocks(gate_sets[i], {blocks[i]}, kq_, n, mats, x3_);
                         pb.push_back(v[0]);
                     }
                 }
@@ -457,7 +459,7 @@ namespace qcore
                     float ms = 0.f;
                     qsv::check(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]), "hipEventElapsedTime");
                     const int row = start + i;
-                    block_features(n, f64_, kq_, layout_block(gate_sets[i], blocks[i], kq_, n), (int)blocks[i].gates.size(), features + size_t(row) * kPlanFeatures);
+                    block_features(n, math_feature(), kq_, layout_block(gate_sets[i], blocks[i], kq_, n), (int)blocks[i].gates.size(), features + size_t(row) * kPlanFeatures);
                     log2_us[row] = float(std::log2(std::max(1e-3, double(ms) * 1e3)));
                 }
             }
@@ -483,7 +485,37 @@ namespace qcore
         apply_blocks(gates, plan_greedy(gates, k_));
     }
 
-    std::vector<PreparedBlock> prepare_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan, int kq, int n, std::vector<float> &mats)
+    namespace
+    {
+        // float -> bfloat16 bits, round to nearest even (NaN stays NaN)
+        uint16_t bf16_bits(float f)
+        {
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            if ((u & 0x7fffffffu) > 0x7f800000u)
+                return uint16_t((u >> 16) | 0x40u);
+            u += 0x7fffu + ((u >> 16) & 1u);
+            return uint16_t(u >> 16);
+        }
+        float bf16_value(uint16_t b)
+        {
+            const uint32_t u = uint32_t(b) << 16;
+            float f;
+            std::memcpy(&f, &u, 4);
+            return f;
+        }
+    } // namespace
+
+    void split_bf16(const double *R, size_t count, uint16_t *hi, uint16_t *lo)
+    {
+        for (size_t i = 0; i < count; ++i)
+        {
+            hi[i] = bf16_bits(float(R[i]));
+            lo[i] = bf16_bits(float(R[i] - double(bf16_value(hi[i]))));
+        }
+    }
+
+    std::vector<PreparedBlock> prepare_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan, int kq, int n, std::vector<float> &mats, bool x3)
     {
         const int D = 2 << kq;
         const size_t per = size_t(D) * D;
@@ -499,17 +531,27 @@ namespace qcore
             b.gates = (int)pb.gates.size();
             b.matrix_offset = mats.size();
             mats.resize(mats.size() + per);
-            float *R = mats.data() + b.matrix_offset;
+            std::vector<double> R(per);
             for (int r = 0; r < K; ++r)
                 for (int c = 0; c < K; ++c)
                 {
                     const cd z = M[size_t(r) * K + c];
-                    const float re = float(z.real()), im = float(z.imag());
-                    R[(2 * r) * D + 2 * c] = re;
-                    R[(2 * r) * D + 2 * c + 1] = -im;
-                    R[(2 * r + 1) * D + 2 * c] = im;
-                    R[(2 * r + 1) * D + 2 * c + 1] = re;
+                    R[(2 * r) * D + 2 * c] = z.real();
+                    R[(2 * r) * D + 2 * c + 1] = -z.imag();
+                    R[(2 * r + 1) * D + 2 * c] = z.imag();
+                    R[(2 * r + 1) * D + 2 * c + 1] = z.real();
                 }
+            if (x3)
+            { // bfloat16 high parts then low parts, 2 * per halves = per floats of storage
+                std::vector<uint16_t> parts(2 * per);
+                split_bf16(R.data(), per, parts.data(), parts.data() + per);
+                std::memcpy(mats.data() + b.matrix_offset, parts.data(), 2 * per * sizeof(uint16_t));
+            }
+            else
+            {
+                for (size_t i = 0; i < per; ++i)
+                    mats[b.matrix_offset + i] = float(R[i]);
+            }
             std::vector<int> outside = lay.pos_out;
             outside.insert(outside.end(), lay.neg_out.begin(), lay.neg_out.end());
             uint64_t setb = 0;
@@ -533,6 +575,7 @@ namespace qcore
                 std::sort(tb.begin(), tb.end());
                 b.lop = qsv::BlockLdsOp{};
                 b.lop.kq = kq;
+                b.lop.x3 = x3 ? 1 : 0;
                 const int row = K + 2;
                 for (size_t i = 0; i < tb.size(); ++i)
                 {
@@ -547,6 +590,7 @@ namespace qcore
                 b.lds = false;
                 b.op = qsv::BlockOp{};
                 b.op.kq = kq;
+                b.op.x3 = x3 ? 1 : 0;
                 for (int e = 0; e < K; ++e)
                 {
                     uint64_t o = 0;
@@ -676,7 +720,7 @@ namespace qcore
             return;
         }
         std::vector<float> mats;
-        auto blocks = prepare_blocks(gates, plan, kq_, n, mats);
+        auto blocks = prepare_blocks(gates, plan, kq_, n, mats, x3_);
         const size_t bytes = mats.size() * sizeof(float);
         reserve_matrices(bytes);
         std::memcpy(rhost_, mats.data(), bytes);

@@ -51,7 +51,10 @@ namespace qcore
     };
     // Lays out and composes every planned block; appends the real (2^(kq+1))^2 matrices to
     // `mats` (host) in launch order.
-    std::vector<PreparedBlock> prepare_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan, int kq, int n, std::vector<float> &mats);
+    // x3: store each matrix as bfloat16 high parts followed by low parts (BF16x3 kernels).
+    std::vector<PreparedBlock> prepare_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan, int kq, int n, std::vector<float> &mats, bool x3 = false);
+    // hi[i] + lo[i] ~ R[i] with both parts bfloat16 bits rounded to nearest even.
+    void split_bf16(const double *R, size_t count, uint16_t *hi, uint16_t *lo);
     // Launches one prepared block whose matrices were uploaded to mats_dev.
     void launch_prepared(hipStream_t s, qsv::c64 *state, int n, PreparedBlock &b, const float *mats_dev);
 
@@ -74,8 +77,9 @@ namespace qcore
     class FusedEngine
     {
     public:
-        // f64: complex128 state and FP64 blocks of 4 qubits; otherwise complex64 and FP32 MFMA.
-        FusedEngine(Simulator &sim, int max_block_qubits, bool f64);
+        // mode 1: complex64 state, FP32 MFMA blocks; 2: complex128 state, FP64 blocks of 4
+        // qubits; 3: complex64 state, BF16x3 split-precision MFMA blocks.
+        FusedEngine(Simulator &sim, int max_block_qubits, int mode);
         ~FusedEngine();
         FusedEngine(const FusedEngine &) = delete;
         FusedEngine &operator=(const FusedEngine &) = delete;
@@ -83,6 +87,9 @@ namespace qcore
         qsv::c64 *state() const { return state_; }
         qsv::c128 *state128() const { return state128_; }
         bool f64() const { return f64_; }
+        bool x3() const { return x3_; }
+        // planner feature 2: 1 complex128, 0.5 complex64 BF16x3, 0 complex64 FP32
+        float math_feature() const { return f64_ ? 1.f : (x3_ ? 0.5f : 0.f); }
         void enqueue(const ResolvedGate &g);
         void flush();
         void drop() { queue_.clear(); }
@@ -107,6 +114,7 @@ namespace qcore
         int k_;  // planner limit (qubits per block)
         int kq_; // kernel block size (4 or 5)
         bool f64_ = false;
+        bool x3_ = false;
         qsv::c64 *state_ = nullptr;
         qsv::c128 *state128_ = nullptr;
         std::vector<ResolvedGate> queue_;

@@ -1,4 +1,5 @@
-// Precision form: dense k-qubit blocks on a complex64 state, FP32 MFMA (gfx950).
+// Precision form: dense k-qubit blocks on a complex64 state, FP32 or BF16x3 split-precision
+// MFMA (gfx950).
 //
 // A block acts on KQ qubits (KQ = 4 or 5).  Its 2^KQ x 2^KQ complex matrix M is given as
 // the real D x D matrix R (D = 2^(KQ+1)) on interleaved (re, im) vectors:
@@ -13,7 +14,8 @@
 // therefore needs the contiguous half h of column c -- 2^KQ/2 amplitudes loaded straight
 // from HBM -- and keeps its rows of R in registers for the whole launch.  The D/32
 // accumulators hold, per lane, pairs of consecutive amplitudes of column c, written back
-// in place.  Accumulation is FP32.
+// in place.  Accumulation is FP32.  The BF16x3 form (BlockMatrix<KQ, true>) runs the same
+// K split in steps of 8 values per lane on the bf16 MFMA.
 #include "fused_kernels.hpp"
 
 #include <stdexcept>
@@ -30,6 +32,98 @@ namespace qsv
         constexpr int kWaves = 4;
         constexpr int kThreads = 64 * kWaves;
 
+        typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+        // The block matrix R as MFMA A-operands, held in registers for the whole launch, and
+        // its product with a wave's 32 columns.  Lane (h, c) supplies rows 32*mt + c of R and
+        // the half h of column c: x[0..H) are the K values h*H .. h*H + H - 1.
+        template <int KQ, bool X3>
+        struct BlockMatrix;
+
+        // FP32 on v_mfma_f32_32x32x2f32: step st pairs K value st of lanes 0-31 with K value
+        // st of lanes 32-63.
+        template <int KQ>
+        struct BlockMatrix<KQ, false>
+        {
+            static constexpr int D = 2 << KQ, MT = D / 32, H = D / 2;
+            float a[MT][H];
+            __device__ __forceinline__ void load(const float *R, int c, int h)
+            {
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    const v4f *rp = reinterpret_cast<const v4f *>(R + (32 * mt + c) * D + h * H);
+#pragma unroll
+                    for (int q = 0; q < H / 4; ++q)
+                    {
+                        const v4f v = rp[q];
+                        a[mt][4 * q + 0] = v.x;
+                        a[mt][4 * q + 1] = v.y;
+                        a[mt][4 * q + 2] = v.z;
+                        a[mt][4 * q + 3] = v.w;
+                    }
+                }
+            }
+            __device__ __forceinline__ void apply(const float (&x)[H], v16f (&d)[MT]) const
+            {
+#pragma unroll
+                for (int st = 0; st < H; ++st)
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                        d[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt][st], x[st], d[mt], 0, 0, 0);
+            }
+        };
+
+        // BF16x3 split precision on v_mfma_f32_32x32x16_bf16: R = Rh + Rl and x = xh + xl
+        // with each part a bfloat16 rounded to nearest even (the two parts carry 16
+        // significand bits), and R x ~ Rl xh + Rh xl + Rh xh accumulated in FP32.  The
+        // dropped Rl xl term is about 2^-16 of each product.  Step st takes the K values
+        // h*H + 8*st .. h*H + 8*st + 7 from lane half h.  R points to Rh (D*D bfloat16,
+        // row-major) followed by Rl.
+        template <int KQ>
+        struct BlockMatrix<KQ, true>
+        {
+            static constexpr int D = 2 << KQ, MT = D / 32, H = D / 2, ST = H / 8;
+            bf16x8 hi[MT][ST], lo[MT][ST];
+            __device__ __forceinline__ void load(const float *R, int c, int h)
+            {
+                const bf16x8 *rh = reinterpret_cast<const bf16x8 *>(R);
+                const bf16x8 *rl = rh + D * D / 8;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int st = 0; st < ST; ++st)
+                    {
+                        const int v = ((32 * mt + c) * D + h * H) / 8 + st;
+                        hi[mt][st] = rh[v];
+                        lo[mt][st] = rl[v];
+                    }
+            }
+            __device__ __forceinline__ void apply(const float (&x)[H], v16f (&d)[MT]) const
+            {
+#pragma unroll
+                for (int st = 0; st < ST; ++st)
+                {
+                    bf16x8 xh, xl;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                    {
+                        const float v = x[8 * st + j];
+                        const __bf16 t = (__bf16)v;
+                        xh[j] = t;
+                        xl[j] = (__bf16)(v - (float)t);
+                    }
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                    {
+                        d[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo[mt][st], xh, d[mt], 0, 0, 0);
+                        d[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[mt][st], xl, d[mt], 0, 0, 0);
+                        d[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[mt][st], xh, d[mt], 0, 0, 0);
+                    }
+                }
+            }
+        };
+
         __device__ __forceinline__ uint64_t expand(uint64_t t, const Subspace &sub)
         {
             for (int k = 0; k < sub.nops; ++k)
@@ -40,7 +134,7 @@ namespace qsv
             return t | sub.set_bits;
         }
 
-        template <int KQ>
+        template <int KQ, bool X3>
         __global__ __launch_bounds__(kThreads) void k_block(c64 *__restrict__ s, uint64_t ncols, Subspace sub, BlockOp op)
         {
             constexpr int D = 2 << KQ;   // real dimension
@@ -53,21 +147,8 @@ namespace qsv
             const int h = lane >> 5;
 
             // rows 32*mt + c of R, columns [h*H, h*H + H): the A operands of every step
-            float a[MT][H];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-            {
-                const v4f *rp = reinterpret_cast<const v4f *>(op.R + (32 * mt + c) * D + h * H);
-#pragma unroll
-                for (int q = 0; q < H / 4; ++q)
-                {
-                    v4f v = rp[q];
-                    a[mt][4 * q + 0] = v.x;
-                    a[mt][4 * q + 1] = v.y;
-                    a[mt][4 * q + 2] = v.z;
-                    a[mt][4 * q + 3] = v.w;
-                }
-            }
+            BlockMatrix<KQ, X3> A;
+            A.load(op.R, c, h);
 
             const uint64_t ntiles = (ncols + 31) / 32;
             const uint64_t nwaves = uint64_t(gridDim.x) * kWaves;
@@ -89,11 +170,7 @@ namespace qsv
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
                     d[mt] = v16f{0.f};
-#pragma unroll
-                for (int st = 0; st < H; ++st)
-#pragma unroll
-                    for (int mt = 0; mt < MT; ++mt)
-                        d[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt][st], b[st], d[mt], 0, 0, 0);
+                A.apply(b, d);
                 if (!ok)
                     continue;
                 // accumulator register r of tile mt holds row 32*mt + 8*(r/4) + 4*h + r%4
@@ -124,7 +201,7 @@ namespace qsv
         // moved in address order -- lanes take consecutive positions, so each load/store
         // instruction covers contiguous runs -- and transposed through LDS into
         // [column][local amplitude] rows for the MFMA operands and back.
-        template <int KQ>
+        template <int KQ, bool X3>
         __global__ __launch_bounds__(kThreads) void k_block_lds(c64 *__restrict__ s, uint64_t ntiles, Subspace tsub, BlockLdsOp op)
         {
             constexpr int D = 2 << KQ;
@@ -143,21 +220,8 @@ namespace qsv
             const int h = lane >> 5;
             v2f *tl = lds[wave];
 
-            float a[MT][H];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-            {
-                const v4f *rp = reinterpret_cast<const v4f *>(op.R + (32 * mt + c) * D + h * H);
-#pragma unroll
-                for (int q = 0; q < H / 4; ++q)
-                {
-                    v4f v = rp[q];
-                    a[mt][4 * q + 0] = v.x;
-                    a[mt][4 * q + 1] = v.y;
-                    a[mt][4 * q + 2] = v.z;
-                    a[mt][4 * q + 3] = v.w;
-                }
-            }
+            BlockMatrix<KQ, X3> A;
+            A.load(op.R, c, h);
             // lane part of the position -> global offset and LDS index (tile bits 0..5)
             uint64_t g_lane = 0;
             int l_lane = 0;
@@ -209,11 +273,7 @@ namespace qsv
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
                     d[mt] = v16f{0.f};
-#pragma unroll
-                for (int st = 0; st < H; ++st)
-#pragma unroll
-                    for (int mt = 0; mt < MT; ++mt)
-                        d[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt][st], b_[st], d[mt], 0, 0, 0);
+                A.apply(b_, d);
                 wave_sync();
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
@@ -419,10 +479,14 @@ namespace qsv
         uint64_t grid = (ntiles + kWaves - 1) / kWaves;
         if (grid > 4096)
             grid = 4096;
-        if (op.kq == 5)
-            hipLaunchKernelGGL(k_block_lds<5>, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+        if (op.kq == 5 && op.x3)
+            hipLaunchKernelGGL((k_block_lds<5, true>), dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+        else if (op.kq == 5)
+            hipLaunchKernelGGL((k_block_lds<5, false>), dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+        else if (op.x3)
+            hipLaunchKernelGGL((k_block_lds<4, true>), dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
         else
-            hipLaunchKernelGGL(k_block_lds<4>, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+            hipLaunchKernelGGL((k_block_lds<4, false>), dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
         check(hipGetLastError(), "block kernel");
     }
 
@@ -445,10 +509,14 @@ namespace qsv
         uint64_t grid = (tiles + kWaves - 1) / kWaves;
         if (grid > 4096)
             grid = 4096;
-        if (op.kq == 5)
-            hipLaunchKernelGGL(k_block<5>, dim3(grid), dim3(kThreads), 0, st, state, ncols, sub, op);
+        if (op.kq == 5 && op.x3)
+            hipLaunchKernelGGL((k_block<5, true>), dim3(grid), dim3(kThreads), 0, st, state, ncols, sub, op);
+        else if (op.kq == 5)
+            hipLaunchKernelGGL((k_block<5, false>), dim3(grid), dim3(kThreads), 0, st, state, ncols, sub, op);
+        else if (op.x3)
+            hipLaunchKernelGGL((k_block<4, true>), dim3(grid), dim3(kThreads), 0, st, state, ncols, sub, op);
         else
-            hipLaunchKernelGGL(k_block<4>, dim3(grid), dim3(kThreads), 0, st, state, ncols, sub, op);
+            hipLaunchKernelGGL((k_block<4, false>), dim3(grid), dim3(kThreads), 0, st, state, ncols, sub, op);
         check(hipGetLastError(), "block kernel");
     }
 } // namespace qsv

@@ -6,10 +6,13 @@ namespace qsv
 {
     // One dense block on kq qubits: off[e] is the amplitude offset of local basis state e
     // (bit j of e = the block's j-th qubit); R points to the (2^(kq+1))^2 real matrix on
-    // interleaved (re, im) vectors, row-major, in device memory.
+    // interleaved (re, im) vectors, row-major, in device memory.  x3 = 0: R is FP32 and the
+    // product runs on FP32 MFMA; x3 = 1: R holds the bfloat16 high parts of the D*D entries
+    // followed by their bfloat16 low parts (same byte size), BF16x3 split-precision MFMA.
     struct BlockOp
     {
         int kq;
+        int x3;
         uint64_t off[32];
         const float *R;
     };
@@ -27,6 +30,7 @@ namespace qsv
     struct BlockLdsOp
     {
         int kq;
+        int x3; // as in BlockOp
         uint64_t gw[10];
         int lw[10];
         const float *R;

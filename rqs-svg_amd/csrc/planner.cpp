@@ -10,14 +10,14 @@
 namespace qcore
 {
     /* Features of one block launch:
-     *  0 n/32            1 kq/5             2 complex128 (1) or complex64 (0)
+     *  0 n/32            1 kq/5             2 math: complex128 1, complex64 BF16x3 0.5, FP32 0
      *  3 LDS-staged kernel (5 or 6 qubits outside the block)
      *  4..8 block qubits ascending / 32 (missing entries n/32)
      *  9 block qubits below 3 / 5        10 block qubits below 6 / 5
      * 11 controls outside the block / 4   12 gates in the block / 32
      * 13 (n - operands) / 32 = log2(columns)/32
      * 14 lowest qubit outside the block / 8   15 constant 1 */
-    void block_features(int n, bool f64, int kq, const BlockLayout &lay, int gates, float *f)
+    void block_features(int n, float math, int kq, const BlockLayout &lay, int gates, float *f)
     {
         std::vector<int> q = lay.qubits;
         std::sort(q.begin(), q.end());
@@ -30,8 +30,8 @@ namespace qcore
         const int free_q = n - (int)q.size() - outside;
         f[0] = n / 32.f;
         f[1] = kq / 5.f;
-        f[2] = f64 ? 1.f : 0.f;
-        f[3] = free_q >= (f64 ? 6 : 5) ? 1.f : 0.f;
+        f[2] = math;
+        f[3] = free_q >= (math == 1.f ? 6 : 5) ? 1.f : 0.f;
         for (int i = 0; i < 5; ++i)
             f[4 + i] = (i < (int)q.size() ? q[i] : n) / 32.f;
         int below3 = 0, below6 = 0;

@@ -14,7 +14,7 @@ namespace qcore
     constexpr int kPlanFeatures = 16;
 
     // Features of one block launch (all in [0, ~1]); see planner.cpp for the list.
-    void block_features(int n, bool f64, int kq, const BlockLayout &lay, int gates, float *f);
+    void block_features(int n, float math, int kq, const BlockLayout &lay, int gates, float *f);
 
     class CostModel
     {

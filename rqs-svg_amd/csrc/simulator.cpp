@@ -119,8 +119,8 @@ namespace qcore
     {
         if (allocated_)
             throw std::runtime_error("set_precision must be called before allocate_memory");
-        if (mode < 0 || mode > 2)
-            throw std::runtime_error("precision mode must be 0 (complex128 per gate), 1 (complex64 fused) or 2 (complex128 fused)");
+        if (mode < 0 || mode > 3)
+            throw std::runtime_error("precision mode must be 0 (complex128 per gate), 1 (complex64 fused, FP32), 2 (complex128 fused) or 3 (complex64 fused, BF16x3)");
         if (mode != 0 && (k < 1 || k > 5))
             throw std::runtime_error("fused block size must be in [1, 5]");
         precision_ = mode;
@@ -155,7 +155,7 @@ namespace qcore
             throw std::runtime_error(fmt("num_qubits %d out of range [1, %d]", nq, qsv::kMaxQubits));
         size_t free_b = 0, total_b = 0;
         qsv::check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-        const uint64_t S = precision_ == 1 ? 8 : 16;
+        const uint64_t S = c64_state() ? 8 : 16;
         int cap = 0;
         while ((uint64_t(1) << (cap + 1)) <= free_b / S)
             ++cap;
@@ -171,7 +171,7 @@ namespace qcore
         }
         else
         {
-            fused_ = new FusedEngine(*this, block_qubits_, precision_ == 2);
+            fused_ = new FusedEngine(*this, block_qubits_, precision_);
         }
         work_len_ = qsv::reduce_workspace(count) + 16;
         qsv::check(hipMalloc((void **)&work_, work_len_ * sizeof(double)), "hipMalloc workspace");
@@ -221,7 +221,7 @@ namespace qcore
         clear_measurement_state();
         if (fused_)
             fused_->drop(); // queued gates act on a state that is being overwritten
-        prof_begin(QCS_EXT_K_INIT, (precision_ == 1 ? 8.0 : 16.0) * (1ull << n_));
+        prof_begin(QCS_EXT_K_INIT, (c64_state() ? 8.0 : 16.0) * (1ull << n_));
         with_state([&](auto *st) { qsv::launch_fill_basis(stream_, st, uint64_t(1) << n_, 0); });
         prof_end();
     }
@@ -571,7 +571,7 @@ namespace qcore
             ops.push_back(l2p_[x]);
         auto sub = make_sub(ops, setb);
         const uint64_t items = uint64_t(1) << (n_ - sub.nops);
-        const double S = precision_ == 1 ? 8.0 : 16.0;
+        const double S = c64_state() ? 8.0 : 16.0;
         prof_begin(QCS_EXT_K_REDUCE, 2.0 * S * items);
         with_state([&](auto *st) { qsv::reduce_pair_probs(stream_, st, sub, uint64_t(1) << tq, items, work_, pinned_); });
         prof_end();
@@ -741,7 +741,7 @@ namespace qcore
         need_alloc();
         flush();
         ensure_levels();
-        prof_begin(QCS_EXT_K_REDUCE, (precision_ == 1 ? 8.0 : 16.0) * (1ull << n_));
+        prof_begin(QCS_EXT_K_REDUCE, (c64_state() ? 8.0 : 16.0) * (1ull << n_));
         with_state([&](auto *st) { qsv::build_fold_levels(stream_, st, uint64_t(1) << n_, levels_); });
         prof_end();
         qsv::check(hipMemcpyAsync(pinned_, levels_.back(), sizeof(double), hipMemcpyDeviceToHost, stream_), "copy");
@@ -780,7 +780,7 @@ namespace qcore
     {
         need_alloc();
         flush();
-        prof_begin(QCS_EXT_K_REDUCE, (precision_ == 1 ? 8.0 : 16.0) * (1ull << n_));
+        prof_begin(QCS_EXT_K_REDUCE, (c64_state() ? 8.0 : 16.0) * (1ull << n_));
         with_state([&](auto *st) { qsv::reduce_probs(stream_, st, uint64_t(1) << n_, 0, work_, pinned_); });
         prof_end();
         return pinned_[0];
@@ -794,7 +794,7 @@ namespace qcore
             throw std::runtime_error("expect_z count out of range");
         for (int q = 0; q < count; ++q)
         {
-            prof_begin(QCS_EXT_K_REDUCE, (precision_ == 1 ? 8.0 : 16.0) * (1ull << n_));
+            prof_begin(QCS_EXT_K_REDUCE, (c64_state() ? 8.0 : 16.0) * (1ull << n_));
             with_state([&](auto *st) { qsv::reduce_probs(stream_, st, uint64_t(1) << n_, uint64_t(1) << l2p_[q], work_, pinned_); });
             prof_end();
             out[q] = pinned_[0];
@@ -875,7 +875,7 @@ namespace qcore
     void Simulator::set_planner_model(CostModel *m)
     {
         if (!fused_)
-            throw std::runtime_error("the fusion planner needs precision mode 1 or 2 (allocated)");
+            throw std::runtime_error("the fusion planner needs a fused precision mode (1, 2 or 3, allocated)");
         flush();
         fused_->set_model(m);
     }
@@ -883,7 +883,7 @@ namespace qcore
     void Simulator::planner_stats(double out[5])
     {
         if (!fused_)
-            throw std::runtime_error("the fusion planner needs precision mode 1 or 2 (allocated)");
+            throw std::runtime_error("the fusion planner needs a fused precision mode (1, 2 or 3, allocated)");
         flush();
         fused_->planner_stats(out);
     }
@@ -891,7 +891,7 @@ namespace qcore
     void Simulator::planner_record(int count, uint64_t seed, float *features, float *log2_us)
     {
         if (!fused_)
-            throw std::runtime_error("recording block timings needs precision mode 1 or 2 (allocated)");
+            throw std::runtime_error("recording block timings needs a fused precision mode (1, 2 or 3, allocated)");
         if (count < 0 || (count > 0 && (!features || !log2_us)))
             throw std::runtime_error("planner_record: bad output buffers");
         fused_->record(count, seed, features, log2_us);

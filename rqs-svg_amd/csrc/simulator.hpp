@@ -114,6 +114,7 @@ namespace qcore
         int n_ = 0;
         int n_local_ = 0;
         int precision_ = 0;
+        bool c64_state() const { return precision_ == 1 || precision_ == 3; }
         int block_qubits_ = 5;
         qsv::c128 *state_ = nullptr; // complex128 state (precision 0)
         std::vector<int> p2l_, l2p_;

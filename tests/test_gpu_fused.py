@@ -1,4 +1,5 @@
-"""Precision form on the GPU: complex64 state, gates fused into k-qubit blocks on FP32 MFMA.
+"""Precision form on the GPU: complex64 state, gates fused into k-qubit blocks on FP32 MFMA
+(precision mode 1) or BF16x3 split-precision MFMA (mode 3); complex128 blocks in FP64 (mode 2).
 
 Bars: fidelity >= 1 - 1e-6 against the FP64 oracle (and against the GPU FP64 plain path at
 sizes the oracle does not reach); max|d amp| is reported and bounded by 1e-5 (FP32 level).
@@ -24,8 +25,8 @@ def say(msg):
     sys.stdout.flush()
 
 
-def fused_vs_oracle(gpu, oracle, n, circ, k, mapping=None, psi=None):
-    with pyqcs.Simulator(gpu, n, precision=1, block_qubits=k, mapping=mapping) as f:
+def fused_vs_oracle(gpu, oracle, n, circ, k, mapping=None, psi=None, precision=1):
+    with pyqcs.Simulator(gpu, n, precision=precision, block_qubits=k, mapping=mapping) as f:
         if psi is not None:
             f.set_statevector(psi)
         f.run(circ)
@@ -39,12 +40,13 @@ def fused_vs_oracle(gpu, oracle, n, circ, k, mapping=None, psi=None):
     return got, want, stats
 
 
+@pytest.mark.parametrize("precision", [1, 3])
 @pytest.mark.parametrize("k", [4, 5])
-def test_random_all_gates(gpu, oracle, k):
+def test_random_all_gates(gpu, oracle, k, precision):
     for n, seed in ((6, 1), (9, 2), (13, 3)):
         circ = circuits.random_all_gates(n, 200, seed=seed, max_ctrls=3)
-        say(f"fused random circuit n={n} k={k}")
-        got, want, stats = fused_vs_oracle(gpu, oracle, n, circ, k)
+        say(f"fused random circuit n={n} k={k} precision={precision}")
+        got, want, stats = fused_vs_oracle(gpu, oracle, n, circ, k, precision=precision)
         fid = fidelity(got, want)
         err = np.max(np.abs(got - want))
         say(f"  fidelity {fid:.12f} max|d| {err:.2e} gates {stats[0]} blocks {stats[1]}")
@@ -52,7 +54,8 @@ def test_random_all_gates(gpu, oracle, k):
         assert stats[0] == len(circ) and stats[1] <= len(circ)
 
 
-def test_each_gate_single(gpu, oracle):
+@pytest.mark.parametrize("precision", [1, 3])
+def test_each_gate_single(gpu, oracle, precision):
     n = 8
     for name in sorted(pyqcs.GATE_TARGETS):
         rng = np.random.default_rng(len(name))
@@ -62,7 +65,7 @@ def test_each_gate_single(gpu, oracle):
         k = 1 if name == "global_phase" else pyqcs.GATE_PARAMS[name]
         params = tuple(rng.uniform(-math.pi, math.pi, size=k))
         psi = random_state(n, 5)
-        got, want, _ = fused_vs_oracle(gpu, oracle, n, [(name, targets, params, ctl[:1], ctl[1:])], 5, psi=psi)
+        got, want, _ = fused_vs_oracle(gpu, oracle, n, [(name, targets, params, ctl[:1], ctl[1:])], 5, psi=psi, precision=precision)
         assert np.max(np.abs(got - want)) <= AMP_TOL, name
 
 
@@ -75,15 +78,17 @@ def test_wide_gates(gpu, oracle):
              ("u3", (7,), (0.3, 0.2, 0.1), (1, 2, 3, 4), (5, 6)),
              ("swap", (2, 6), (), (0, 1, 3), ())]
     for k in (4, 5):
-        got, want, stats = fused_vs_oracle(gpu, oracle, n, circ, k)
-        assert np.max(np.abs(got - want)) <= AMP_TOL
+        for precision in (1, 3):
+            got, want, stats = fused_vs_oracle(gpu, oracle, n, circ, k, precision=precision)
+            assert np.max(np.abs(got - want)) <= AMP_TOL
 
 
-def test_brick_mapping_and_fusion_counts(gpu, oracle):
+@pytest.mark.parametrize("precision", [1, 3])
+def test_brick_mapping_and_fusion_counts(gpu, oracle, precision):
     n = 14
     circ = circuits.brick(n, 6, seed=4, mcx=2)
     mapping = [int(q) for q in np.random.default_rng(3).permutation(n)]
-    got, want, stats = fused_vs_oracle(gpu, oracle, n, circ, 5, mapping=mapping)
+    got, want, stats = fused_vs_oracle(gpu, oracle, n, circ, 5, mapping=mapping, precision=precision)
     assert fidelity(got, want) >= 1 - FID_TOL
     # 6 layers x (14 U3 + 6-7 CZ) + 2 MCX fuse into far fewer sweeps
     assert stats[1] * 3 <= stats[0]
@@ -129,11 +134,12 @@ def test_window_in_fused_mode(gpu, oracle):
     assert fidelity(res[0][1], res[1][1]) >= 1 - FID_TOL
 
 
-def test_overlap_with_plain_gpu_path(gpu):
+@pytest.mark.parametrize("precision", [1, 3])
+def test_overlap_with_plain_gpu_path(gpu, precision):
     n = 22
     circ = circuits.brick(n, 10, seed=7, mcx=4)
-    say("overlap fused vs plain n=22")
-    with pyqcs.Simulator(gpu, n) as plain, pyqcs.Simulator(gpu, n, precision=1) as fused:
+    say(f"overlap fused (precision {precision}) vs plain n=22")
+    with pyqcs.Simulator(gpu, n) as plain, pyqcs.Simulator(gpu, n, precision=precision) as fused:
         plain.run(circ)
         fused.run(circ)
         ov, md = fused.overlap(plain)
@@ -151,6 +157,10 @@ def test_precision_mode_errors(gpu):
     sim.close()
     with pyqcs.Simulator(gpu, 6) as sim:
         assert sim.lib.qcs_ext_set_precision(sim.h, 1, 5) == 0  # only before allocate_memory
+    for mode, k in ((4, 5), (-1, 5), (3, 6), (3, 0)):
+        sim = pyqcs.Simulator(gpu, 6, allocate=False)
+        assert sim.lib.qcs_ext_set_precision(sim.h, mode, k) == 0
+        sim.close()
 
 
 def test_fused_complex128_matches_oracle(gpu, oracle):
