@@ -383,9 +383,10 @@ namespace qcore
             qsv::check(hipEventCreate(&e), "hipEventCreate");
         try
         {
-            for (int start = -3; start < count; start += chunk)
+            // start = -3: three untimed warm-up launches, then rows [0, count) in chunks
+            for (int start = -3; start < count; start = start < 0 ? 0 : start + chunk)
             {
-                const int m = start < 0 ? 3 : std::min(chunk, count - start); // 3 warm-up launches first
+                const int m = start < 0 ? 3 : std::min(chunk, count - start);
                 std::vector<std::vector<ResolvedGate>> gate_sets(m);
                 std::vector<PlannedBlock> blocks(m);
                 std::vector<float> mats;

@@ -31,6 +31,9 @@ namespace qsv
 
         constexpr int kWaves = 4;
         constexpr int kThreads = 64 * kWaves;
+#ifndef QSV_BLOCK_PREFETCH
+#define QSV_BLOCK_PREFETCH 1
+#endif
 
         typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -234,10 +237,10 @@ namespace qsv
                 }
 
             const uint64_t nwaves = uint64_t(gridDim.x) * kWaves;
-            for (uint64_t tile = uint64_t(blockIdx.x) * kWaves + wave; tile < ntiles; tile += nwaves)
-            {
+            // The next tile's loads are issued before this tile's MFMA work and stores, so
+            // each wave keeps a tile of reads in flight while it computes.
+            auto load_tile = [&](uint64_t tile, v2f (&v)[NJ]) {
                 const uint64_t base = expand(tile, tsub) + g_lane;
-                v2f v[NJ];
 #pragma unroll
                 for (int j = 0; j < NJ; ++j)
                 {
@@ -248,6 +251,14 @@ namespace qsv
                             gj += op.gw[b];
                     v[j] = __builtin_nontemporal_load(reinterpret_cast<const v2f *>(s + (base + gj)));
                 }
+            };
+            v2f v[NJ];
+            uint64_t tile = uint64_t(blockIdx.x) * kWaves + wave;
+            if (tile < ntiles)
+                load_tile(tile, v);
+            for (; tile < ntiles; tile += nwaves)
+            {
+                const uint64_t base = expand(tile, tsub) + g_lane;
 #pragma unroll
                 for (int j = 0; j < NJ; ++j)
                 {
@@ -259,6 +270,10 @@ namespace qsv
                     tl[l_lane + lj] = v[j];
                 }
                 wave_sync();
+#if QSV_BLOCK_PREFETCH
+                if (tile + nwaves < ntiles)
+                    load_tile(tile + nwaves, v);
+#endif
                 float b_[H];
 #pragma unroll
                 for (int r = 0; r < AMPS / 2; ++r)
@@ -299,6 +314,10 @@ namespace qsv
                     __builtin_nontemporal_store(tl[l_lane + lj], reinterpret_cast<v2f *>(s + (base + gj)));
                 }
                 wave_sync();
+#if !QSV_BLOCK_PREFETCH
+                if (tile + nwaves < ntiles)
+                    load_tile(tile + nwaves, v);
+#endif
             }
         }
         // complex128 block on 4 qubits.  Tile = 16 amplitudes x 64 columns (tile bits = the 4

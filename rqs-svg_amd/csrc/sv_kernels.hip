@@ -75,6 +75,14 @@ namespace qsv
         // p(i) = fma(re, re, im*im), the probability definition shared with the oracle.
         __device__ __forceinline__ double prob(v2d a) { return __builtin_fma(a.x, a.x, a.y * a.y); }
 
+        // Orders this wave's LDS accesses against each other (lanes exchange data through LDS).
+        __device__ __forceinline__ void wave_sync()
+        {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+
         typedef float v2f __attribute__((ext_vector_type(2)));
 
         // Amplitude load/store in FP64 registers for both storage types; complex64 values are
@@ -402,36 +410,57 @@ namespace qsv
         }
 
         // ---------------- sampling ----------------
-        // One wave folds 64 consecutive blocks of 64 entries (left to right).
+        // One wave folds 64 consecutive blocks of 64 entries, each left to right: output
+        // entry 64*g + L = (((x0 + x1) + x2) + ... + x63) over entries 4096*g + 64*L + 0..63.
+        // The 4096 entries go through LDS in 4 chunks of 16 columns, so that every load
+        // instruction covers runs of 16 consecutive entries (256 B of complex128 state) and
+        // lane L then continues its own row's sum over the chunk in order.  4 waves per
+        // workgroup, 8.5 KiB of LDS per wave.
+        constexpr int kFoldWaves = 4;
+        constexpr int kFoldRow = 17; // doubles per LDS row (16 + 1 of padding)
+
         template <bool FROM_STATE, class A>
-        __global__ __launch_bounds__(64) void k_fold64(const void *__restrict__ src, uint64_t count, double *__restrict__ out, uint64_t nout)
+        __global__ __launch_bounds__(64 * kFoldWaves) void k_fold64(const void *__restrict__ src, uint64_t count, double *__restrict__ out, uint64_t nout, uint64_t ngroups)
         {
-            __shared__ double tile[64][65];
-            const uint64_t group = blockIdx.x;
-            const int lane = threadIdx.x;
-#pragma unroll 8
-            for (int j = 0; j < 64; ++j)
+            __shared__ double tiles[kFoldWaves][64 * kFoldRow];
+            const int lane = threadIdx.x & 63;
+            const int wave = threadIdx.x >> 6;
+            const uint64_t group = uint64_t(blockIdx.x) * kFoldWaves + wave;
+            if (group >= ngroups) // whole wave leaves together; no workgroup barrier below
+                return;
+            double *tl = tiles[wave];
+            const int run = lane >> 4, col = lane & 15;
+            double acc = 0.0;
+#pragma unroll
+            for (int chunk = 0; chunk < 4; ++chunk)
             {
-                uint64_t e = group * 4096 + uint64_t(j) * 64 + lane;
-                double v = 0.0;
-                if (e < count)
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
                 {
-                    if (FROM_STATE)
-                        v = prob(lda(static_cast<const A *>(src) + e));
-                    else
-                        v = static_cast<const double *>(src)[e];
+                    const int r = 4 * j + run;
+                    const uint64_t e = group * 4096 + uint64_t(r) * 64 + 16 * chunk + col;
+                    double v = 0.0;
+                    if (e < count)
+                    {
+                        if (FROM_STATE)
+                            v = prob(lda(static_cast<const A *>(src) + e));
+                        else
+                            v = static_cast<const double *>(src)[e];
+                    }
+                    tl[r * kFoldRow + col] = v;
                 }
-                tile[j][lane] = v;
+                wave_sync();
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                {
+                    const double x = tl[lane * kFoldRow + c];
+                    acc = (chunk == 0 && c == 0) ? x : acc + x;
+                }
+                wave_sync();
             }
-            __syncthreads();
-            uint64_t ob = group * 64 + lane;
+            const uint64_t ob = group * 64 + lane;
             if (ob < nout)
-            {
-                double acc = tile[lane][0];
-                for (int e = 1; e < 64; ++e)
-                    acc = acc + tile[lane][e];
                 out[ob] = acc;
-            }
         }
 
         struct Levels
@@ -835,12 +864,13 @@ namespace qsv
             throw std::runtime_error("fold: level buffer count mismatch");
         for (size_t lev = 1; lev < sz.size(); ++lev)
         {
-            uint64_t nout = sz[lev];
-            uint64_t groups = (nout + 63) / 64;
+            const uint64_t nout = sz[lev];
+            const uint64_t groups = (nout + 63) / 64;
+            const uint64_t blocks = (groups + kFoldWaves - 1) / kFoldWaves;
             if (lev == 1)
-                hipLaunchKernelGGL((k_fold64<true, A>), dim3(groups), dim3(64), 0, s, static_cast<const void *>(state), sz[0], levels[0], nout);
+                hipLaunchKernelGGL((k_fold64<true, A>), dim3(blocks), dim3(64 * kFoldWaves), 0, s, static_cast<const void *>(state), sz[0], levels[0], nout, groups);
             else
-                hipLaunchKernelGGL((k_fold64<false, A>), dim3(groups), dim3(64), 0, s, static_cast<const void *>(levels[lev - 2]), sz[lev - 1], levels[lev - 1], nout);
+                hipLaunchKernelGGL((k_fold64<false, A>), dim3(blocks), dim3(64 * kFoldWaves), 0, s, static_cast<const void *>(levels[lev - 2]), sz[lev - 1], levels[lev - 1], nout, groups);
             after_launch("fold kernel");
         }
     }
