@@ -153,8 +153,9 @@ def fused_leg(n, depth, k, shots, modes=(1, 3)):
 
 def batch_leg(count, seed=20250404):
     """512 variational circuits (12-22 qubits, 4-16 layers of RY/RZ + CNOT ladder): one
-    autotuned hipGraph launch (complex64 states, FP64 <Z> reductions) against a per-circuit
-    plain-method loop (complex128, one kernel per gate, one simulator per qubit count)."""
+    autotuned hipGraph launch (complex64 states, FP64 <Z> reductions) against the per-circuit
+    plain-method loop in C++ (complex128, one kernel per gate, one simulator per qubit
+    count; qcs_ext_batch_run_plain)."""
     lib = pyqcs.gpu_library()
     work = circuits.variational_batch(count, seed=seed)
     gates = sum(len(g) for _, g in work)
@@ -176,19 +177,11 @@ def batch_leg(count, seed=20250404):
     dev = min(r[1] for r in runs)
     log(f"batch leg: {count} circuits, {gates} gates, graph {dev * 1e3:.2f} ms device "
         f"({wall * 1e3:.2f} ms wall), tuning {tuning} found in {t_tune:.1f}s")
-    sims = {}
-    z_plain = []
-    t0 = time.perf_counter()
+    b = pyqcs.Batch(lib)
     for n, g in work:
-        if n not in sims:
-            sims[n] = pyqcs.Simulator(lib, n, seed=1)
-        s = sims[n]
-        s.call("set_zero_state")
-        s.run(circuits.batch_to_circuit(g))
-        z_plain.append(s.expect_z())
-    t_plain = time.perf_counter() - t0
-    for s in sims.values():
-        s.close()
+        b.add(n, g)
+    z_plain, t_plain = b.run_plain()
+    b.close()
     err = max(float(abs(x - y).max()) for x, y in zip(z_batch, z_plain))
     log(f"batch leg: plain loop {t_plain:.3f}s, max |d<Z>| {err:.2e}")
     return {"config": f"{count} variational circuits, 12-22 qubits, 4-16 layers RY+RZ on every "

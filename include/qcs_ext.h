@@ -135,6 +135,11 @@ int qcs_ext_batch_autotune(qcs_ext_batch *batch, int *tuning);
 /* expz receives sum(num_qubits) doubles, circuit by circuit; seconds = device time. */
 int qcs_ext_batch_run(qcs_ext_batch *batch, double *expz, uint64_t count, double *seconds);
 int qcs_ext_batch_info(qcs_ext_batch *batch, int *num_circuits, int *total_qubits, uint64_t *graph_nodes);
+/* The same batch by the plain method, circuit after circuit (complex128, one kernel per
+ * gate, one simulator per distinct qubit count, in C++): the reference's way of running
+ * such a workload, as a baseline.  Same outputs as qcs_ext_batch_run (up to the complex64
+ * rounding of the batched form); seconds = host wall time of the whole loop. */
+int qcs_ext_batch_run_plain(qcs_ext_batch *batch, double *expz, uint64_t count, double *seconds);
 
 /* Learned fusion planner.  A cost model is a 3-layer MLP, 64 units wide (F -> 64 -> 64 ->
  * 1, ReLU, FP32) with parameters W1[64][F], b1[64], W2[64][64], b2[64], w3[64], b3.

@@ -6,10 +6,13 @@
 // the block sequences spread over several streams, the <Z> reductions -- is captured once
 // into a hipGraph and replayed with a single launch.
 #include "batch.hpp"
+#include "simulator.hpp"
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <map>
+#include <memory>
 #include <stdexcept>
 
 namespace qcore
@@ -314,6 +317,36 @@ namespace qcore
         if (expz)
             qsv::check(hipMemcpy(expz, d_out_, total_qubits_ * sizeof(double), hipMemcpyDeviceToHost), "batch copy");
         return ms * 1e-3;
+    }
+
+    double Batch::run_plain(double *expz)
+    {
+        std::map<int, std::unique_ptr<Simulator>> sims;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (const auto &c : circuits_)
+        {
+            auto &sim = sims[c.n];
+            if (!sim)
+            {
+                sim = std::make_unique<Simulator>();
+                sim->num_qubits_cfg = c.n;
+                sim->allocate();
+            }
+            else
+                sim->set_zero_state();
+            for (const auto &g : c.gates)
+            {
+                std::vector<int> pos;
+                if (g.control >= 0)
+                    pos.push_back(g.control);
+                sim->operate(make_gate(g.name.c_str(), g.params), {g.target}, {}, pos);
+            }
+            std::vector<double> z(c.n);
+            sim->expect_z(z.data(), c.n);
+            if (expz)
+                std::copy(z.begin(), z.end(), expz + c.out_offset);
+        }
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
 
     BatchTuning Batch::autotune()

@@ -43,6 +43,9 @@ namespace qcore
         // Runs every circuit from |0...0>, writes <Z_q> of circuit c at offset
         // sum_{c' < c} n_c' + q; returns seconds between graph launch and completion.
         double run(double *expz);
+        // The plain method per circuit (complex128 simulators, one kernel per gate); returns
+        // host wall seconds of the loop.
+        double run_plain(double *expz);
         // Coordinate search over the tuning parameters on this batch; keeps the best.
         BatchTuning autotune();
         uint64_t graph_nodes() const { return graph_nodes_; }

@@ -26,6 +26,8 @@ def test_oracle_batch_matches_textbook(oracle):
     for (n, g), z in zip(work, res):
         assert np.max(np.abs(z - expz_textbook(n, g))) < 1e-13
     assert b.info()["circuits"] == len(work)
+    again, _ = b.run_plain()  # the oracle's batch run is the plain method already
+    assert all(np.array_equal(x, y) for x, y in zip(res, again))
 
 
 def test_batch_rejects_bad_input(oracle):
@@ -74,3 +76,19 @@ def test_gpu_batch_autotune(gpu):
     assert t["block_qubits"] in (4, 5) and t["streams"] in (1, 2, 4)
     after, _ = b.run()
     assert max(np.max(np.abs(x - y)) for x, y in zip(before, after)) < 2e-5
+
+
+@pytest.mark.gpu
+def test_gpu_plain_loop_matches_oracle(gpu, oracle):
+    work = circuits.variational_batch(12, seed=8, nmin=3, nmax=12, dmin=1, dmax=4)
+    ref = pyqcs.Batch(oracle)
+    b = pyqcs.Batch(gpu)
+    for n, g in work:
+        ref.add(n, g)
+        b.add(n, g)
+    want, _ = ref.run()
+    got, secs = b.run_plain()
+    assert secs > 0
+    assert max(np.max(np.abs(x - y)) for x, y in zip(got, want)) < 1e-12
+    fast, _ = b.run()
+    assert max(np.max(np.abs(x - y)) for x, y in zip(fast, want)) < 2e-5
