@@ -89,9 +89,19 @@ def cpu_baseline(n, circ, budget_s=20.0):
     }
 
 
-def fused_leg(n, depth, k, shots, modes=(1, 3)):
+FUSED_VARIANTS = (
+    # (result key, precision mode, stage tile bits)
+    ("fused_c64", 1, 13),
+    ("fused_c64_bf16x3", 3, 13),
+    ("fused_c64_bf16x3_per_block", 3, 0),
+)
+
+
+def fused_leg(n, depth, k, shots, variants=FUSED_VARIANTS):
     """Config: 32 qubits, brick depth 32; fused complex64 vs plain complex128 on one GPU.
-    Precision mode 1 runs the blocks on FP32 MFMA, mode 3 on BF16x3 split-precision MFMA."""
+    Precision mode 1 runs the blocks on FP32 MFMA, mode 3 on BF16x3 split-precision MFMA;
+    stage tile bits 13 apply several blocks per sweep on LDS-resident tiles of 2^13
+    amplitudes, 0 launches one sweep per block."""
     lib = pyqcs.gpu_library()
     circ = circuits.brick(n, depth, seed=CIRCUIT_SEED + 3)
     warm = circuits.brick(n, 1, seed=1)[: n + 2]
@@ -105,12 +115,14 @@ def fused_leg(n, depth, k, shots, modes=(1, 3)):
     t_plain = time.perf_counter() - t0
     log(f"fused leg: plain c128 {t_plain:.3f}s")
     out = {}
-    for mode in modes:
+    for key, mode, stage_bits in variants:
         math_name = {1: "FP32 MFMA", 3: "BF16x3 split-precision MFMA"}[mode]
+        how = f"stages on 2^{stage_bits}-amplitude LDS tiles" if stage_bits else "one sweep per block"
         res = {"config": f"brick{n}: {n} qubits, {depth} brick layers (U3 all + CZ even/odd), "
-                         f"{len(circ)} gates, {shots} samples; complex64 state, k={k} blocks on {math_name}",
-               "gates": len(circ), "block_qubits": k, "precision_mode": mode}
-        fused = pyqcs.Simulator(lib, n, seed=1, precision=mode, block_qubits=k)
+                         f"{len(circ)} gates, {shots} samples; complex64 state, k={k} blocks on "
+                         f"{math_name}, {how}",
+               "gates": len(circ), "block_qubits": k, "precision_mode": mode, "stage_bits": stage_bits}
+        fused = pyqcs.Simulator(lib, n, seed=1, precision=mode, block_qubits=k, stage_bits=stage_bits)
         fused.run(warm)
         fused.call("set_zero_state")
         fused.norm()
@@ -130,23 +142,22 @@ def fused_leg(n, depth, k, shots, modes=(1, 3)):
         ov, maxdiff = fused.overlap(plain)
         fid = abs(ov) ** 2 / (fused.norm() * plain.norm())
         fused.close()
-        log(f"fused leg: mode {mode} {t_fused:.3f}s ({b1 - b0} blocks), fidelity {fid:.12f}, max|d| {maxdiff:.2e}")
+        log(f"fused leg: {key} {t_fused:.3f}s ({b1 - b0} sweeps), fidelity {fid:.12f}, max|d| {maxdiff:.2e}")
         res.update({"plain_c128_s": t_plain, "fused_c64_s": t_fused, "speedup": t_plain / t_fused,
-                    "blocks": b1 - b0,
+                    "sweeps": b1 - b0,
                     "fidelity": fid, "infidelity": 1 - fid, "max_abs_diff": maxdiff,
                     "probability_total": total, "sample_s": t_sample})
-        # block kernel: algorithmic bytes 2 * 8 B * amplitudes per launch; the real-ified
-        # 2^(k+1) x 2^(k+1) product costs 8 * 2^k flops per amplitude
+        # fused kernel: algorithmic bytes = one read and one write of the touched amplitudes
+        # per launch (2 * 8 B * 2^n for a stage); each block costs 8 * 2^k flops per amplitude
         gbs = blk["bytes"] / blk["seconds"] / 1e9
-        tflops = blk["bytes"] / 16 * 8 * (1 << k) / blk["seconds"] / 1e12
-        res["roofline"] = {"bound": "hbm", "kernel": f"k_block_lds<{k}, {'true' if mode == 3 else 'false'}>",
+        x3 = "true" if mode == 3 else "false"
+        kern = f"k_stage<{k}, {stage_bits}, {x3}>" if stage_bits else f"k_block_lds<{k}, {x3}>"
+        res["roofline"] = {"bound": "hbm", "kernel": kern,
                            "achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                            "frac": gbs / (HBM_PEAK / 1e9), "launches": blk["launches"],
-                           "avg_ms": blk["seconds"] / blk["launches"] * 1e3,
-                           "mfma_tflops": tflops,
-                           "mfma_peak_tflops": 157.3 if mode == 1 else 2500.0 / 3}
-        log(f"fused leg: block kernel {gbs:.1f} GB/s, {tflops:.1f} TFLOP/s over {blk['launches']} launches")
-        out["fused_c64" if mode == 1 else "fused_c64_bf16x3"] = res
+                           "avg_ms": blk["seconds"] / blk["launches"] * 1e3}
+        log(f"fused leg: {kern} {gbs:.1f} GB/s over {blk['launches']} launches")
+        out[key] = res
     plain.close()
     return out
 
@@ -201,10 +212,13 @@ def planner_leg(n, record_qubits, record_count, depth, k):
     lib = pyqcs.gpu_library()
     Xs, ys = [], []
     t0 = time.perf_counter()
-    per = max(1, record_count // len(record_qubits))
-    for i, m in enumerate(record_qubits):
+    # blocks per size fall with the state size (a 32-qubit complex128 block launch streams
+    # 128 GiB) so that recording stays within tens of seconds
+    weights = [0.5 ** i for i in range(len(record_qubits))]
+    counts = [max(1, int(record_count * w / sum(weights))) for w in weights]
+    for i, (m, cnt) in enumerate(zip(record_qubits, counts)):
         with pyqcs.Simulator(lib, m, precision=2, block_qubits=k) as s:
-            X, y = pyqcs.planner_record(s, per, seed=77 + i)
+            X, y = pyqcs.planner_record(s, cnt, seed=77 + i)
         Xs.append(X)
         ys.append(y)
     X = np.concatenate(Xs)
@@ -221,7 +235,9 @@ def planner_leg(n, record_qubits, record_count, depth, k):
                      f"{min(record_qubits)}-{max(record_qubits)} qubits",
            "recorded_blocks": int(len(y)), "record_s": t_rec, "train_s": t_train,
            "train_mse_log2us": float(loss), "target_variance": float(np.var(y))}
-    sim = pyqcs.Simulator(lib, n, seed=1, precision=2, block_qubits=k)
+    # the learned choice is among one-sweep-per-block plans; the stage kernel's time on the
+    # same circuits is reported beside it
+    sim = pyqcs.Simulator(lib, n, seed=1, precision=2, block_qubits=k, stage_bits=0)
     try:
         sim.run(circuits.brick(n, 1, seed=1)[: n + 2])
         for name, circ in (("qft", circuits.qft(n)), ("brick", circuits.brick(n, depth, seed=CIRCUIT_SEED + 5))):
@@ -247,6 +263,20 @@ def planner_leg(n, record_qubits, record_count, depth, k):
                     res["pred_chosen_us"] = st["pred_chosen_us"]
                 log(f"planner leg: {name} {mode}: {dt:.3f}s, {b1 - b0} blocks")
             res["speedup"] = res["greedy_s"] / res["learned_s"]
+            sim.set_stage_bits(12)
+            sim.call("set_zero_state")
+            sim.norm()
+            _, b0 = sim.fusion_stats()
+            t0 = time.perf_counter()
+            sim.run(circ)
+            sim.probability_total()
+            res["stages_s"] = time.perf_counter() - t0
+            _, b1 = sim.fusion_stats()
+            res["stages_sweeps"] = b1 - b0
+            zst = sim.expect_z()
+            res["stages_max_abs_diff_expz"] = float(np.max(np.abs(zst - zs["greedy"])))
+            sim.set_stage_bits(0)
+            log(f"planner leg: {name} stages: {res['stages_s']:.3f}s, {b1 - b0} sweeps")
             res["max_abs_diff_expz"] = float(np.max(np.abs(zs["greedy"] - zs["learned"])))
             out[name] = res
     finally:
@@ -277,7 +307,8 @@ def main():
     ap.add_argument("--planner-depth", type=int, default=40)
     ap.add_argument("--planner-k", type=int, default=4)
     ap.add_argument("--planner-record", type=int, default=10000,
-                    help="blocks timed for the cost model, spread over 26, 28, 30 and 32 qubits")
+                    help="blocks timed for the cost model over 26, 28, 30 and 32 qubits "
+                         "(8:4:2:1)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one write-only fill and one read-only norm pass of the "
                          "full state (known HBM bytes, for scaling PMC byte counters)")

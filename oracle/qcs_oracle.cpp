@@ -1384,6 +1384,14 @@ extern "C"
             c.l2p = wl2p;
         });
     }
+    int qcs_ext_set_stage_bits(qcs_simulator *s, int m)
+    { // the oracle has no fused path; the setting is validated and ignored
+        return guard([&] {
+            (void)core(s);
+            if (m != 0 && m != 12 && m != 13)
+                throw std::runtime_error("stage tile bits must be 0, 12 or 13");
+        });
+    }
     int qcs_ext_set_precision(qcs_simulator *s, int mode, int k)
     {
         return guard([&] {

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <stdexcept>
 
 namespace qcore
@@ -306,6 +307,11 @@ namespace qcore
             return;
         std::vector<ResolvedGate> gates;
         gates.swap(queue_);
+        if (uses_stages())
+        {
+            apply_stages(gates);
+            return;
+        }
         auto plan = plan_greedy(gates, k_);
         if (model_)
         {
@@ -693,6 +699,324 @@ namespace qcore
     {
         b.op.M = mats_dev + b.matrix_offset;
         qsv::launch_block128(s, state, n, b.sub, b.op, b.staged);
+    }
+
+    std::vector<PlannedStage> plan_stages(const std::vector<ResolvedGate> &gates, int n, int m, int k)
+    {
+        if (m < 8 || m > n)
+            throw std::runtime_error("stages: tile bits out of range");
+        std::vector<PlannedStage> out;
+        std::vector<int> rem(gates.size());
+        std::iota(rem.begin(), rem.end(), 0);
+        while (!rem.empty())
+        {
+            std::vector<char> in_tile(n, 0), blocked(n, 0);
+            int tile_size = 0;
+            for (int q = 0; q < 4; ++q)
+                in_tile[q] = 1, ++tile_size;
+            PlannedStage st;
+            std::vector<int> deferred;
+            for (int gi : rem)
+            {
+                const auto qs = gate_qubits(gates[gi]);
+                bool take = (int)qs.size() <= k;
+                int grow = 0;
+                for (int q : qs)
+                {
+                    if (blocked[q])
+                        take = false;
+                    grow += in_tile[q] ? 0 : 1;
+                }
+                if (take && tile_size + grow > m)
+                    take = false;
+                if (take)
+                {
+                    for (int q : qs)
+                        if (!in_tile[q])
+                            in_tile[q] = 1, ++tile_size;
+                    st.gates.push_back(gi);
+                }
+                else
+                {
+                    deferred.push_back(gi);
+                    for (int q : qs)
+                        blocked[q] = 1;
+                }
+            }
+            if (st.gates.empty())
+            { // the first remaining gate is wider than k: it runs on its own
+                st.gates.push_back(deferred.front());
+                deferred.erase(deferred.begin());
+            }
+            else
+            {
+                // pad the tile with the lowest free qubits (longer contiguous runs)
+                for (int q = 4; q < n && tile_size < m; ++q)
+                    if (!in_tile[q])
+                        in_tile[q] = 1, ++tile_size;
+                for (int q = 0; q < n; ++q)
+                    if (in_tile[q])
+                        st.tile.push_back(q);
+            }
+            out.push_back(std::move(st));
+            rem.swap(deferred);
+        }
+        return out;
+    }
+
+    PreparedStage prepare_stage(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int kq, int m, int n, std::vector<float> &mats, bool x3)
+    {
+        if ((int)st.tile.size() != m)
+            throw std::runtime_error("stage: tile size mismatch");
+        std::vector<ResolvedGate> sub;
+        sub.reserve(st.gates.size());
+        for (int gi : st.gates)
+            sub.push_back(gates[gi]);
+        const auto plan = plan_greedy(sub, kq);
+        const int D = 2 << kq;
+        const size_t per = size_t(D) * D;
+        const int K = 1 << kq;
+        std::vector<qsv::StageBlock> desc;
+        for (const auto &pb : plan)
+        {
+            if (pb.wide)
+                throw std::runtime_error("stage: wide gate inside a stage");
+            BlockLayout lay;
+            lay.qubits = pb.qubits;
+            // pad with the highest tile qubits: low tile bits stay column bits
+            for (int i = m - 1; i >= 0 && (int)lay.qubits.size() < kq; --i)
+                if (local_pos(lay.qubits, st.tile[i]) < 0)
+                    lay.qubits.push_back(st.tile[i]);
+            const auto M = block_matrix(sub, pb, lay);
+            std::vector<double> R(per);
+            for (int r = 0; r < K; ++r)
+                for (int c = 0; c < K; ++c)
+                {
+                    const cd z = M[size_t(r) * K + c];
+                    R[(2 * r) * D + 2 * c] = z.real();
+                    R[(2 * r) * D + 2 * c + 1] = -z.imag();
+                    R[(2 * r + 1) * D + 2 * c] = z.imag();
+                    R[(2 * r + 1) * D + 2 * c + 1] = z.real();
+                }
+            qsv::StageBlock sb{};
+            sb.matrix = uint32_t(mats.size());
+            mats.resize(mats.size() + per);
+            if (x3)
+            {
+                std::vector<uint16_t> parts(2 * per);
+                split_bf16(R.data(), per, parts.data(), parts.data() + per);
+                std::memcpy(mats.data() + sb.matrix, parts.data(), 2 * per * sizeof(uint16_t));
+            }
+            else
+            {
+                for (size_t i = 0; i < per; ++i)
+                    mats[sb.matrix + i] = float(R[i]);
+            }
+            std::vector<char> is_block(m, 0);
+            for (int j = 0; j < kq; ++j)
+            {
+                const int b = local_pos(st.tile, lay.qubits[j]);
+                if (b < 0)
+                    throw std::runtime_error("stage: block qubit outside the tile");
+                sb.w[j] = 1 << b;
+                is_block[b] = 1;
+            }
+            int j = kq;
+            for (int b = 0; b < m; ++b)
+                if (!is_block[b])
+                    sb.w[j++] = 1 << b;
+            desc.push_back(sb);
+        }
+        static_assert(sizeof(qsv::StageBlock) % 16 == 0, "StageBlock keeps 16-B alignment");
+        PreparedStage p;
+        p.gates = (int)st.gates.size();
+        p.nblocks = (int)desc.size();
+        p.bytes = 2.0 * 8.0 * double(uint64_t(1) << n);
+        while (mats.size() % 4)
+            mats.push_back(0.f);
+        p.desc_offset = mats.size();
+        mats.resize(mats.size() + desc.size() * sizeof(qsv::StageBlock) / sizeof(float));
+        std::memcpy(mats.data() + p.desc_offset, desc.data(), desc.size() * sizeof(qsv::StageBlock));
+        p.tsub.set_bits = 0;
+        p.tsub.nops = m;
+        p.op.kq = kq;
+        p.op.m = m;
+        p.op.x3 = x3 ? 1 : 0;
+        p.op.nblocks = p.nblocks;
+        for (int b = 0; b < m; ++b)
+        {
+            p.tsub.ops[b] = st.tile[b];
+            p.op.gw[b] = uint64_t(1) << st.tile[b];
+        }
+        return p;
+    }
+
+    void launch_prepared_stage(hipStream_t s, qsv::c64 *state, int n, PreparedStage &p, const float *mats_dev)
+    {
+        p.op.R = mats_dev;
+        p.op.blocks = reinterpret_cast<const qsv::StageBlock *>(mats_dev + p.desc_offset);
+        qsv::launch_stage(s, state, n, p.tsub, p.op);
+    }
+
+    PreparedStage128 prepare_stage128(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int n, std::vector<double> &mats)
+    {
+        constexpr int m = 12;
+        if ((int)st.tile.size() != m)
+            throw std::runtime_error("stage128: tile size mismatch");
+        std::vector<ResolvedGate> sub;
+        sub.reserve(st.gates.size());
+        for (int gi : st.gates)
+            sub.push_back(gates[gi]);
+        const auto plan = plan_greedy(sub, 4);
+        std::vector<qsv::StageBlock> desc;
+        for (const auto &pb : plan)
+        {
+            if (pb.wide)
+                throw std::runtime_error("stage128: wide gate inside a stage");
+            BlockLayout lay;
+            lay.qubits = pb.qubits;
+            for (int i = m - 1; i >= 0 && (int)lay.qubits.size() < 4; --i)
+                if (local_pos(lay.qubits, st.tile[i]) < 0)
+                    lay.qubits.push_back(st.tile[i]);
+            const auto M = block_matrix(sub, pb, lay);
+            qsv::StageBlock sb{};
+            sb.matrix = uint32_t(mats.size());
+            for (int i = 0; i < 256; ++i)
+            {
+                mats.push_back(M[i].real());
+                mats.push_back(M[i].imag());
+            }
+            std::vector<char> is_block(m, 0);
+            for (int j = 0; j < 4; ++j)
+            {
+                const int b = local_pos(st.tile, lay.qubits[j]);
+                if (b < 0)
+                    throw std::runtime_error("stage128: block qubit outside the tile");
+                sb.w[j] = 1 << b;
+                is_block[b] = 1;
+            }
+            int j = 4;
+            for (int b = 0; b < m; ++b)
+                if (!is_block[b])
+                    sb.w[j++] = 1 << b;
+            desc.push_back(sb);
+        }
+        PreparedStage128 p;
+        p.gates = (int)st.gates.size();
+        p.nblocks = (int)desc.size();
+        p.bytes = 2.0 * 16.0 * double(uint64_t(1) << n);
+        while (mats.size() % 2)
+            mats.push_back(0.0);
+        p.desc_offset = mats.size();
+        mats.resize(mats.size() + desc.size() * sizeof(qsv::StageBlock) / sizeof(double));
+        std::memcpy(mats.data() + p.desc_offset, desc.data(), desc.size() * sizeof(qsv::StageBlock));
+        p.tsub.set_bits = 0;
+        p.tsub.nops = m;
+        p.op.m = m;
+        p.op.nblocks = p.nblocks;
+        for (int b = 0; b < m; ++b)
+        {
+            p.tsub.ops[b] = st.tile[b];
+            p.op.gw[b] = uint64_t(1) << st.tile[b];
+        }
+        return p;
+    }
+
+    void launch_prepared_stage128(hipStream_t s, qsv::c128 *state, int n, PreparedStage128 &p, const double *mats_dev)
+    {
+        p.op.M = mats_dev;
+        p.op.blocks = reinterpret_cast<const qsv::StageBlock *>(mats_dev + p.desc_offset);
+        qsv::launch_stage128(s, state, n, p.tsub, p.op);
+    }
+
+    void FusedEngine::set_stage_bits(int m)
+    {
+        if (m != 0 && m != 12 && m != 13)
+            throw std::runtime_error("stage tile bits must be 0 (per-block launches), 12 or 13");
+        flush();
+        stage_bits_ = m;
+    }
+
+    void FusedEngine::apply_stages(const std::vector<ResolvedGate> &gates)
+    {
+        const int n = sim_.n_;
+        const int m = tile_bits();
+        const auto stages = plan_stages(gates, n, m, kq_);
+        // one matrix buffer for the whole flush: FP32 (complex64 modes) or FP64 entries
+        std::vector<float> mats;
+        std::vector<double> mats64;
+        struct Item
+        {
+            bool stage;
+            PreparedStage st;
+            PreparedStage128 st128;
+            std::vector<PreparedBlock> blocks;
+            std::vector<PreparedBlock128> blocks128;
+        };
+        std::vector<Item> items;
+        for (const auto &s : stages)
+        {
+            Item it;
+            it.stage = !s.tile.empty();
+            if (it.stage)
+            {
+                if (f64_)
+                    it.st128 = prepare_stage128(gates, s, n, mats64);
+                else
+                    it.st = prepare_stage(gates, s, kq_, m, n, mats, x3_);
+            }
+            else
+            {
+                std::vector<ResolvedGate> one{gates[s.gates[0]]};
+                const auto plan = plan_greedy(one, k_);
+                if (f64_)
+                {
+                    it.blocks128 = prepare_blocks128(one, plan, n, mats64);
+                    for (auto &b : it.blocks128)
+                        b.gates = 1;
+                }
+                else
+                    it.blocks = prepare_blocks(one, plan, kq_, n, mats, x3_);
+            }
+            items.push_back(std::move(it));
+        }
+        const size_t bytes = f64_ ? mats64.size() * sizeof(double) : mats.size() * sizeof(float);
+        reserve_matrices(bytes);
+        std::memcpy(rhost_, f64_ ? (const void *)mats64.data() : (const void *)mats.data(), bytes);
+        qsv::check(hipMemcpyAsync(rdev_, rhost_, bytes, hipMemcpyHostToDevice, sim_.stream_), "matrix upload");
+        qsv::check(hipEventRecord(copy_done_, sim_.stream_), "hipEventRecord");
+        copy_pending_ = true;
+        for (auto &it : items)
+        {
+            if (it.stage)
+            {
+                sim_.prof_begin(QCS_EXT_K_FUSED, f64_ ? it.st128.bytes : it.st.bytes);
+                if (f64_)
+                    launch_prepared_stage128(sim_.stream_, state128_, n, it.st128, static_cast<const double *>(rdev_));
+                else
+                    launch_prepared_stage(sim_.stream_, state_, n, it.st, static_cast<const float *>(rdev_));
+                sim_.prof_end();
+                blocks_applied_ += 1;
+                gates_applied_ += f64_ ? it.st128.gates : it.st.gates;
+                continue;
+            }
+            for (auto &b : it.blocks128)
+            {
+                sim_.prof_begin(QCS_EXT_K_FUSED, b.bytes);
+                launch_prepared128(sim_.stream_, state128_, n, b, static_cast<const double *>(rdev_));
+                sim_.prof_end();
+                blocks_applied_ += 1;
+                gates_applied_ += b.gates;
+            }
+            for (auto &b : it.blocks)
+            {
+                sim_.prof_begin(QCS_EXT_K_FUSED, b.bytes);
+                launch_prepared(sim_.stream_, state_, n, b, static_cast<const float *>(rdev_));
+                sim_.prof_end();
+                blocks_applied_ += 1;
+                gates_applied_ += b.gates;
+            }
+        }
     }
 
     void FusedEngine::apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan)

@@ -3,6 +3,7 @@
 #include "fused_kernels.hpp"
 #include "simulator.hpp"
 
+#include <algorithm>
 #include <vector>
 
 namespace qcore
@@ -72,6 +73,48 @@ namespace qcore
     std::vector<PreparedBlock128> prepare_blocks128(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan, int n, std::vector<double> &mats);
     void launch_prepared128(hipStream_t s, qsv::c128 *state, int n, PreparedBlock128 &b, const double *mats_dev);
 
+    // Stages: runs of gates whose qubits all lie in one set of m tile qubits (qubits 0..3
+    // always among them), applied as several blocks to LDS-resident tiles in one sweep.
+    // Gates are taken in order; a gate that does not fit (or touches a qubit of a gate
+    // already deferred, or is wider than k) is deferred to a later stage, so every gate
+    // taken commutes with every deferred one.  A stage with an empty tile holds one gate
+    // wider than k, run by the per-block path.
+    struct PlannedStage
+    {
+        std::vector<int> tile;  // m tile qubits, ascending (empty: one wide gate)
+        std::vector<int> gates; // indices into the gate list, in application order
+    };
+    std::vector<PlannedStage> plan_stages(const std::vector<ResolvedGate> &gates, int n, int m, int k);
+
+    // A stage ready to launch: its blocks' matrices and descriptors in a matrix buffer.
+    struct PreparedStage
+    {
+        int gates = 0;
+        int nblocks = 0;
+        double bytes = 0;
+        size_t desc_offset = 0; // floats: StageBlock array
+        qsv::Subspace tsub{};
+        qsv::StageOp op{};
+    };
+    // Plans the stage's gates into blocks of at most kq qubits inside the tile and appends
+    // their matrices (FP32, or bfloat16 halves when x3) and descriptors to `mats`.
+    PreparedStage prepare_stage(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int kq, int m, int n, std::vector<float> &mats, bool x3);
+    void launch_prepared_stage(hipStream_t s, qsv::c64 *state, int n, PreparedStage &p, const float *mats_dev);
+
+    // complex128 stages (tile of 2^12, FP64 blocks of 4 qubits): matrices and descriptors
+    // appended to `mats` as doubles.
+    struct PreparedStage128
+    {
+        int gates = 0;
+        int nblocks = 0;
+        double bytes = 0;
+        size_t desc_offset = 0; // doubles: StageBlock array
+        qsv::Subspace tsub{};
+        qsv::Stage128Op op{};
+    };
+    PreparedStage128 prepare_stage128(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int n, std::vector<double> &mats);
+    void launch_prepared_stage128(hipStream_t s, qsv::c128 *state, int n, PreparedStage128 &p, const double *mats_dev);
+
     class CostModel;
 
     class FusedEngine
@@ -94,6 +137,17 @@ namespace qcore
         void flush();
         void drop() { queue_.clear(); }
         void swap_physical(int a, int b);
+        // LDS tile bits of the stage kernel (12 or 13), or 0 for one launch per block.
+        void set_stage_bits(int m);
+        int stage_bits() const { return stage_bits_; }
+        // tile bits of the stage kernel for this state (complex128 tiles hold 2^12), 0 if
+        // per-block launches are used; stages need at least 16 tiles
+        int tile_bits() const
+        {
+            const int m = f64_ ? std::min(stage_bits_, 12) : stage_bits_;
+            return (stage_bits_ > 0 && sim_.n_ >= m + 4) ? m : 0;
+        }
+        bool uses_stages() const { return tile_bits() > 0; }
         int block_qubits() const { return k_; }
         uint64_t blocks_applied() const { return blocks_applied_; }
         uint64_t gates_applied() const { return gates_applied_; }
@@ -108,11 +162,13 @@ namespace qcore
 
     private:
         void apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan);
+        void apply_stages(const std::vector<ResolvedGate> &gates);
         void reserve_matrices(size_t bytes);
 
         Simulator &sim_;
         int k_;  // planner limit (qubits per block)
         int kq_; // kernel block size (4 or 5)
+        int stage_bits_ = 13;
         bool f64_ = false;
         bool x3_ = false;
         qsv::c64 *state_ = nullptr;

@@ -437,6 +437,215 @@ namespace qsv
                 }
             }
         }
+
+        // Stage kernel: one workgroup holds a tile of 2^M amplitudes in LDS -- the
+        // amplitudes whose index bits outside the stage's tile qubits T are fixed -- and
+        // applies every block of the stage to it before writing it back, so a stage costs
+        // one HBM sweep however many blocks it holds.  T[0..3] are qubits 0..3, so each
+        // thread moves 2 adjacent amplitudes (16 B) per load/store instruction and 8 lanes
+        // cover one 128-B line.  LDS index bit b <-> tile qubit T[b].
+        //
+        // Per block, the 2^(M-KQ) columns (tile bits outside the block) are split into
+        // groups of 32 over the 4 waves; lane (h, c) of a wave gathers column c's half h
+        // from LDS (LDS weights of the block's bits and of the column bits come with the
+        // block), multiplies on MFMA as the block kernels do, and writes its rows back in
+        // place.  Workgroup barriers separate the blocks.
+        template <int KQ, int M, bool X3>
+        __global__ __launch_bounds__(kThreads) void k_stage(c64 *__restrict__ s, uint64_t ntiles, Subspace tsub, StageOp op)
+        {
+            constexpr int D = 2 << KQ;
+            constexpr int MT = D / 32;
+            constexpr int H = D / 2;
+            constexpr int AMPS = H / 2;
+            constexpr int NT = 1 << M;
+            constexpr int COLS = NT >> KQ;
+            constexpr int PER_WAVE = COLS / 32 / kWaves;
+            constexpr int LOADS = NT / (2 * kThreads);
+            static_assert(PER_WAVE >= 1 && LOADS >= 1, "tile too small");
+            __shared__ __attribute__((aligned(16))) v2f tile[NT];
+
+            const int lane = threadIdx.x & 63;
+            const int wave = threadIdx.x >> 6;
+            const int c = lane & 31;
+            const int h = lane >> 5;
+
+            // global offset of this thread's first amplitude (LDS index 2 * threadIdx.x)
+            uint64_t g_t = 0;
+#pragma unroll
+            for (int b = 1; b < 9 && b < M; ++b)
+                if (((2 * threadIdx.x) >> b) & 1)
+                    g_t += op.gw[b];
+
+            for (uint64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x)
+            {
+                const uint64_t base = expand(tl, tsub) + g_t;
+#pragma unroll
+                for (int i = 0; i < LOADS; ++i)
+                {
+                    uint64_t gi = 0;
+#pragma unroll
+                    for (int b = 9; b < M; ++b)
+                        if (((i * 2 * kThreads) >> b) & 1)
+                            gi += op.gw[b];
+                    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (base + gi)));
+                    *reinterpret_cast<v4f *>(&tile[2 * threadIdx.x + i * 2 * kThreads]) = v;
+                }
+                __syncthreads();
+                for (int bi = 0; bi < op.nblocks; ++bi)
+                {
+                    const StageBlock &B = op.blocks[bi];
+                    BlockMatrix<KQ, X3> A;
+                    A.load(op.R + B.matrix, c, h);
+#pragma unroll
+                    for (int g = 0; g < PER_WAVE; ++g)
+                    {
+                        const int col = (wave * PER_WAVE + g) * 32 + c;
+                        int cbase = 0;
+#pragma unroll
+                        for (int j = 0; j < M - KQ; ++j)
+                            if ((col >> j) & 1)
+                                cbase += B.w[KQ + j];
+                        float x[H];
+#pragma unroll
+                        for (int e = 0; e < AMPS; ++e)
+                        {
+                            const int amp = h * AMPS + e;
+                            int off = 0;
+#pragma unroll
+                            for (int j = 0; j < KQ; ++j)
+                                if ((amp >> j) & 1)
+                                    off += B.w[j];
+                            const v2f a = tile[cbase + off];
+                            x[2 * e] = a.x;
+                            x[2 * e + 1] = a.y;
+                        }
+                        v16f d[MT];
+#pragma unroll
+                        for (int mt = 0; mt < MT; ++mt)
+                            d[mt] = v16f{0.f};
+                        A.apply(x, d);
+                        // accumulator register r of tile mt holds real row 32*mt + 8*(r/4) + 4*h + r%4,
+                        // i.e. amplitudes 16*mt + 4*(r/4) + 2*h and the next one
+#pragma unroll
+                        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                            for (int q = 0; q < 4; ++q)
+                            {
+                                const int e0 = 16 * mt + 4 * q + 2 * h;
+                                int o0 = 0, o1 = 0;
+#pragma unroll
+                                for (int j = 0; j < KQ; ++j)
+                                {
+                                    if ((e0 >> j) & 1)
+                                        o0 += B.w[j];
+                                    if (((e0 + 1) >> j) & 1)
+                                        o1 += B.w[j];
+                                }
+                                tile[cbase + o0] = v2f{d[mt][4 * q + 0], d[mt][4 * q + 1]};
+                                tile[cbase + o1] = v2f{d[mt][4 * q + 2], d[mt][4 * q + 3]};
+                            }
+                    }
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int i = 0; i < LOADS; ++i)
+                {
+                    uint64_t gi = 0;
+#pragma unroll
+                    for (int b = 9; b < M; ++b)
+                        if (((i * 2 * kThreads) >> b) & 1)
+                            gi += op.gw[b];
+                    const v4f v = *reinterpret_cast<const v4f *>(&tile[2 * threadIdx.x + i * 2 * kThreads]);
+                    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(s + (base + gi)));
+                }
+                __syncthreads();
+            }
+        }
+
+        // complex128 stage: a tile of 2^12 amplitudes (64 KiB) in LDS, tile bits as in
+        // k_stage; each block of 4 qubits has 256 columns, one per thread, applied in FP64
+        // with the 16x16 matrix read through uniform loads.
+        constexpr int kStage128Bits = 12;
+
+        __global__ __launch_bounds__(kThreads) void k_stage128(c128 *__restrict__ s, uint64_t ntiles, Subspace tsub, Stage128Op op)
+        {
+            typedef double v2d __attribute__((ext_vector_type(2)));
+            constexpr int NT = 1 << kStage128Bits;
+            constexpr int LOADS = NT / kThreads;
+            __shared__ __attribute__((aligned(16))) v2d tile[NT];
+            const int t = threadIdx.x;
+            uint64_t g_t = 0;
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+                if ((t >> b) & 1)
+                    g_t += op.gw[b];
+            for (uint64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x)
+            {
+                const uint64_t base = expand(tl, tsub) + g_t;
+#pragma unroll
+                for (int i = 0; i < LOADS; ++i)
+                {
+                    uint64_t gi = 0;
+#pragma unroll
+                    for (int b = 8; b < kStage128Bits; ++b)
+                        if (((i * kThreads) >> b) & 1)
+                            gi += op.gw[b];
+                    tile[t + i * kThreads] = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(s + (base + gi)));
+                }
+                __syncthreads();
+                for (int bi = 0; bi < op.nblocks; ++bi)
+                {
+                    const StageBlock &B = op.blocks[bi];
+                    const double *__restrict__ Mx = op.M + B.matrix;
+                    int cbase = 0;
+#pragma unroll
+                    for (int j = 0; j < kStage128Bits - 4; ++j)
+                        if ((t >> j) & 1)
+                            cbase += B.w[4 + j];
+                    int off[16];
+#pragma unroll
+                    for (int e = 0; e < 16; ++e)
+                    {
+                        off[e] = cbase;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if ((e >> j) & 1)
+                                off[e] += B.w[j];
+                    }
+                    v2d x[16];
+#pragma unroll
+                    for (int k = 0; k < 16; ++k)
+                        x[k] = tile[off[k]];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                    {
+                        double yr = 0.0, yi = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 16; ++k)
+                        {
+                            const double mr = Mx[2 * (16 * r + k)], mi = Mx[2 * (16 * r + k) + 1];
+                            yr = __builtin_fma(mr, x[k].x, yr);
+                            yr = __builtin_fma(-mi, x[k].y, yr);
+                            yi = __builtin_fma(mr, x[k].y, yi);
+                            yi = __builtin_fma(mi, x[k].x, yi);
+                        }
+                        tile[off[r]] = v2d{yr, yi};
+                    }
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int i = 0; i < LOADS; ++i)
+                {
+                    uint64_t gi = 0;
+#pragma unroll
+                    for (int b = 8; b < kStage128Bits; ++b)
+                        if (((i * kThreads) >> b) & 1)
+                            gi += op.gw[b];
+                    __builtin_nontemporal_store(tile[t + i * kThreads], reinterpret_cast<v2d *>(s + (base + gi)));
+                }
+                __syncthreads();
+            }
+        }
     } // namespace
 
     void launch_block128(hipStream_t st, c128 *state, int n, const Subspace &sub, const Block128Op &op, bool staged)
@@ -474,6 +683,68 @@ namespace qsv
             hipLaunchKernelGGL(k_block128, dim3(grid), dim3(kThreads), 0, st, state, count, sub, op);
         }
         check(hipGetLastError(), "block128 kernel");
+    }
+
+
+    void launch_stage(hipStream_t st, c64 *state, int n, const Subspace &tsub, const StageOp &op)
+    {
+        if (op.kq != 4 && op.kq != 5)
+            throw std::runtime_error("stage: kq must be 4 or 5");
+        if (op.m != 12 && op.m != 13)
+            throw std::runtime_error("stage: tile bits must be 12 or 13");
+        if (n <= op.m || tsub.nops != op.m)
+            throw std::runtime_error("stage: the tile must be a proper part of the state");
+        for (int k = 0; k < tsub.nops; ++k)
+            if (tsub.ops[k] < 0 || tsub.ops[k] >= n || (k > 0 && tsub.ops[k] <= tsub.ops[k - 1]))
+                throw std::runtime_error("stage: tile qubits must be ascending and < n");
+        for (int b = 0; b < 4; ++b)
+            if (tsub.ops[b] != b)
+                throw std::runtime_error("stage: tile qubits must include qubits 0..3");
+        for (int b = 0; b < op.m; ++b)
+            if (op.gw[b] != (uint64_t(1) << tsub.ops[b]))
+                throw std::runtime_error("stage: tile weights do not match the tile qubits");
+        if (tsub.set_bits != 0)
+            throw std::runtime_error("stage: tiles take no controls");
+        if (op.nblocks < 1 || op.blocks == nullptr || op.R == nullptr)
+            throw std::runtime_error("stage: no blocks");
+        const uint64_t ntiles = uint64_t(1) << (n - op.m);
+        uint64_t grid = ntiles < 8192 ? ntiles : 8192;
+        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op); };
+        if (op.kq == 5 && op.m == 13)
+            op.x3 ? go(k_stage<5, 13, true>) : go(k_stage<5, 13, false>);
+        else if (op.kq == 5)
+            op.x3 ? go(k_stage<5, 12, true>) : go(k_stage<5, 12, false>);
+        else if (op.m == 13)
+            op.x3 ? go(k_stage<4, 13, true>) : go(k_stage<4, 13, false>);
+        else
+            op.x3 ? go(k_stage<4, 12, true>) : go(k_stage<4, 12, false>);
+        check(hipGetLastError(), "stage kernel");
+    }
+
+
+    void launch_stage128(hipStream_t st, c128 *state, int n, const Subspace &tsub, const Stage128Op &op)
+    {
+        if (op.m != kStage128Bits)
+            throw std::runtime_error("stage128: tile bits must be 12");
+        if (n <= op.m || tsub.nops != op.m)
+            throw std::runtime_error("stage128: the tile must be a proper part of the state");
+        for (int k = 0; k < tsub.nops; ++k)
+            if (tsub.ops[k] < 0 || tsub.ops[k] >= n || (k > 0 && tsub.ops[k] <= tsub.ops[k - 1]))
+                throw std::runtime_error("stage128: tile qubits must be ascending and < n");
+        for (int b = 0; b < 4; ++b)
+            if (tsub.ops[b] != b)
+                throw std::runtime_error("stage128: tile qubits must include qubits 0..3");
+        for (int b = 0; b < op.m; ++b)
+            if (op.gw[b] != (uint64_t(1) << tsub.ops[b]))
+                throw std::runtime_error("stage128: tile weights do not match the tile qubits");
+        if (tsub.set_bits != 0)
+            throw std::runtime_error("stage128: tiles take no controls");
+        if (op.nblocks < 1 || op.blocks == nullptr || op.M == nullptr)
+            throw std::runtime_error("stage128: no blocks");
+        const uint64_t ntiles = uint64_t(1) << (n - op.m);
+        const uint64_t grid = ntiles < 8192 ? ntiles : 8192;
+        hipLaunchKernelGGL(k_stage128, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+        check(hipGetLastError(), "stage128 kernel");
     }
 
     void launch_block_lds(hipStream_t st, c64 *state, int n, const Subspace &tsub, const BlockLdsOp &op)

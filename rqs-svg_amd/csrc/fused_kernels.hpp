@@ -37,6 +37,43 @@ namespace qsv
     };
     void launch_block_lds(hipStream_t s, c64 *state, int n, const Subspace &tsub, const BlockLdsOp &op);
 
+    // Stage: several blocks applied to LDS-resident tiles in one sweep.  The tile qubits T
+    // (tsub.ops, ascending, m of them, T[0..3] = 0..3) are the stage's; gw[b] = 1 << T[b].
+    // Block i's matrix is at R + blocks[i].matrix (layout of BlockOp for x3); w[j] (j < kq)
+    // is the LDS index weight (1 << tile bit position) of the block's j-th qubit and
+    // w[kq + j] that of the j-th column bit (the other tile bits, ascending).  blocks
+    // points to device memory.
+    struct StageBlock
+    {
+        int w[16];
+        uint32_t matrix; // offset in floats from StageOp::R
+        uint32_t pad[3];
+    };
+    struct StageOp
+    {
+        int kq;
+        int m;
+        int x3;
+        int nblocks;
+        uint64_t gw[16];
+        const StageBlock *blocks;
+        const float *R;
+    };
+    void launch_stage(hipStream_t s, c64 *state, int n, const Subspace &tsub, const StageOp &op);
+
+    // complex128 stage: tile of 2^12 amplitudes, blocks of 4 qubits in FP64; blocks[i].matrix
+    // is the offset in doubles of the 16x16 complex matrix (row-major, interleaved) from M;
+    // w[0..3] block bits, w[4..11] column bits.
+    struct Stage128Op
+    {
+        int m;
+        int nblocks;
+        uint64_t gw[16];
+        const StageBlock *blocks;
+        const double *M;
+    };
+    void launch_stage128(hipStream_t s, c128 *state, int n, const Subspace &tsub, const Stage128Op &op);
+
     // complex128 block on 4 qubits in FP64.  M: 16x16 complex, row-major, interleaved
     // doubles, in device memory.  Staged form: gw/lw as for BlockLdsOp with 4 block bits and
     // the 6 lowest other qubits as column bits (LDS weight of column bit i = 17 << i);

@@ -127,6 +127,15 @@ namespace qcore
         block_qubits_ = k;
     }
 
+    void Simulator::set_stage_bits(int m)
+    {
+        if (m != 0 && m != 12 && m != 13)
+            throw std::runtime_error("stage tile bits must be 0 (one sweep per block), 12 or 13");
+        stage_bits_ = m;
+        if (fused_)
+            fused_->set_stage_bits(m);
+    }
+
     void Simulator::reinitialize_mapping()
     { // src/qcs.cu:1860-1885
         p2l_.assign(n_, 0);
@@ -172,6 +181,7 @@ namespace qcore
         else
         {
             fused_ = new FusedEngine(*this, block_qubits_, precision_);
+            fused_->set_stage_bits(stage_bits_);
         }
         work_len_ = qsv::reduce_workspace(count) + 16;
         qsv::check(hipMalloc((void **)&work_, work_len_ * sizeof(double)), "hipMalloc workspace");

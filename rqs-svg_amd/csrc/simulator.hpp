@@ -64,6 +64,7 @@ namespace qcore
         void get_mapping(int16_t *out, int count) const;
         void permute(const std::vector<int> &want_p2l);
         void set_precision(int mode, int k);
+        void set_stage_bits(int m);
         void flush();
         void overlap(Simulator &other, double out[3]);
         void set_planner_model(CostModel *m);
@@ -114,6 +115,7 @@ namespace qcore
         int n_ = 0;
         int n_local_ = 0;
         int precision_ = 0;
+        int stage_bits_ = 13;
         bool c64_state() const { return precision_ == 1 || precision_ == 3; }
         int block_qubits_ = 5;
         qsv::c128 *state_ = nullptr; // complex128 state (precision 0)

@@ -123,6 +123,7 @@ class Library:
         lib.qcs_ext_set_local_qubits.argtypes = [vp, bit_num_t]
         lib.qcs_ext_get_mapping.argtypes = [vp, P(bit_num_t), bit_num_t]
         lib.qcs_ext_permute.argtypes = [vp, P(bit_num_t), bit_num_t]
+        lib.qcs_ext_set_stage_bits.argtypes = [vp, ctypes.c_int]
         lib.qcs_ext_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.qcs_ext_flush.argtypes = [vp]
         lib.qcs_ext_fusion_stats.argtypes = [vp, P(u64), P(u64)]
@@ -178,7 +179,7 @@ class Simulator:
 
     def __init__(self, library: Library, num_qubits: int, num_clbits: int = 0,
                  mapping=None, seed: int | None = None, precision: int = 0,
-                 block_qubits: int = 5, allocate: bool = True):
+                 block_qubits: int = 5, allocate: bool = True, stage_bits: int | None = None):
         self.L = library
         self.lib = library.lib
         h = ctypes.c_void_p()
@@ -194,6 +195,8 @@ class Simulator:
             self._call(self.lib.qcs_ext_set_seed(self.h, seed), "set_seed")
         if precision:
             self._call(self.lib.qcs_ext_set_precision(self.h, precision, block_qubits), "set_precision")
+        if stage_bits is not None:
+            self._call(self.lib.qcs_ext_set_stage_bits(self.h, stage_bits), "set_stage_bits")
         if allocate:
             self.allocate()
 
@@ -328,6 +331,9 @@ class Simulator:
     def permute(self, p2l):
         a, c = _arr(p2l)
         self._call(self.lib.qcs_ext_permute(self.h, a, c), "permute")
+
+    def set_stage_bits(self, m):
+        self._call(self.lib.qcs_ext_set_stage_bits(self.h, m), "set_stage_bits")
 
     def flush(self):
         self._call(self.lib.qcs_ext_flush(self.h), "flush")

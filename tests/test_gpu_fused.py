@@ -193,3 +193,72 @@ def test_fused_complex128_qft_and_measure(gpu, oracle):
             res.append((sv, bits))
     assert np.max(np.abs(res[0][0] - res[1][0])) <= 1e-13
     assert res[0][1] == res[1][1]
+
+
+@pytest.mark.parametrize("precision", [1, 3])
+@pytest.mark.parametrize("m", [12, 13])
+def test_stages_match_oracle(gpu, oracle, precision, m):
+    # stages (several blocks per sweep on LDS-resident tiles), with gates wider than a
+    # block (up to 7 qubits) falling back to single launches
+    for n, seed in ((m + 4, 11), (m + 6, 12)):
+        circ = circuits.random_all_gates(n, 300, seed=seed, max_ctrls=3)
+        mapping = [int(q) for q in np.random.default_rng(seed).permutation(n)]
+        say(f"stages n={n} m={m} precision={precision}")
+        with pyqcs.Simulator(gpu, n, precision=precision, block_qubits=5, mapping=mapping, stage_bits=m) as f:
+            f.run(circ)
+            got = f.statevector()
+            gates, launches = f.fusion_stats()
+        with pyqcs.Simulator(oracle, n, mapping=mapping) as o:
+            o.run(circ)
+            want = o.statevector()
+        fid = fidelity(got, want)
+        err = np.max(np.abs(got - want))
+        say(f"  fidelity {fid:.12f} max|d| {err:.2e} gates {gates} launches {launches}")
+        assert fid >= 1 - FID_TOL and err <= AMP_TOL
+        assert gates == len(circ)
+
+
+@pytest.mark.parametrize("precision", [1, 3])
+def test_stages_vs_block_launches(gpu, precision):
+    n = 20
+    circ = circuits.brick(n, 12, seed=5, mcx=3)
+    res = {}
+    for m in (0, 12, 13):
+        with pyqcs.Simulator(gpu, n, precision=precision, block_qubits=5, stage_bits=m, num_clbits=2, seed=4) as f:
+            f.run(circ[: len(circ) // 2])
+            a = f.measure_to_clbit(7, 0)  # a flush boundary with a collapse in between
+            f.run(circ[len(circ) // 2:])
+            res[m] = (f.statevector(), f.fusion_stats(), a)
+    base = res[0][0]
+    for m in (12, 13):
+        assert res[m][2] == res[0][2]
+        assert fidelity(res[m][0], base) >= 1 - 1e-9
+        assert res[m][1][0] == res[0][1][0]        # same gates applied
+        assert res[m][1][1] * 2 <= res[0][1][1]    # at most half the sweeps
+    say(f"launches: per block {res[0][1][1]}, m=12 {res[12][1][1]}, m=13 {res[13][1][1]}")
+
+
+def test_stage_bits_validation(gpu):
+    with pyqcs.Simulator(gpu, 8, precision=1) as sim:
+        for bad in (5, 11, 14, -1):
+            assert sim.lib.qcs_ext_set_stage_bits(sim.h, bad) == 0
+        sim.set_stage_bits(0)
+        sim.set_stage_bits(12)
+
+
+def test_stages_complex128_match_oracle(gpu, oracle):
+    # precision mode 2 with stages: FP64 blocks of 4 qubits on LDS tiles of 2^12; FP64 bars
+    for n, seed in ((16, 21), (18, 22)):
+        circ = circuits.random_all_gates(n, 250, seed=seed, max_ctrls=3) + circuits.qft(n)
+        mapping = [int(q) for q in np.random.default_rng(seed).permutation(n)]
+        with pyqcs.Simulator(gpu, n, precision=2, block_qubits=4, mapping=mapping) as f:
+            f.run(circ)
+            got = f.statevector()
+            gates, launches = f.fusion_stats()
+        with pyqcs.Simulator(oracle, n, mapping=mapping) as o:
+            o.run(circ)
+            want = o.statevector()
+        say(f"c128 stages n={n}: max|d| {np.max(np.abs(got - want)):.2e} gates {gates} launches {launches}")
+        assert np.max(np.abs(got - want)) <= 1e-13
+        assert fidelity(got, want) >= 1 - 1e-12
+        assert gates == len(circ)

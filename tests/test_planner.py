@@ -115,7 +115,8 @@ def test_planner_record_train_and_plan(gpu, oracle):
     circ = circuits.qft(n)
     states = []
     for use in (False, True):
-        with pyqcs.Simulator(gpu, n, precision=1, block_qubits=5) as sim:
+        # the learned choice is among one-sweep-per-block plans
+        with pyqcs.Simulator(gpu, n, precision=1, block_qubits=5, stage_bits=0) as sim:
             if use:
                 pyqcs.set_planner(sim, model)
             sim.run(circ)
