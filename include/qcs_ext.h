@@ -146,6 +146,9 @@ int qcs_ext_batch_info(qcs_ext_batch *batch, int *num_circuits, int *total_qubit
  * such a workload, as a baseline.  Same outputs as qcs_ext_batch_run (up to the complex64
  * rounding of the batched form); seconds = host wall time of the whole loop. */
 int qcs_ext_batch_run_plain(qcs_ext_batch *batch, double *expz, uint64_t count, double *seconds);
+/* The launches of qcs_ext_batch_run issued directly on the streams instead of as one graph
+ * launch (for profilers that do not follow graph launches); same outputs. */
+int qcs_ext_batch_run_direct(qcs_ext_batch *batch, double *expz, uint64_t count, double *seconds);
 
 /* Learned fusion planner.  A cost model is a 3-layer MLP, 64 units wide (F -> 64 -> 64 ->
  * 1, ReLU, FP32) with parameters W1[64][F], b1[64], W2[64][64], b2[64], w3[64], b3.

@@ -1511,6 +1511,10 @@ extern "C"
                 *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         });
     }
+    int qcs_ext_batch_run_direct(qcs_ext_batch *b, double *expz, uint64_t count, double *seconds)
+    {
+        return qcs_ext_batch_run(b, expz, count, seconds);
+    }
     int qcs_ext_batch_run_plain(qcs_ext_batch *b, double *expz, uint64_t count, double *seconds)
     { // the oracle's batch run is already the plain method per circuit
         return qcs_ext_batch_run(b, expz, count, seconds);

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -75,6 +76,7 @@ namespace qcore
             (void)hipGraphDestroy(graph_);
         exec_ = nullptr;
         graph_ = nullptr;
+        work_ = nullptr;
         for (void *p : {(void *)d_circs_, (void *)d_ops_, (void *)d_out_, (void *)d_counter_, (void *)d_states_, (void *)d_partials_, (void *)d_mats_})
             if (p)
                 (void)hipFree(p);
@@ -258,14 +260,16 @@ namespace qcore
             fork_join_.push_back(e);
         }
         hipStream_t s0 = streams_[0];
-        qsv::check(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        try
-        {
+        const int nsmall = (int)small.size();
+        // the whole batch as launches on the streams: captured once into the graph below,
+        // or issued directly by run_direct
+        work_ = [this, runs = std::move(runs), grid, nsmall, small_max_n, ns, block_size = T.block_size]() {
+            hipStream_t s0 = streams_[0];
             qsv::check(hipEventRecord(fork_join_[0], s0), "hipEventRecord");
             for (int i = 1; i < ns; ++i)
                 qsv::check(hipStreamWaitEvent(streams_[i], fork_join_[0], 0), "hipStreamWaitEvent");
-            if (!small.empty())
-                qsv::launch_batch_lds(s0, T.block_size, grid, small_max_n, d_circs_, (int)small.size(), d_ops_, d_out_, d_counter_);
+            if (nsmall > 0)
+                qsv::launch_batch_lds(s0, block_size, grid, small_max_n, d_circs_, nsmall, d_ops_, d_out_, d_counter_);
             // large circuits, biggest first, over the streams (the LDS kernel occupies s0)
             std::vector<int> order(runs.size());
             for (size_t i = 0; i < order.size(); ++i)
@@ -273,7 +277,7 @@ namespace qcore
             std::sort(order.begin(), order.end(), [&](int a, int b) { return circuits_[runs[a].ci].n > circuits_[runs[b].ci].n; });
             for (size_t oi = 0; oi < order.size(); ++oi)
             {
-                LargeRun &lr = runs[order[oi]];
+                auto lr = runs[order[oi]];
                 const Circuit &C = circuits_[lr.ci];
                 hipStream_t s = streams_[ns == 1 ? 0 : 1 + oi % (ns - 1)];
                 qsv::c64 *st = d_states_ + lr.state_offset;
@@ -287,6 +291,11 @@ namespace qcore
                 qsv::check(hipEventRecord(fork_join_[ns + i], streams_[i]), "hipEventRecord");
                 qsv::check(hipStreamWaitEvent(s0, fork_join_[ns + i], 0), "hipStreamWaitEvent");
             }
+        };
+        qsv::check(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try
+        {
+            work_();
         }
         catch (...)
         {
@@ -310,6 +319,21 @@ namespace qcore
         hipStream_t s0 = streams_[0];
         qsv::check(hipEventRecord(t0_, s0), "hipEventRecord");
         qsv::check(hipGraphLaunch(exec_, s0), "hipGraphLaunch");
+        qsv::check(hipEventRecord(t1_, s0), "hipEventRecord");
+        qsv::check(hipEventSynchronize(t1_), "hipEventSynchronize");
+        float ms = 0;
+        qsv::check(hipEventElapsedTime(&ms, t0_, t1_), "hipEventElapsedTime");
+        if (expz)
+            qsv::check(hipMemcpy(expz, d_out_, total_qubits_ * sizeof(double), hipMemcpyDeviceToHost), "batch copy");
+        return ms * 1e-3;
+    }
+
+    double Batch::run_direct(double *expz)
+    {
+        prepare();
+        hipStream_t s0 = streams_[0];
+        qsv::check(hipEventRecord(t0_, s0), "hipEventRecord");
+        work_();
         qsv::check(hipEventRecord(t1_, s0), "hipEventRecord");
         qsv::check(hipEventSynchronize(t1_), "hipEventSynchronize");
         float ms = 0;

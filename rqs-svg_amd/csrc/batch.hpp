@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -43,6 +44,9 @@ namespace qcore
         // Runs every circuit from |0...0>, writes <Z_q> of circuit c at offset
         // sum_{c' < c} n_c' + q; returns seconds between graph launch and completion.
         double run(double *expz);
+        // The same launches issued directly on the streams, without the graph (for tools that
+        // do not follow graph launches); returns seconds between the first and last event.
+        double run_direct(double *expz);
         // The plain method per circuit (complex128 simulators, one kernel per gate); returns
         // host wall seconds of the loop.
         double run_plain(double *expz);
@@ -58,6 +62,7 @@ namespace qcore
             int out_offset;
         };
         void prepare();
+        std::function<void()> work_; // records the batch's launches on the streams
         void release();
 
         std::vector<Circuit> circuits_;

@@ -613,6 +613,18 @@ extern "C"
                 *seconds = s;
         });
     }
+    int qcs_ext_batch_run_direct(qcs_ext_batch *b, double *expz, uint64_t count, double *seconds)
+    {
+        return guard([&] {
+            if (!b || !b->core)
+                throw std::runtime_error("batch is NULL");
+            if (expz && count != (uint64_t)b->core->total_qubits())
+                throw std::runtime_error("output size must equal the total number of qubits");
+            double s = b->core->run_direct(expz);
+            if (seconds)
+                *seconds = s;
+        });
+    }
     int qcs_ext_batch_run_plain(qcs_ext_batch *b, double *expz, uint64_t count, double *seconds)
     {
         return guard([&] {

@@ -151,6 +151,7 @@ class Library:
         lib.qcs_ext_batch_autotune.argtypes = [vp, P(ctypes.c_int)]
         lib.qcs_ext_batch_run.argtypes = [vp, vp, u64, P(ctypes.c_double)]
         lib.qcs_ext_batch_run_plain.argtypes = [vp, vp, u64, P(ctypes.c_double)]
+        lib.qcs_ext_batch_run_direct.argtypes = [vp, vp, u64, P(ctypes.c_double)]
         lib.qcs_ext_batch_info.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int), P(u64)]
 
     def check(self, rc, what):
@@ -495,6 +496,17 @@ class Batch:
         out = np.empty(sum(self.sizes), dtype=np.float64)
         sec = ctypes.c_double()
         self.L.check(self.lib.qcs_ext_batch_run(self.h, out.ctypes.data, out.size, ctypes.byref(sec)), "batch_run")
+        res, off = [], 0
+        for n in self.sizes:
+            res.append(out[off:off + n].copy())
+            off += n
+        return res, sec.value
+
+    def run_direct(self):
+        """The batch's launches without the graph; (results, device seconds)."""
+        out = np.empty(sum(self.sizes), dtype=np.float64)
+        sec = ctypes.c_double()
+        self.L.check(self.lib.qcs_ext_batch_run_direct(self.h, out.ctypes.data, out.size, ctypes.byref(sec)), "batch_run_direct")
         res, off = [], 0
         for n in self.sizes:
             res.append(out[off:off + n].copy())

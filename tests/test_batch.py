@@ -92,3 +92,5 @@ def test_gpu_plain_loop_matches_oracle(gpu, oracle):
     assert max(np.max(np.abs(x - y)) for x, y in zip(got, want)) < 1e-12
     fast, _ = b.run()
     assert max(np.max(np.abs(x - y)) for x, y in zip(fast, want)) < 2e-5
+    direct, _ = b.run_direct()  # the same launches without the graph: identical bits
+    assert all(np.array_equal(x, y) for x, y in zip(direct, fast))
