@@ -22,10 +22,10 @@ Also reported:
                    HBM, compared on the GPU in FP64);
   batch         -- 512 variational circuits of 12-22 qubits in one autotuned hipGraph
                    launch against a per-circuit plain-method loop, with max |d<Z>|;
-  planner       -- the learned fusion planner: block timings recorded at 26-32 qubits,
+  planner       -- the learned fusion planner: launch timings recorded at 26-32 qubits,
                    cost-model MLP trained on the GPU, then a 33-qubit QFT and a 33-qubit
-                   brick circuit of depth 40 (complex128 fused) with the greedy plan and
-                   with the learned plan.
+                   brick circuit of depth 40 (complex128 fused): one sweep per greedy
+                   block, the default stage plan, and the plan the model picks.
 """
 import argparse
 import json
@@ -205,15 +205,18 @@ def batch_leg(count, seed=20250404, direct=False):
 
 
 def planner_leg(n, record_qubits, record_count, depth, k):
-    """Learned fusion planner on complex128 fused blocks: record block timings on states of
-    `record_qubits` qubits, train the cost-model MLP on the GPU, then run a QFT and a random
-    brick circuit at n qubits with the greedy plan and with the learned plan. The two
-    complex128 results must agree (<Z_q> and probability total in FP64); agreement with the
-    FP64 oracle is covered by the tests at smaller n."""
+    """Learned fusion planner on complex128 fused blocks: record launch timings (single
+    blocks and multi-block stages) on states of `record_qubits` qubits, train the cost-model
+    MLP on the GPU, then run a QFT and a random brick circuit at n qubits three ways: one
+    sweep per greedy block, the default plan (stages on 2^12-amplitude LDS tiles) and the
+    plan the model picks among both kinds (kept only when predicted 3 % faster than the
+    default).  The complex128 results must agree (<Z_q> in FP64); agreement with the FP64
+    oracle is covered by the tests at smaller n."""
     lib = pyqcs.gpu_library()
+    F = pyqcs.planner_num_features(lib)
     Xs, ys = [], []
     t0 = time.perf_counter()
-    # blocks per size fall with the state size (a 32-qubit complex128 block launch streams
+    # launches per size fall with the state size (a 32-qubit complex128 launch streams
     # 128 GiB) so that recording stays within tens of seconds
     weights = [0.5 ** i for i in range(len(record_qubits))]
     counts = [max(1, int(record_count * w / sum(weights))) for w in weights]
@@ -225,27 +228,28 @@ def planner_leg(n, record_qubits, record_count, depth, k):
     X = np.concatenate(Xs)
     y = np.concatenate(ys)
     t_rec = time.perf_counter() - t0
-    model = pyqcs.Mlp(lib, 16, seed=5)
+    model = pyqcs.Mlp(lib, F, seed=5)
     t0 = time.perf_counter()
     loss = model.train(X, y, epochs=300, batch=256, lr=2e-3, seed=2)
     t_train = time.perf_counter() - t0
-    log(f"planner leg: recorded {len(y)} blocks at {list(record_qubits)} qubits in {t_rec:.1f}s; "
-        f"trained in {t_train:.2f}s, mse {loss:.4f} (variance {np.var(y):.4f})")
+    log(f"planner leg: recorded {len(y)} launches ({int(X[:, 4].sum())} stages) at "
+        f"{list(record_qubits)} qubits in {t_rec:.1f}s; trained in {t_train:.2f}s, mse {loss:.4f} "
+        f"(variance {np.var(y):.4f})")
     out = {"config": f"complex128 fused blocks (k<={k}), {n}-qubit QFT and {n}-qubit brick "
-                     f"depth {depth}; cost model trained on {len(y)} recorded blocks at "
+                     f"depth {depth}; cost model trained on {len(y)} recorded launches at "
                      f"{min(record_qubits)}-{max(record_qubits)} qubits",
-           "recorded_blocks": int(len(y)), "record_s": t_rec, "train_s": t_train,
+           "recorded_launches": int(len(y)), "recorded_stages": int(X[:, 4].sum()),
+           "record_s": t_rec, "train_s": t_train,
            "train_mse_log2us": float(loss), "target_variance": float(np.var(y))}
-    # the learned choice is among one-sweep-per-block plans; the stage kernel's time on the
-    # same circuits is reported beside it
-    sim = pyqcs.Simulator(lib, n, seed=1, precision=2, block_qubits=k, stage_bits=0)
+    sim = pyqcs.Simulator(lib, n, seed=1, precision=2, block_qubits=k)
     try:
         sim.run(circuits.brick(n, 1, seed=1)[: n + 2])
         for name, circ in (("qft", circuits.qft(n)), ("brick", circuits.brick(n, depth, seed=CIRCUIT_SEED + 5))):
             res = {"gates": len(circ)}
             zs = {}
-            for mode in ("greedy", "learned"):
-                pyqcs.set_planner(sim, model if mode == "learned" else None)
+            for mode, stage_bits, use_model in (("per_block", 0, False), ("default", 12, False), ("learned", 12, True)):
+                sim.set_stage_bits(stage_bits)
+                pyqcs.set_planner(sim, model if use_model else None)
                 sim.call("set_zero_state")
                 sim.norm()
                 _, b0 = sim.fusion_stats()
@@ -256,29 +260,18 @@ def planner_leg(n, record_qubits, record_count, depth, k):
                 _, b1 = sim.fusion_stats()
                 zs[mode] = sim.expect_z()
                 res[f"{mode}_s"] = dt
-                res[f"{mode}_blocks"] = b1 - b0
+                res[f"{mode}_sweeps"] = b1 - b0
                 res[f"{mode}_probability_total"] = total
-                if mode == "learned":
+                if use_model:
                     st = pyqcs.planner_stats(sim)
-                    res["pred_greedy_us"] = st["pred_greedy_us"]
+                    res["pred_default_us"] = st["pred_greedy_us"]
                     res["pred_chosen_us"] = st["pred_chosen_us"]
-                log(f"planner leg: {name} {mode}: {dt:.3f}s, {b1 - b0} blocks")
-            res["speedup"] = res["greedy_s"] / res["learned_s"]
-            sim.set_stage_bits(12)
-            sim.call("set_zero_state")
-            sim.norm()
-            _, b0 = sim.fusion_stats()
-            t0 = time.perf_counter()
-            sim.run(circ)
-            sim.probability_total()
-            res["stages_s"] = time.perf_counter() - t0
-            _, b1 = sim.fusion_stats()
-            res["stages_sweeps"] = b1 - b0
-            zst = sim.expect_z()
-            res["stages_max_abs_diff_expz"] = float(np.max(np.abs(zst - zs["greedy"])))
-            sim.set_stage_bits(0)
-            log(f"planner leg: {name} stages: {res['stages_s']:.3f}s, {b1 - b0} sweeps")
-            res["max_abs_diff_expz"] = float(np.max(np.abs(zs["greedy"] - zs["learned"])))
+                    res["learned_choices"] = st["learned_choices"]
+                    res["flushes"] = st["flushes"]
+                log(f"planner leg: {name} {mode}: {dt:.3f}s, {b1 - b0} sweeps")
+            res["learned_vs_default"] = res["default_s"] / res["learned_s"]
+            res["default_vs_per_block"] = res["per_block_s"] / res["default_s"]
+            res["max_abs_diff_expz"] = float(max(np.max(np.abs(zs[a] - zs["per_block"])) for a in ("default", "learned")))
             out[name] = res
     finally:
         pyqcs.set_planner(sim, None)

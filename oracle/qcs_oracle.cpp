@@ -1640,7 +1640,7 @@ extern "C"
                 *loss = l;
         });
     }
-    int qcs_ext_planner_num_features(int *count) { return guard([&] { put(count, 16); }); }
+    int qcs_ext_planner_num_features(int *count) { return guard([&] { put(count, 20); }); }
     int qcs_ext_planner_record(qcs_simulator *, int, uint64_t, float *, float *)
     {
         return guard([&] { throw std::runtime_error("the oracle has no kernels to time"); });

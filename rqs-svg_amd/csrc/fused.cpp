@@ -307,60 +307,93 @@ namespace qcore
             return;
         std::vector<ResolvedGate> gates;
         gates.swap(queue_);
-        if (uses_stages())
+        const int m0 = tile_bits();
+        if (!model_)
         {
-            apply_stages(gates);
+            if (m0 > 0)
+                apply_stages(gates, m0);
+            else
+                apply_blocks(gates, plan_greedy(gates, k_));
             return;
         }
-        auto plan = plan_greedy(gates, k_);
-        if (model_)
+        // Learned planner: predicted cost of each candidate plan = sum over its launches of
+        // 2^prediction (the model predicts log2 microseconds).  Candidate 0 is the plan the
+        // engine runs without a model (stages when enabled, else greedy blocks), and it
+        // judges the others: it is kept unless a candidate is predicted 3 % faster.
+        const int n = sim_.n_;
+        const float math = math_feature();
+        std::vector<int> tiles;
+        for (int m : f64_ ? std::vector<int>{12} : std::vector<int>{12, 13})
+            if (n >= m + 4)
+                tiles.push_back(m);
+        std::vector<PlanOption> opts;
+        opts.push_back(PlanOption{m0, kq_, 64, 0});
+        for (const auto &o : candidate_options(kq_, tiles))
+            if (o.k <= k_ || o.stage_bits > 0)
+                opts.push_back(o);
+        std::vector<float> X;
+        std::vector<size_t> first(opts.size() + 1, 0);
+        auto add_block = [&](const std::vector<ResolvedGate> &gs, const PlannedBlock &b) {
+            const BlockLayout lay = layout_block(gs, b, kq_, n);
+            float f[kPlanFeatures];
+            block_features(n, math, kq_, lay, (int)b.gates.size(), block_uses_lds(lay, n, f64_), f);
+            X.insert(X.end(), f, f + kPlanFeatures);
+        };
+        for (size_t oi = 0; oi < opts.size(); ++oi)
         {
-            // predicted cost of each candidate plan = sum over its blocks of 2^prediction
-            const int n = sim_.n_;
-            auto opts = candidate_options(k_);
-            std::vector<std::vector<PlannedBlock>> plans;
-            plans.push_back(plan);
-            for (const auto &o : opts)
-                if (o.k <= k_)
-                    plans.push_back(plan_with(gates, o));
-            std::vector<float> X;
-            std::vector<size_t> first(plans.size() + 1, 0);
-            for (size_t pi = 0; pi < plans.size(); ++pi)
+            first[oi] = X.size() / kPlanFeatures;
+            const PlanOption &o = opts[oi];
+            if (o.stage_bits == 0)
             {
-                first[pi] = X.size() / kPlanFeatures;
-                for (const auto &b : plans[pi])
+                for (const auto &b : plan_with(gates, o))
+                    add_block(gates, b);
+                continue;
+            }
+            for (const auto &st : plan_stages(gates, n, o.stage_bits, kq_))
+            {
+                std::vector<ResolvedGate> sub;
+                for (int gi : st.gates)
+                    sub.push_back(gates[gi]);
+                const auto blocks = plan_greedy(sub, kq_);
+                if (st.tile.empty())
                 {
-                    float f[kPlanFeatures];
-                    block_features(n, math_feature(), kq_, layout_block(gates, b, kq_, n), (int)b.gates.size(), f);
-                    X.insert(X.end(), f, f + kPlanFeatures);
+                    for (const auto &b : blocks)
+                        add_block(sub, b);
+                    continue;
                 }
-            }
-            first[plans.size()] = X.size() / kPlanFeatures;
-            std::vector<float> Y(first.back());
-            model_->predict(X.data(), (int)Y.size(), Y.data());
-            std::vector<double> cost(plans.size(), 0.0);
-            for (size_t pi = 0; pi < plans.size(); ++pi)
-                for (size_t r = first[pi]; r < first[pi + 1]; ++r)
-                    cost[pi] += std::exp2(double(Y[r]));
-            size_t best = 0;
-            for (size_t pi = 1; pi < plans.size(); ++pi)
-                if (std::isfinite(cost[pi]) && cost[pi] < cost[best])
-                    best = pi;
-            stat_flushes_ += 1;
-            stat_pred_greedy_ += cost[0];
-            // the greedy plan judges the prediction: keep it unless clearly beaten
-            if (best != 0 && std::isfinite(cost[0]) && cost[best] < 0.97 * cost[0])
-            {
-                plan = plans[best];
-                stat_learned_ += 1;
-                stat_pred_chosen_ += cost[best];
-            }
-            else
-            {
-                stat_pred_chosen_ += cost[0];
+                float f[kPlanFeatures];
+                stage_features(n, math, kq_, st.tile, (int)blocks.size(), (int)st.gates.size(), f);
+                X.insert(X.end(), f, f + kPlanFeatures);
             }
         }
-        apply_blocks(gates, plan);
+        first[opts.size()] = X.size() / kPlanFeatures;
+        std::vector<float> Y(first.back());
+        model_->predict(X.data(), (int)Y.size(), Y.data());
+        std::vector<double> cost(opts.size(), 0.0);
+        for (size_t oi = 0; oi < opts.size(); ++oi)
+            for (size_t r = first[oi]; r < first[oi + 1]; ++r)
+                cost[oi] += std::exp2(double(Y[r]));
+        size_t best = 0;
+        for (size_t oi = 1; oi < opts.size(); ++oi)
+            if (std::isfinite(cost[oi]) && cost[oi] < cost[best])
+                best = oi;
+        stat_flushes_ += 1;
+        stat_pred_greedy_ += cost[0];
+        if (best != 0 && std::isfinite(cost[0]) && cost[best] < 0.97 * cost[0])
+        {
+            stat_learned_ += 1;
+            stat_pred_chosen_ += cost[best];
+        }
+        else
+        {
+            best = 0;
+            stat_pred_chosen_ += cost[0];
+        }
+        const PlanOption &o = opts[best];
+        if (o.stage_bits > 0)
+            apply_stages(gates, o.stage_bits);
+        else
+            apply_blocks(gates, plan_with(gates, o));
     }
 
     void FusedEngine::planner_stats(double out[5]) const
@@ -376,6 +409,7 @@ namespace qcore
     {
         flush();
         const int n = sim_.n_;
+        const float math = math_feature();
         auto rnd = [&seed]() {
             uint64_t z = (seed += 0x9E3779B97F4A7C15ull);
             z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -383,7 +417,48 @@ namespace qcore
             return z ^ (z >> 31);
         };
         auto unif = [&]() { return double(rnd() >> 11) * 0x1.0p-53; };
-        const int chunk = 128;
+        std::vector<int> tiles;
+        for (int m : f64_ ? std::vector<int>{12} : std::vector<int>{12, 13})
+            if (n >= m + 4)
+                tiles.push_back(m);
+        // 1..24 random gates (U3, or CX/CZ with probability 0.3) on the qubit set qs
+        auto random_gates = [&](const std::vector<int> &qs, std::vector<ResolvedGate> &out) {
+            const int ng = 1 + int(rnd() % 24);
+            for (int g = 0; g < ng; ++g)
+            {
+                ResolvedGate r;
+                if (qs.size() >= 2 && unif() < 0.3)
+                {
+                    double none[1] = {0.0};
+                    r.spec = make_gate(unif() < 0.5 ? "z" : "x", none);
+                    int a = qs[rnd() % qs.size()], b = a;
+                    while (b == a)
+                        b = qs[rnd() % qs.size()];
+                    r.tphys = {a};
+                    r.pphys = {b};
+                }
+                else
+                {
+                    double prm[3] = {6.283 * unif(), 6.283 * unif(), 6.283 * unif()};
+                    r.spec = make_gate("u3", prm);
+                    r.tphys = {qs[rnd() % qs.size()]};
+                }
+                out.push_back(r);
+            }
+        };
+        auto random_subset = [&](const std::vector<int> &from, int size) {
+            std::vector<int> qs;
+            while ((int)qs.size() < size)
+            {
+                int q = from[rnd() % from.size()];
+                if (std::find(qs.begin(), qs.end(), q) == qs.end())
+                    qs.push_back(q);
+            }
+            return qs;
+        };
+        std::vector<int> all(n);
+        std::iota(all.begin(), all.end(), 0);
+        const int chunk = 64;
         std::vector<hipEvent_t> ev(2 * chunk);
         for (auto &e : ev)
             qsv::check(hipEventCreate(&e), "hipEventCreate");
@@ -392,81 +467,98 @@ namespace qcore
             // start = -3: three untimed warm-up launches, then rows [0, count) in chunks
             for (int start = -3; start < count; start = start < 0 ? 0 : start + chunk)
             {
-                const int m = start < 0 ? 3 : std::min(chunk, count - start);
-                std::vector<std::vector<ResolvedGate>> gate_sets(m);
-                std::vector<PlannedBlock> blocks(m);
+                const int mcount = start < 0 ? 3 : std::min(chunk, count - start);
                 std::vector<float> mats;
                 std::vector<double> mats64;
-                std::vector<PreparedBlock> pb;
-                std::vector<PreparedBlock128> pb128;
-                for (int i = 0; i < m; ++i)
+                struct Launch
                 {
-                    const int kb = 1 + int(rnd() % uint64_t(k_));
-                    std::vector<int> qs;
-                    while ((int)qs.size() < kb)
+                    int tile_bits = 0;
+                    PreparedBlock b;
+                    PreparedBlock128 b128;
+                    PreparedStage st;
+                    PreparedStage128 st128;
+                    float f[kPlanFeatures];
+                };
+                std::vector<Launch> ls(mcount);
+                for (int i = 0; i < mcount; ++i)
+                {
+                    Launch &L = ls[i];
+                    // half of the launches are stages when the state is large enough
+                    if (!tiles.empty() && (rnd() & 1))
                     {
-                        int q = int(rnd() % uint64_t(n));
-                        if (std::find(qs.begin(), qs.end(), q) == qs.end())
-                            qs.push_back(q);
-                    }
-                    const int ng = 1 + int(rnd() % 24);
-                    for (int g = 0; g < ng; ++g)
-                    {
-                        ResolvedGate r;
-                        if (kb >= 2 && unif() < 0.3)
+                        const int m = tiles[rnd() % tiles.size()];
+                        std::vector<int> upper(all.begin() + 4, all.end());
+                        auto extra = random_subset(upper, m - 4);
+                        PlannedStage st;
+                        st.tile = {0, 1, 2, 3};
+                        st.tile.insert(st.tile.end(), extra.begin(), extra.end());
+                        std::sort(st.tile.begin(), st.tile.end());
+                        std::vector<ResolvedGate> gs;
+                        const int nb = 1 + int(rnd() % 12);
+                        for (int b = 0; b < nb; ++b)
+                            random_gates(random_subset(st.tile, 1 + int(rnd() % kq_)), gs);
+                        st.gates.resize(gs.size());
+                        std::iota(st.gates.begin(), st.gates.end(), 0);
+                        L.tile_bits = m;
+                        int nblocks;
+                        if (f64_)
                         {
-                            double none[1] = {0.0};
-                            r.spec = make_gate(unif() < 0.5 ? "z" : "x", none);
-                            int a = qs[rnd() % qs.size()], b = a;
-                            while (b == a)
-                                b = qs[rnd() % qs.size()];
-                            r.tphys = {a};
-                            r.pphys = {b};
+                            L.st128 = prepare_stage128(gs, st, n, mats64);
+                            nblocks = L.st128.nblocks;
                         }
                         else
                         {
-                            double prm[3] = {6.283 * unif(), 6.283 * unif(), 6.283 * unif()};
-                            r.spec = make_gate("u3", prm);
-                            r.tphys = {qs[rnd() % qs.size()]};
+                            L.st = prepare_stage(gs, st, kq_, m, n, mats, x3_);
+                            nblocks = L.st.nblocks;
                         }
-                        gate_sets[i].push_back(r);
+                        stage_features(n, math, kq_, st.tile, nblocks, (int)gs.size(), L.f);
+                        continue;
                     }
-                    blocks[i].qubits = qs;
-                    for (int g = 0; g < ng; ++g)
-                        blocks[i].gates.push_back(g);
+                    std::vector<ResolvedGate> gs;
+                    random_gates(random_subset(all, 1 + int(rnd() % uint64_t(k_))), gs);
+                    PlannedBlock pb;
+                    for (const auto &g : gs)
+                        for (int q : gate_qubits(g))
+                            if (local_pos(pb.qubits, q) < 0)
+                                pb.qubits.push_back(q);
+                    for (size_t g = 0; g < gs.size(); ++g)
+                        pb.gates.push_back((int)g);
+                    const BlockLayout lay = layout_block(gs, pb, kq_, n);
                     if (f64_)
-                    {
-                        auto v = prepare_blocks128(gate_sets[i], {blocks[i]}, n, mats64);
-                        pb128.push_back(v[0]);
-                    }
+                        L.b128 = prepare_blocks128(gs, {pb}, n, mats64)[0];
                     else
-                    {
-                        auto v = prepare_blocks(gate_sets[i], {blocks[i]}, kq_, n, mats, x3_);
-                        pb.push_back(v[0]);
-                    }
+                        L.b = prepare_blocks(gs, {pb}, kq_, n, mats, x3_)[0];
+                    block_features(n, math, kq_, lay, (int)gs.size(), block_uses_lds(lay, n, f64_), L.f);
                 }
                 const size_t bytes = f64_ ? mats64.size() * sizeof(double) : mats.size() * sizeof(float);
                 reserve_matrices(bytes);
                 qsv::check(hipStreamSynchronize(sim_.stream_), "sync");
                 qsv::check(hipMemcpy(rdev_, f64_ ? (const void *)mats64.data() : (const void *)mats.data(), bytes, hipMemcpyHostToDevice), "matrix upload");
-                for (int i = 0; i < m; ++i)
+                const float *devf = static_cast<const float *>(rdev_);
+                const double *devd = static_cast<const double *>(rdev_);
+                for (int i = 0; i < mcount; ++i)
                 {
+                    Launch &L = ls[i];
                     qsv::check(hipEventRecord(ev[2 * i], sim_.stream_), "hipEventRecord");
-                    if (f64_)
-                        launch_prepared128(sim_.stream_, state128_, n, pb128[i], static_cast<const double *>(rdev_));
+                    if (L.tile_bits && f64_)
+                        launch_prepared_stage128(sim_.stream_, state128_, n, L.st128, devd);
+                    else if (L.tile_bits)
+                        launch_prepared_stage(sim_.stream_, state_, n, L.st, devf);
+                    else if (f64_)
+                        launch_prepared128(sim_.stream_, state128_, n, L.b128, devd);
                     else
-                        launch_prepared(sim_.stream_, state_, n, pb[i], static_cast<const float *>(rdev_));
+                        launch_prepared(sim_.stream_, state_, n, L.b, devf);
                     qsv::check(hipEventRecord(ev[2 * i + 1], sim_.stream_), "hipEventRecord");
                 }
                 qsv::check(hipStreamSynchronize(sim_.stream_), "sync");
                 if (start < 0)
                     continue;
-                for (int i = 0; i < m; ++i)
+                for (int i = 0; i < mcount; ++i)
                 {
                     float ms = 0.f;
                     qsv::check(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]), "hipEventElapsedTime");
                     const int row = start + i;
-                    block_features(n, math_feature(), kq_, layout_block(gate_sets[i], blocks[i], kq_, n), (int)blocks[i].gates.size(), features + size_t(row) * kPlanFeatures);
+                    std::memcpy(features + size_t(row) * kPlanFeatures, ls[i].f, sizeof(ls[i].f));
                     log2_us[row] = float(std::log2(std::max(1e-3, double(ms) * 1e3)));
                 }
             }
@@ -937,10 +1029,9 @@ namespace qcore
         stage_bits_ = m;
     }
 
-    void FusedEngine::apply_stages(const std::vector<ResolvedGate> &gates)
+    void FusedEngine::apply_stages(const std::vector<ResolvedGate> &gates, int m)
     {
         const int n = sim_.n_;
-        const int m = tile_bits();
         const auto stages = plan_stages(gates, n, m, kq_);
         // one matrix buffer for the whole flush: FP32 (complex64 modes) or FP64 entries
         std::vector<float> mats;

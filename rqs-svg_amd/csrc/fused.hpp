@@ -162,7 +162,7 @@ namespace qcore
 
     private:
         void apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan);
-        void apply_stages(const std::vector<ResolvedGate> &gates);
+        void apply_stages(const std::vector<ResolvedGate> &gates, int m);
         void reserve_matrices(size_t bytes);
 
         Simulator &sim_;

@@ -9,15 +9,19 @@
 
 namespace qcore
 {
-    /* Features of one block launch:
-     *  0 n/32            1 kq/5             2 math: complex128 1, complex64 BF16x3 0.5, FP32 0
-     *  3 LDS-staged kernel (5 or 6 qubits outside the block)
-     *  4..8 block qubits ascending / 32 (missing entries n/32)
-     *  9 block qubits below 3 / 5        10 block qubits below 6 / 5
-     * 11 controls outside the block / 4   12 gates in the block / 32
-     * 13 (n - operands) / 32 = log2(columns)/32
-     * 14 lowest qubit outside the block / 8   15 constant 1 */
-    void block_features(int n, float math, int kq, const BlockLayout &lay, int gates, float *f)
+    /* Features of one launch (kPlanFeatures = 20):
+     *  0 n/32      1 kq/5      2 arithmetic: complex128 FP64 1, complex64 BF16x3 0.5, FP32 0
+     *  3 LDS-staged block kernel      4 stage kernel
+     *  5..9  block launch: block qubits ascending / 32 (missing entries n/32);
+     *        stage launch: the 5 highest tile qubits ascending / 32
+     * 10 block qubits below 3 / 5      11 block qubits below 6 / 5      (stages: 0)
+     * 12 controls outside the block / 4                                  (stages: 0)
+     * 13 gates / 32 (stages: gates / 128)
+     * 14 log2(columns) / 32 (stages: log2(tiles) / 32)
+     * 15 lowest qubit outside the block / 8                              (stages: 0)
+     * 16 blocks in the launch / 16     17 tile bits / 16 (blocks: 0)
+     * 18 gates per block / 32          19 constant 1 */
+    void block_features(int n, float math, int kq, const BlockLayout &lay, int gates, bool lds, float *f)
     {
         std::vector<int> q = lay.qubits;
         std::sort(q.begin(), q.end());
@@ -27,35 +31,66 @@ namespace qcore
                std::find(lay.pos_out.begin(), lay.pos_out.end(), lowest_free) != lay.pos_out.end() ||
                std::find(lay.neg_out.begin(), lay.neg_out.end(), lowest_free) != lay.neg_out.end())
             ++lowest_free;
-        const int free_q = n - (int)q.size() - outside;
+        std::fill(f, f + kPlanFeatures, 0.f);
         f[0] = n / 32.f;
         f[1] = kq / 5.f;
         f[2] = math;
-        f[3] = free_q >= (math == 1.f ? 6 : 5) ? 1.f : 0.f;
+        f[3] = lds ? 1.f : 0.f;
+        f[4] = 0.f;
         for (int i = 0; i < 5; ++i)
-            f[4 + i] = (i < (int)q.size() ? q[i] : n) / 32.f;
+            f[5 + i] = (i < (int)q.size() ? q[i] : n) / 32.f;
         int below3 = 0, below6 = 0;
         for (int x : q)
         {
             below3 += x < 3;
             below6 += x < 6;
         }
-        f[9] = below3 / 5.f;
-        f[10] = below6 / 5.f;
-        f[11] = outside / 4.f;
-        f[12] = gates / 32.f;
-        f[13] = (n - (int)q.size() - outside) / 32.f;
-        f[14] = lowest_free / 8.f;
-        f[15] = 1.f;
+        f[10] = below3 / 5.f;
+        f[11] = below6 / 5.f;
+        f[12] = outside / 4.f;
+        f[13] = gates / 32.f;
+        f[14] = (n - (int)q.size() - outside) / 32.f;
+        f[15] = lowest_free / 8.f;
+        f[16] = 1.f / 16.f;
+        f[17] = 0.f;
+        f[18] = gates / 32.f;
+        f[19] = 1.f;
     }
 
-    std::vector<PlanOption> candidate_options(int kq)
+    void stage_features(int n, float math, int kq, const std::vector<int> &tile, int nblocks, int gates, float *f)
+    {
+        std::fill(f, f + kPlanFeatures, 0.f);
+        const int m = (int)tile.size();
+        f[0] = n / 32.f;
+        f[1] = kq / 5.f;
+        f[2] = math;
+        f[3] = 0.f;
+        f[4] = 1.f;
+        for (int i = 0; i < 5; ++i)
+            f[5 + i] = (m - 5 + i >= 0 ? tile[m - 5 + i] : n) / 32.f;
+        f[13] = gates / 128.f;
+        f[14] = (n - m) / 32.f;
+        f[16] = nblocks / 16.f;
+        f[17] = m / 16.f;
+        f[18] = nblocks ? float(gates) / nblocks / 32.f : 0.f;
+        f[19] = 1.f;
+    }
+
+    bool block_uses_lds(const BlockLayout &lay, int n, bool f64)
+    {
+        const int free_q = n - (int)lay.qubits.size() - (int)(lay.pos_out.size() + lay.neg_out.size());
+        return free_q >= (f64 ? 6 : 5);
+    }
+
+    std::vector<PlanOption> candidate_options(int kq, const std::vector<int> &tile_bits)
     {
         std::vector<PlanOption> out;
         for (int k = std::max(2, kq - 2); k <= kq; ++k)
             for (int w : {4, 64})
                 for (int rule : {0, 1, 2})
-                    out.push_back(PlanOption{k, w, rule});
+                    out.push_back(PlanOption{0, k, w, rule});
+        for (int m : tile_bits)
+            out.push_back(PlanOption{m, kq, 64, 0});
         return out;
     }
 

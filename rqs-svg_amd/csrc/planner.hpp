@@ -11,10 +11,15 @@
 
 namespace qcore
 {
-    constexpr int kPlanFeatures = 16;
+    constexpr int kPlanFeatures = 20;
 
-    // Features of one block launch (all in [0, ~1]); see planner.cpp for the list.
-    void block_features(int n, float math, int kq, const BlockLayout &lay, int gates, float *f);
+    // Features of one launch that sweeps the state (all in [0, ~1]; list in planner.cpp):
+    // a block launch (lds: the LDS-staged block kernel) or a stage launch.
+    void block_features(int n, float math, int kq, const BlockLayout &lay, int gates, bool lds, float *f);
+    void stage_features(int n, float math, int kq, const std::vector<int> &tile, int nblocks, int gates, float *f);
+    // Whether a block with this layout runs in the LDS-staged block kernel (enough free
+    // qubits outside the block and its controls for the tile's column bits).
+    bool block_uses_lds(const BlockLayout &lay, int n, bool f64);
 
     class CostModel
     {
@@ -48,10 +53,13 @@ namespace qcore
 
     struct PlanOption
     {
-        int k;      // qubits per block
-        int window; // blocks examined after the dependency frontier
-        int rule;   // 0: least growth, 1: fullest block, 2: earliest block that fits
+        int stage_bits; // > 0: stages on tiles of 2^stage_bits amplitudes; 0: one launch per block
+        int k;          // qubits per block
+        int window;     // blocks examined after the dependency frontier (block plans)
+        int rule;       // 0: least growth, 1: fullest block, 2: earliest block that fits
     };
     std::vector<PlannedBlock> plan_with(const std::vector<ResolvedGate> &gates, const PlanOption &opt);
-    std::vector<PlanOption> candidate_options(int kq);
+    // Block plans: k from max(2, kq - 2) to kq x windows {4, 64} x rules {0, 1, 2}; stage
+    // plans: each tile size in `tile_bits` (block size kq).
+    std::vector<PlanOption> candidate_options(int kq, const std::vector<int> &tile_bits);
 } // namespace qcore

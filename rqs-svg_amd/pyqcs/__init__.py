@@ -438,8 +438,16 @@ class Mlp:
             pass
 
 
+def planner_num_features(library):
+    F = ctypes.c_int()
+    library.check(library.lib.qcs_ext_planner_num_features(ctypes.byref(F)), "planner_num_features")
+    return F.value
+
+
 def planner_record(sim, count, seed):
-    F = 16
+    """Times `count` random launches (blocks, and stages on large enough states) on the
+    simulator's state: (features [count, F], log2 microseconds [count])."""
+    F = planner_num_features(sim.L)
     X = np.empty((count, F), dtype=np.float32)
     y = np.empty(count, dtype=np.float32)
     sim._call(sim.lib.qcs_ext_planner_record(sim.h, count, seed, X.ctypes.data, y.ctypes.data), "planner_record")

@@ -102,21 +102,28 @@ def test_gpu_mlp_matches_oracle(gpu, oracle):
     assert l1 < 0.3 * lg
 
 
+def test_feature_count(gpu, oracle):
+    assert pyqcs.planner_num_features(oracle) == pyqcs.planner_num_features(gpu) == 20
+
+
 @pytest.mark.gpu
-def test_planner_record_train_and_plan(gpu, oracle):
+@pytest.mark.parametrize("precision,tol", [(1, 1e-6), (2, 1e-12)])
+def test_planner_record_train_and_plan(gpu, oracle, precision, tol):
     n = 20
-    with pyqcs.Simulator(gpu, n, precision=1, block_qubits=5) as sim:
+    F = pyqcs.planner_num_features(gpu)
+    with pyqcs.Simulator(gpu, n, precision=precision, block_qubits=5) as sim:
         X, y = pyqcs.planner_record(sim, 400, seed=9)
-    assert X.shape == (400, 16) and np.all(np.isfinite(y))
-    assert np.all(X[:, 15] == 1) and np.all(X[:, 0] == np.float32(n / 32))
-    model = pyqcs.Mlp(gpu, 16, seed=3)
+    assert X.shape == (400, F) and np.all(np.isfinite(y))
+    assert np.all(X[:, F - 1] == 1) and np.all(X[:, 0] == np.float32(n / 32))
+    stages = X[:, 4] == 1
+    assert 0 < stages.sum() < len(stages)  # both launch kinds recorded
+    model = pyqcs.Mlp(gpu, F, seed=3)
     loss = model.train(X, y, epochs=100, batch=64, lr=3e-3, seed=1)
     assert loss < np.var(y)  # better than predicting the mean
-    circ = circuits.qft(n)
+    circ = circuits.qft(n) + circuits.brick(n, 4, seed=3)
     states = []
     for use in (False, True):
-        # the learned choice is among one-sweep-per-block plans
-        with pyqcs.Simulator(gpu, n, precision=1, block_qubits=5, stage_bits=0) as sim:
+        with pyqcs.Simulator(gpu, n, precision=precision, block_qubits=5) as sim:
             if use:
                 pyqcs.set_planner(sim, model)
             sim.run(circ)
@@ -128,4 +135,4 @@ def test_planner_record_train_and_plan(gpu, oracle):
         o.run(circ)
         want = o.statevector()
     for s in states:
-        assert fidelity(s, want) >= 1 - 1e-6
+        assert fidelity(s, want) >= 1 - tol
