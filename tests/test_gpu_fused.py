@@ -230,9 +230,11 @@ def test_stages_vs_block_launches(gpu, precision):
             f.run(circ[len(circ) // 2:])
             res[m] = (f.statevector(), f.fusion_stats(), a)
     base = res[0][0]
+    # different block decompositions of the same circuit: FP32 and BF16x3 rounding apart
+    tol = 1e-9 if precision == 1 else 1e-7
     for m in (12, 13):
         assert res[m][2] == res[0][2]
-        assert fidelity(res[m][0], base) >= 1 - 1e-9
+        assert fidelity(res[m][0], base) >= 1 - tol
         assert res[m][1][0] == res[0][1][0]        # same gates applied
         assert res[m][1][1] * 2 <= res[0][1][1]    # at most half the sweeps
     say(f"launches: per block {res[0][1][1]}, m=12 {res[12][1][1]}, m=13 {res[13][1][1]}")
