@@ -135,8 +135,13 @@ int qcs_ext_batch_destroy(qcs_ext_batch *batch);
 /* gate_names[i] is a qcs.h gate name with one target (e.g. "ry", "rz", "x", "u3");
  * controls[i] is a control qubit or -1; params holds 4 doubles per gate (unused ones 0). */
 int qcs_ext_batch_add_circuit(qcs_ext_batch *batch, bit_num_t num_qubits, int32_t num_gates, const char *const *gate_names, const bit_num_t *targets, const bit_num_t *controls, const double *params);
-int qcs_ext_batch_set_tuning(qcs_ext_batch *batch, int lds_max_qubits, int block_size, int block_qubits, int streams);
-/* Searches the tuning parameters on this batch; writes the chosen 4 values to tuning. */
+/* Tuning: circuits of up to lds_max_qubits (0..14) qubits run in the LDS-resident kernel with
+ * block_size (256, 512, 1024) threads per workgroup; larger ones as fused blocks of
+ * block_qubits (4, 5) qubits, spread over `streams` (1..4) graph branches, grouped into stages
+ * on LDS tiles of 2^stage_bits amplitudes when they have at least stage_bits + 4 qubits
+ * (stage_bits 12 or 13; 0: one launch per block). */
+int qcs_ext_batch_set_tuning(qcs_ext_batch *batch, int lds_max_qubits, int block_size, int block_qubits, int streams, int stage_bits);
+/* Searches the tuning parameters on this batch; writes the chosen 5 values to tuning. */
 int qcs_ext_batch_autotune(qcs_ext_batch *batch, int *tuning);
 /* expz receives sum(num_qubits) doubles, circuit by circuit; seconds = device time. */
 int qcs_ext_batch_run(qcs_ext_batch *batch, double *expz, uint64_t count, double *seconds);

@@ -1468,13 +1468,13 @@ extern "C"
             b->total_qubits += nq;
         });
     }
-    int qcs_ext_batch_set_tuning(qcs_ext_batch *b, int, int, int, int) { return guard([&] { if (!b) throw std::runtime_error("batch is NULL"); }); }
+    int qcs_ext_batch_set_tuning(qcs_ext_batch *b, int, int, int, int, int) { return guard([&] { if (!b) throw std::runtime_error("batch is NULL"); }); }
     int qcs_ext_batch_autotune(qcs_ext_batch *b, int *tuning)
     {
         return guard([&] {
             if (!b || !tuning)
                 throw std::runtime_error("batch is NULL");
-            tuning[0] = tuning[1] = tuning[2] = tuning[3] = 0;
+            tuning[0] = tuning[1] = tuning[2] = tuning[3] = tuning[4] = 0;
         });
     }
     int qcs_ext_batch_run(qcs_ext_batch *b, double *expz, uint64_t count, double *seconds)

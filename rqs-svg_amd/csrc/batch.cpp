@@ -123,6 +123,8 @@ namespace qcore
             throw std::runtime_error("batch: block qubits must be 4 or 5");
         if (t.streams < 1 || t.streams > 4)
             throw std::runtime_error("batch: streams must be in [1, 4]");
+        if (t.stage_bits != 0 && t.stage_bits != 12 && t.stage_bits != 13)
+            throw std::runtime_error("batch: stage bits must be 0, 12 or 13");
         release();
         tuning_ = t;
     }
@@ -189,12 +191,18 @@ namespace qcore
             small_max_n = std::max(small_max_n, C.n);
         }
         // ---- large circuits: fused blocks on HBM states ----
+        struct LargeItem
+        {
+            bool stage;
+            PreparedStage st;
+            PreparedBlock b;
+        };
         struct LargeRun
         {
             int ci;
             uint64_t state_offset;
             size_t partial_offset;
-            std::vector<PreparedBlock> blocks;
+            std::vector<LargeItem> items;
         };
         std::vector<LargeRun> runs;
         std::vector<float> mats;
@@ -218,7 +226,25 @@ namespace qcore
             if (C.n < kq)
                 throw std::runtime_error("batch: circuit too small for the HBM path");
             LargeRun lr{ci, state_total, partial_total, {}};
-            lr.blocks = prepare_blocks(rg, plan_greedy(rg, kq), kq, C.n, mats);
+            if (T.stage_bits > 0 && C.n >= T.stage_bits + 4)
+            {
+                for (const auto &st : plan_stages(rg, C.n, T.stage_bits, kq))
+                {
+                    if (!st.tile.empty())
+                    {
+                        lr.items.push_back(LargeItem{true, prepare_stage(rg, st, kq, T.stage_bits, C.n, mats, false), {}});
+                        continue;
+                    }
+                    std::vector<ResolvedGate> one{rg[st.gates[0]]};
+                    for (auto &b : prepare_blocks(one, plan_greedy(one, kq), kq, C.n, mats))
+                        lr.items.push_back(LargeItem{false, {}, b});
+                }
+            }
+            else
+            {
+                for (auto &b : prepare_blocks(rg, plan_greedy(rg, kq), kq, C.n, mats))
+                    lr.items.push_back(LargeItem{false, {}, b});
+            }
             state_total += uint64_t(1) << C.n;
             partial_total += size_t(qsv::zsum_blocks(C.n)) * 32;
             runs.push_back(std::move(lr));
@@ -282,8 +308,13 @@ namespace qcore
                 hipStream_t s = streams_[ns == 1 ? 0 : 1 + oi % (ns - 1)];
                 qsv::c64 *st = d_states_ + lr.state_offset;
                 qsv::launch_fill_zero_c64(s, st, uint64_t(1) << C.n);
-                for (auto &b : lr.blocks)
-                    launch_prepared(s, st, C.n, b, d_mats_);
+                for (auto &it : lr.items)
+                {
+                    if (it.stage)
+                        launch_prepared_stage(s, st, C.n, it.st, d_mats_);
+                    else
+                        launch_prepared(s, st, C.n, it.b, d_mats_);
+                }
                 qsv::launch_zsum(s, st, C.n, d_partials_ + lr.partial_offset, d_out_ + C.out_offset);
             }
             for (int i = 1; i < ns; ++i)
@@ -402,6 +433,7 @@ namespace qcore
         try_set([](BatchTuning &t, int v) { t.block_size = v; }, {256, 512, 1024});
         try_set([](BatchTuning &t, int v) { t.block_qubits = v; }, {4, 5});
         try_set([](BatchTuning &t, int v) { t.streams = v; }, {1, 2, 4});
+        try_set([](BatchTuning &t, int v) { t.stage_bits = v; }, {0, 12, 13});
         set_tuning(best);
         return best;
     }

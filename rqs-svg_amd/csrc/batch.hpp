@@ -26,6 +26,8 @@ namespace qcore
         int block_size = 512;    // threads per workgroup of that kernel
         int block_qubits = 5;    // fused block size for the circuits in HBM
         int streams = 4;         // graph branches the HBM circuits are spread over
+        int stage_bits = 12;     // HBM circuits of >= stage_bits + 4 qubits run as stages on
+                                 // LDS tiles of 2^stage_bits amplitudes (0: one launch per block)
     };
 
     class Batch

@@ -576,7 +576,7 @@ extern "C"
             b->core->add_circuit(nq, gates);
         });
     }
-    int qcs_ext_batch_set_tuning(qcs_ext_batch *b, int lds_max_qubits, int block_size, int block_qubits, int streams)
+    int qcs_ext_batch_set_tuning(qcs_ext_batch *b, int lds_max_qubits, int block_size, int block_qubits, int streams, int stage_bits)
     {
         return guard([&] {
             if (!b || !b->core)
@@ -586,6 +586,7 @@ extern "C"
             t.block_size = block_size;
             t.block_qubits = block_qubits;
             t.streams = streams;
+            t.stage_bits = stage_bits;
             b->core->set_tuning(t);
         });
     }
@@ -599,6 +600,7 @@ extern "C"
             tuning[1] = t.block_size;
             tuning[2] = t.block_qubits;
             tuning[3] = t.streams;
+            tuning[4] = t.stage_bits;
         });
     }
     int qcs_ext_batch_run(qcs_ext_batch *b, double *expz, uint64_t count, double *seconds)

@@ -147,7 +147,7 @@ class Library:
         lib.qcs_ext_batch_create.argtypes = [P(vp)]
         lib.qcs_ext_batch_destroy.argtypes = [vp]
         lib.qcs_ext_batch_add_circuit.argtypes = [vp, bit_num_t, ctypes.c_int32, P(ctypes.c_char_p), P(bit_num_t), P(bit_num_t), P(ctypes.c_double)]
-        lib.qcs_ext_batch_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.qcs_ext_batch_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         lib.qcs_ext_batch_autotune.argtypes = [vp, P(ctypes.c_int)]
         lib.qcs_ext_batch_run.argtypes = [vp, vp, u64, P(ctypes.c_double)]
         lib.qcs_ext_batch_run_plain.argtypes = [vp, vp, u64, P(ctypes.c_double)]
@@ -491,13 +491,14 @@ class Batch:
         self.L.check(self.lib.qcs_ext_batch_add_circuit(self.h, n, k, names, tg, ct, pa), "batch_add_circuit")
         self.sizes.append(n)
 
-    def set_tuning(self, lds_max_qubits=14, block_size=512, block_qubits=5, streams=4):
-        self.L.check(self.lib.qcs_ext_batch_set_tuning(self.h, lds_max_qubits, block_size, block_qubits, streams), "batch_set_tuning")
+    def set_tuning(self, lds_max_qubits=14, block_size=512, block_qubits=5, streams=4, stage_bits=12):
+        self.L.check(self.lib.qcs_ext_batch_set_tuning(self.h, lds_max_qubits, block_size, block_qubits, streams, stage_bits), "batch_set_tuning")
 
     def autotune(self):
-        t = (ctypes.c_int * 4)()
+        t = (ctypes.c_int * 5)()
         self.L.check(self.lib.qcs_ext_batch_autotune(self.h, t), "batch_autotune")
-        return {"lds_max_qubits": t[0], "block_size": t[1], "block_qubits": t[2], "streams": t[3]}
+        return {"lds_max_qubits": t[0], "block_size": t[1], "block_qubits": t[2], "streams": t[3],
+                "stage_bits": t[4]}
 
     def run(self):
         """Returns (list of <Z> arrays per circuit, device seconds)."""

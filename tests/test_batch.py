@@ -48,7 +48,7 @@ def test_gpu_batch_vs_oracle(gpu, oracle):
     for n, g in work:
         ref.add(n, g)
     want, _ = ref.run()
-    for tuning in [(14, 512, 5, 4), (12, 256, 4, 1), (13, 1024, 5, 2), (0, 256, 5, 4)]:
+    for tuning in [(14, 512, 5, 4, 12), (12, 256, 4, 1, 0), (13, 1024, 5, 2, 12), (0, 256, 5, 4, 12), (0, 512, 4, 3, 0)]:
         b = pyqcs.Batch(gpu)
         for n, g in work:
             b.add(n, g)
@@ -74,6 +74,7 @@ def test_gpu_batch_autotune(gpu):
     t = b.autotune()
     assert t["lds_max_qubits"] in (12, 13, 14) and t["block_size"] in (256, 512, 1024)
     assert t["block_qubits"] in (4, 5) and t["streams"] in (1, 2, 4)
+    assert t["stage_bits"] in (0, 12, 13)
     after, _ = b.run()
     assert max(np.max(np.abs(x - y)) for x, y in zip(before, after)) < 2e-5
 
