@@ -264,3 +264,21 @@ def test_stages_complex128_match_oracle(gpu, oracle):
         assert np.max(np.abs(got - want)) <= 1e-13
         assert fidelity(got, want) >= 1 - 1e-12
         assert gates == len(circ)
+
+
+@pytest.mark.parametrize("precision", [1, 3])
+def test_stage_launch_chunks(gpu, oracle, precision):
+    # deep circuits on few qubits give stages with more blocks than one launch's LDS holds
+    # (5 blocks of 5 qubits with 2^12 tiles): those run as several launches over the tile
+    n = 16
+    circ = circuits.brick(n, 40, seed=31) + circuits.qft(n)
+    with pyqcs.Simulator(gpu, n, precision=precision, block_qubits=5, stage_bits=12) as f:
+        f.run(circ)
+        got = f.statevector()
+        gates, launches = f.fusion_stats()
+    with pyqcs.Simulator(oracle, n) as o:
+        o.run(circ)
+        want = o.statevector()
+    say(f"chunks n={n} precision={precision}: fidelity {fidelity(got, want):.12f} launches {launches}")
+    assert fidelity(got, want) >= 1 - FID_TOL and np.max(np.abs(got - want)) <= AMP_TOL
+    assert gates == len(circ)
