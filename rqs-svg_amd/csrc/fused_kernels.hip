@@ -661,7 +661,8 @@ namespace qsv
 
         // complex128 stage: a tile of 2^12 amplitudes (64 KiB) in LDS, tile bits as in
         // k_stage; each block of 4 qubits has 256 columns, one per thread, applied in FP64
-        // with the 16x16 matrix read through uniform loads.
+        // with the 16x16 matrix read through uniform loads; the next tile is prefetched into
+        // registers while the blocks run.
         constexpr int kStage128Bits = 12;
 
         __global__ __launch_bounds__(kThreads) void k_stage128(c128 *__restrict__ s, uint64_t ntiles, Subspace tsub, Stage128Op op)
@@ -676,20 +677,40 @@ namespace qsv
             for (int b = 0; b < 8; ++b)
                 if ((t >> b) & 1)
                     g_t += op.gw[b];
-            for (uint64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x)
+            auto tile_offset = [&](int i) {
+                uint64_t gi = 0;
+#pragma unroll
+                for (int b = 8; b < kStage128Bits; ++b)
+                    if (((i * kThreads) >> b) & 1)
+                        gi += op.gw[b];
+                return gi;
+            };
+            // the next tile's loads stay in flight in registers while the blocks run (the
+            // matrices come through scalar loads, the tile through LDS: neither waits on them)
+            v2d pf[LOADS];
+            uint64_t tl = blockIdx.x;
+            if (tl < ntiles)
             {
                 const uint64_t base = expand(tl, tsub) + g_t;
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
-                {
-                    uint64_t gi = 0;
+                    pf[i] = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(s + (base + tile_offset(i))));
+            }
+            for (; tl < ntiles; tl += gridDim.x)
+            {
+                const uint64_t base = expand(tl, tsub) + g_t;
 #pragma unroll
-                    for (int b = 8; b < kStage128Bits; ++b)
-                        if (((i * kThreads) >> b) & 1)
-                            gi += op.gw[b];
-                    tile[t + i * kThreads] = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(s + (base + gi)));
-                }
+                for (int i = 0; i < LOADS; ++i)
+                    tile[t + i * kThreads] = pf[i];
                 __syncthreads();
+                const uint64_t next = tl + gridDim.x;
+                if (next < ntiles)
+                {
+                    const uint64_t nbase = expand(next, tsub) + g_t;
+#pragma unroll
+                    for (int i = 0; i < LOADS; ++i)
+                        pf[i] = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(s + (nbase + tile_offset(i))));
+                }
                 for (int bi = 0; bi < op.nblocks; ++bi)
                 {
                     const StageBlock &B = op.blocks[bi];
@@ -732,14 +753,7 @@ namespace qsv
                 }
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
-                {
-                    uint64_t gi = 0;
-#pragma unroll
-                    for (int b = 8; b < kStage128Bits; ++b)
-                        if (((i * kThreads) >> b) & 1)
-                            gi += op.gw[b];
-                    __builtin_nontemporal_store(tile[t + i * kThreads], reinterpret_cast<v2d *>(s + (base + gi)));
-                }
+                    __builtin_nontemporal_store(tile[t + i * kThreads], reinterpret_cast<v2d *>(s + (base + tile_offset(i))));
                 __syncthreads();
             }
         }
