@@ -311,7 +311,7 @@ namespace qcore
         if (!model_)
         {
             if (m0 > 0)
-                apply_stages(gates, m0);
+                apply_stages(gates, m0, kq_);
             else
                 apply_blocks(gates, plan_greedy(gates, k_));
             return;
@@ -328,7 +328,7 @@ namespace qcore
                 tiles.push_back(m);
         std::vector<PlanOption> opts;
         opts.push_back(PlanOption{m0, kq_, 64, 0});
-        for (const auto &o : candidate_options(kq_, tiles))
+        for (const auto &o : candidate_options(kq_, tiles, stage_block_sizes()))
             if (o.k <= k_ || o.stage_bits > 0)
                 opts.push_back(o);
         std::vector<float> X;
@@ -349,20 +349,19 @@ namespace qcore
                     add_block(gates, b);
                 continue;
             }
-            for (const auto &st : plan_stages(gates, n, o.stage_bits, kq_))
+            for (const auto &st : plan_stages(gates, n, o.stage_bits, o.k))
             {
                 std::vector<ResolvedGate> sub;
                 for (int gi : st.gates)
                     sub.push_back(gates[gi]);
-                const auto blocks = plan_greedy(sub, kq_);
                 if (st.tile.empty())
                 {
-                    for (const auto &b : blocks)
+                    for (const auto &b : plan_greedy(sub, k_))
                         add_block(sub, b);
                     continue;
                 }
                 float f[kPlanFeatures];
-                stage_features(n, math, kq_, st.tile, (int)blocks.size(), (int)st.gates.size(), f);
+                stage_features(n, math, o.k, st.tile, (int)plan_greedy(sub, o.k).size(), (int)st.gates.size(), f);
                 X.insert(X.end(), f, f + kPlanFeatures);
             }
         }
@@ -391,7 +390,7 @@ namespace qcore
         }
         const PlanOption &o = opts[best];
         if (o.stage_bits > 0)
-            apply_stages(gates, o.stage_bits);
+            apply_stages(gates, o.stage_bits, o.k);
         else
             apply_blocks(gates, plan_with(gates, o));
     }
@@ -487,6 +486,8 @@ namespace qcore
                     if (!tiles.empty() && (rnd() & 1))
                     {
                         const int m = tiles[rnd() % tiles.size()];
+                        const auto ks = stage_block_sizes();
+                        const int kb = ks[rnd() % ks.size()];
                         std::vector<int> upper(all.begin() + 4, all.end());
                         auto extra = random_subset(upper, m - 4);
                         PlannedStage st;
@@ -496,22 +497,22 @@ namespace qcore
                         std::vector<ResolvedGate> gs;
                         const int nb = 1 + int(rnd() % 12);
                         for (int b = 0; b < nb; ++b)
-                            random_gates(random_subset(st.tile, 1 + int(rnd() % kq_)), gs);
+                            random_gates(random_subset(st.tile, 1 + int(rnd() % kb)), gs);
                         st.gates.resize(gs.size());
                         std::iota(st.gates.begin(), st.gates.end(), 0);
                         L.tile_bits = m;
                         int nblocks;
                         if (f64_)
                         {
-                            L.st128 = prepare_stage128(gs, st, n, mats64);
+                            L.st128 = prepare_stage128(gs, st, n, mats64, kb);
                             nblocks = L.st128.nblocks;
                         }
                         else
                         {
-                            L.st = prepare_stage(gs, st, kq_, m, n, mats, x3_);
+                            L.st = prepare_stage(gs, st, kb, m, n, mats, x3_);
                             nblocks = L.st.nblocks;
                         }
-                        stage_features(n, math, kq_, st.tile, nblocks, (int)gs.size(), L.f);
+                        stage_features(n, math, kb, st.tile, nblocks, (int)gs.size(), L.f);
                         continue;
                     }
                     std::vector<ResolvedGate> gs;
@@ -972,16 +973,19 @@ namespace qcore
         }
     }
 
-    PreparedStage128 prepare_stage128(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int n, std::vector<double> &mats)
+    PreparedStage128 prepare_stage128(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int n, std::vector<double> &mats, int kq)
     {
         constexpr int m = 12;
+        if (kq != 3 && kq != 4)
+            throw std::runtime_error("stage128: block qubits must be 3 or 4");
+        const int K = 1 << kq;
         if ((int)st.tile.size() != m)
             throw std::runtime_error("stage128: tile size mismatch");
         std::vector<ResolvedGate> sub;
         sub.reserve(st.gates.size());
         for (int gi : st.gates)
             sub.push_back(gates[gi]);
-        const auto plan = plan_greedy(sub, 4);
+        const auto plan = plan_greedy(sub, kq);
         std::vector<qsv::StageBlock> desc;
         for (const auto &pb : plan)
         {
@@ -989,19 +993,19 @@ namespace qcore
                 throw std::runtime_error("stage128: wide gate inside a stage");
             BlockLayout lay;
             lay.qubits = pb.qubits;
-            for (int i = m - 1; i >= 0 && (int)lay.qubits.size() < 4; --i)
+            for (int i = m - 1; i >= 0 && (int)lay.qubits.size() < kq; --i)
                 if (local_pos(lay.qubits, st.tile[i]) < 0)
                     lay.qubits.push_back(st.tile[i]);
             const auto M = block_matrix(sub, pb, lay);
             qsv::StageBlock sb{};
             sb.matrix = uint32_t(mats.size());
-            for (int i = 0; i < 256; ++i)
+            for (int i = 0; i < K * K; ++i)
             {
                 mats.push_back(M[i].real());
                 mats.push_back(M[i].imag());
             }
             std::vector<char> is_block(m, 0);
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < kq; ++j)
             {
                 const int b = local_pos(st.tile, lay.qubits[j]);
                 if (b < 0)
@@ -1009,7 +1013,7 @@ namespace qcore
                 sb.w[j] = 1 << b;
                 is_block[b] = 1;
             }
-            int j = 4;
+            int j = kq;
             for (int b = 0; b < m; ++b)
                 if (!is_block[b])
                     sb.w[j++] = 1 << b;
@@ -1027,6 +1031,7 @@ namespace qcore
         p.tsub.set_bits = 0;
         p.tsub.nops = m;
         p.op.m = m;
+        p.op.kq = kq;
         p.op.nblocks = p.nblocks;
         for (int b = 0; b < m; ++b)
         {
@@ -1051,10 +1056,10 @@ namespace qcore
         stage_bits_ = m;
     }
 
-    void FusedEngine::apply_stages(const std::vector<ResolvedGate> &gates, int m)
+    void FusedEngine::apply_stages(const std::vector<ResolvedGate> &gates, int m, int k)
     {
         const int n = sim_.n_;
-        const auto stages = plan_stages(gates, n, m, kq_);
+        const auto stages = plan_stages(gates, n, m, k);
         // one matrix buffer for the whole flush: FP32 (complex64 modes) or FP64 entries
         std::vector<float> mats;
         std::vector<double> mats64;
@@ -1074,9 +1079,9 @@ namespace qcore
             if (it.stage)
             {
                 if (f64_)
-                    it.st128 = prepare_stage128(gates, s, n, mats64);
+                    it.st128 = prepare_stage128(gates, s, n, mats64, k);
                 else
-                    it.st = prepare_stage(gates, s, kq_, m, n, mats, x3_);
+                    it.st = prepare_stage(gates, s, k, m, n, mats, x3_);
             }
             else
             {

@@ -106,8 +106,8 @@ namespace qcore
     // The LDS image of one block matrix for the complex64 stage kernel (StageOp).
     void stage_image(const std::vector<cd> &M, int kq, bool x3, unsigned char *out);
 
-    // complex128 stages (tile of 2^12, FP64 blocks of 4 qubits): matrices and descriptors
-    // appended to `mats` as doubles.
+    // complex128 stages (tile of 2^12, FP64 blocks of kq = 3 or 4 qubits): matrices and
+    // descriptors appended to `mats` as doubles.
     struct PreparedStage128
     {
         int gates = 0;
@@ -117,7 +117,7 @@ namespace qcore
         qsv::Subspace tsub{};
         qsv::Stage128Op op{};
     };
-    PreparedStage128 prepare_stage128(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int n, std::vector<double> &mats);
+    PreparedStage128 prepare_stage128(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int n, std::vector<double> &mats, int kq = 4);
     void launch_prepared_stage128(hipStream_t s, qsv::c128 *state, int n, PreparedStage128 &p, const double *mats_dev);
 
     class CostModel;
@@ -153,6 +153,15 @@ namespace qcore
             return (stage_bits_ > 0 && sim_.n_ >= m + 4) ? m : 0;
         }
         bool uses_stages() const { return tile_bits() > 0; }
+        // block sizes the stage kernels take for this arithmetic (the planner's choices)
+        std::vector<int> stage_block_sizes() const
+        {
+            std::vector<int> ks;
+            for (int k : f64_ ? std::vector<int>{3, 4} : std::vector<int>{4, 5})
+                if (k <= kq_)
+                    ks.push_back(k);
+            return ks;
+        }
         int block_qubits() const { return k_; }
         uint64_t blocks_applied() const { return blocks_applied_; }
         uint64_t gates_applied() const { return gates_applied_; }
@@ -167,7 +176,7 @@ namespace qcore
 
     private:
         void apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan);
-        void apply_stages(const std::vector<ResolvedGate> &gates, int m);
+        void apply_stages(const std::vector<ResolvedGate> &gates, int m, int k);
         void reserve_matrices(size_t bytes);
 
         Simulator &sim_;

@@ -660,16 +660,20 @@ namespace qsv
         }
 
         // complex128 stage: a tile of 2^12 amplitudes (64 KiB) in LDS, tile bits as in
-        // k_stage; each block of 4 qubits has 256 columns, one per thread, applied in FP64
+        // k_stage; a block of KQ = 3 or 4 qubits has 2^(12-KQ) columns, one or two per thread,
+        // applied in FP64
         // with the 16x16 matrix read through uniform loads; the next tile is prefetched into
         // registers while the blocks run.
         constexpr int kStage128Bits = 12;
 
+        template <int KQ>
         __global__ __launch_bounds__(kThreads) void k_stage128(c128 *__restrict__ s, uint64_t ntiles, Subspace tsub, Stage128Op op)
         {
             typedef double v2d __attribute__((ext_vector_type(2)));
             constexpr int NT = 1 << kStage128Bits;
             constexpr int LOADS = NT / kThreads;
+            constexpr int K = 1 << KQ;
+            constexpr int PER_THREAD = (NT >> KQ) / kThreads; // columns per thread and block
             __shared__ __attribute__((aligned(16))) v2d tile[NT];
             const int t = threadIdx.x;
             uint64_t g_t = 0;
@@ -715,39 +719,44 @@ namespace qsv
                 {
                     const StageBlock &B = op.blocks[bi];
                     const double *__restrict__ Mx = op.M + B.matrix;
-                    int cbase = 0;
 #pragma unroll
-                    for (int j = 0; j < kStage128Bits - 4; ++j)
-                        if ((t >> j) & 1)
-                            cbase += B.w[4 + j];
-                    int off[16];
-#pragma unroll
-                    for (int e = 0; e < 16; ++e)
+                    for (int cc = 0; cc < PER_THREAD; ++cc)
                     {
-                        off[e] = cbase;
+                        const int col = t + cc * kThreads;
+                        int cbase = 0;
 #pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            if ((e >> j) & 1)
-                                off[e] += B.w[j];
-                    }
-                    v2d x[16];
+                        for (int j = 0; j < kStage128Bits - KQ; ++j)
+                            if ((col >> j) & 1)
+                                cbase += B.w[KQ + j];
+                        int off[K];
 #pragma unroll
-                    for (int k = 0; k < 16; ++k)
-                        x[k] = tile[off[k]];
-#pragma unroll
-                    for (int r = 0; r < 16; ++r)
-                    {
-                        double yr = 0.0, yi = 0.0;
-#pragma unroll
-                        for (int k = 0; k < 16; ++k)
+                        for (int e = 0; e < K; ++e)
                         {
-                            const double mr = Mx[2 * (16 * r + k)], mi = Mx[2 * (16 * r + k) + 1];
-                            yr = __builtin_fma(mr, x[k].x, yr);
-                            yr = __builtin_fma(-mi, x[k].y, yr);
-                            yi = __builtin_fma(mr, x[k].y, yi);
-                            yi = __builtin_fma(mi, x[k].x, yi);
+                            off[e] = cbase;
+#pragma unroll
+                            for (int j = 0; j < KQ; ++j)
+                                if ((e >> j) & 1)
+                                    off[e] += B.w[j];
                         }
-                        tile[off[r]] = v2d{yr, yi};
+                        v2d x[K];
+#pragma unroll
+                        for (int k = 0; k < K; ++k)
+                            x[k] = tile[off[k]];
+#pragma unroll
+                        for (int r = 0; r < K; ++r)
+                        {
+                            double yr = 0.0, yi = 0.0;
+#pragma unroll
+                            for (int k = 0; k < K; ++k)
+                            {
+                                const double mr = Mx[2 * (K * r + k)], mi = Mx[2 * (K * r + k) + 1];
+                                yr = __builtin_fma(mr, x[k].x, yr);
+                                yr = __builtin_fma(-mi, x[k].y, yr);
+                                yi = __builtin_fma(mr, x[k].y, yi);
+                                yi = __builtin_fma(mi, x[k].x, yi);
+                            }
+                            tile[off[r]] = v2d{yr, yi};
+                        }
                     }
                     __syncthreads();
                 }
@@ -862,6 +871,8 @@ namespace qsv
     {
         if (op.m != kStage128Bits)
             throw std::runtime_error("stage128: tile bits must be 12");
+        if (op.kq != 3 && op.kq != 4)
+            throw std::runtime_error("stage128: block qubits must be 3 or 4");
         if (n <= op.m || tsub.nops != op.m)
             throw std::runtime_error("stage128: the tile must be a proper part of the state");
         for (int k = 0; k < tsub.nops; ++k)
@@ -879,7 +890,10 @@ namespace qsv
             throw std::runtime_error("stage128: no blocks");
         const uint64_t ntiles = uint64_t(1) << (n - op.m);
         const uint64_t grid = ntiles < 8192 ? ntiles : 8192;
-        hipLaunchKernelGGL(k_stage128, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+        if (op.kq == 4)
+            hipLaunchKernelGGL(k_stage128<4>, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+        else
+            hipLaunchKernelGGL(k_stage128<3>, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
         check(hipGetLastError(), "stage128 kernel");
     }
 

@@ -66,12 +66,13 @@ namespace qsv
     int stage_max_blocks(int kq, int m);
     void launch_stage(hipStream_t s, c64 *state, int n, const Subspace &tsub, const StageOp &op);
 
-    // complex128 stage: tile of 2^12 amplitudes, blocks of 4 qubits in FP64; blocks[i].matrix
-    // is the offset in doubles of the 16x16 complex matrix (row-major, interleaved) from M;
-    // w[0..3] block bits, w[4..11] column bits.
+    // complex128 stage: tile of 2^12 amplitudes, blocks of kq = 3 or 4 qubits in FP64;
+    // blocks[i].matrix is the offset in doubles of the 2^kq x 2^kq complex matrix (row-major,
+    // interleaved) from M; w[0..kq-1] block bits, w[kq..11] column bits.
     struct Stage128Op
     {
         int m;
+        int kq;
         int nblocks;
         uint64_t gw[16];
         const StageBlock *blocks;

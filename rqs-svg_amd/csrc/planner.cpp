@@ -82,7 +82,7 @@ namespace qcore
         return free_q >= (f64 ? 6 : 5);
     }
 
-    std::vector<PlanOption> candidate_options(int kq, const std::vector<int> &tile_bits)
+    std::vector<PlanOption> candidate_options(int kq, const std::vector<int> &tile_bits, const std::vector<int> &stage_k)
     {
         std::vector<PlanOption> out;
         for (int k = std::max(2, kq - 2); k <= kq; ++k)
@@ -90,7 +90,8 @@ namespace qcore
                 for (int rule : {0, 1, 2})
                     out.push_back(PlanOption{0, k, w, rule});
         for (int m : tile_bits)
-            out.push_back(PlanOption{m, kq, 64, 0});
+            for (int k : stage_k)
+                out.push_back(PlanOption{m, k, 64, 0});
         return out;
     }
 

@@ -60,6 +60,6 @@ namespace qcore
     };
     std::vector<PlannedBlock> plan_with(const std::vector<ResolvedGate> &gates, const PlanOption &opt);
     // Block plans: k from max(2, kq - 2) to kq x windows {4, 64} x rules {0, 1, 2}; stage
-    // plans: each tile size in `tile_bits` (block size kq).
-    std::vector<PlanOption> candidate_options(int kq, const std::vector<int> &tile_bits);
+    // plans: each tile size in `tile_bits` x each block size in `stage_k`.
+    std::vector<PlanOption> candidate_options(int kq, const std::vector<int> &tile_bits, const std::vector<int> &stage_k);
 } // namespace qcore
