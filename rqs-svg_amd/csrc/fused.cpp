@@ -311,7 +311,7 @@ namespace qcore
         if (!model_)
         {
             if (m0 > 0)
-                apply_stages(gates, m0, kq_);
+                apply_stages(gates, m0, stage_k());
             else
                 apply_blocks(gates, plan_greedy(gates, k_));
             return;
@@ -327,7 +327,7 @@ namespace qcore
             if (n >= m + 4)
                 tiles.push_back(m);
         std::vector<PlanOption> opts;
-        opts.push_back(PlanOption{m0, kq_, 64, 0});
+        opts.push_back(PlanOption{m0, m0 > 0 ? stage_k() : k_, 64, 0});
         for (const auto &o : candidate_options(kq_, tiles, stage_block_sizes()))
             if (o.k <= k_ || o.stage_bits > 0)
                 opts.push_back(o);

@@ -153,6 +153,9 @@ namespace qcore
             return (stage_bits_ > 0 && sim_.n_ >= m + 4) ? m : 0;
         }
         bool uses_stages() const { return tile_bits() > 0; }
+        // block size of the default stage plan: the engine's block limit, within what the
+        // stage kernels take (complex128: 3 or 4, complex64: 4 or 5)
+        int stage_k() const { return f64_ ? std::max(3, std::min(k_, 4)) : kq_; }
         // block sizes the stage kernels take for this arithmetic (the planner's choices)
         std::vector<int> stage_block_sizes() const
         {

@@ -248,12 +248,13 @@ def test_stage_bits_validation(gpu):
         sim.set_stage_bits(12)
 
 
-def test_stages_complex128_match_oracle(gpu, oracle):
-    # precision mode 2 with stages: FP64 blocks of 4 qubits on LDS tiles of 2^12; FP64 bars
+@pytest.mark.parametrize("k", [3, 4])
+def test_stages_complex128_match_oracle(gpu, oracle, k):
+    # precision mode 2 with stages: FP64 blocks of k qubits on LDS tiles of 2^12; FP64 bars
     for n, seed in ((16, 21), (18, 22)):
         circ = circuits.random_all_gates(n, 250, seed=seed, max_ctrls=3) + circuits.qft(n)
         mapping = [int(q) for q in np.random.default_rng(seed).permutation(n)]
-        with pyqcs.Simulator(gpu, n, precision=2, block_qubits=4, mapping=mapping) as f:
+        with pyqcs.Simulator(gpu, n, precision=2, block_qubits=k, mapping=mapping) as f:
             f.run(circ)
             got = f.statevector()
             gates, launches = f.fusion_stats()
