@@ -811,12 +811,12 @@ namespace qsv
         template <int KQ, int M, bool X3>
         void launch_stage_t(hipStream_t st, c64 *state, uint64_t ntiles, const Subspace &tsub, const StageOp &op, uint64_t grid, size_t lds)
         {
-            static bool attr = false; // dynamic LDS beyond 64 KiB (gfx950 has 160 KiB per CU)
-            if (!attr)
-            {
+            // dynamic LDS beyond 64 KiB (gfx950 has 160 KiB per CU); once per kernel instance
+            static const bool attr = [] {
                 (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_stage<KQ, M, X3>), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-                attr = true;
-            }
+                return true;
+            }();
+            (void)attr;
             hipLaunchKernelGGL((k_stage<KQ, M, X3>), dim3(grid), dim3(kThreads), lds, st, state, ntiles, tsub, op);
         }
     } // namespace
