@@ -191,6 +191,26 @@ namespace qsv
                     }
             }
         }
+        // Read-only data of a launch (block descriptors, complex128 matrices) read through the
+        // constant address space: uniform loads from it become scalar loads, which neither
+        // occupy vector registers nor wait behind the vector loads of the tile prefetch.
+        template <class T>
+        __device__ __forceinline__ const __attribute__((address_space(4))) T *as_constant(const T *p)
+        {
+            return (const __attribute__((address_space(4))) T *)(p);
+        }
+
+        // Workgroup barrier that orders LDS accesses only.  __syncthreads() also waits for
+        // every outstanding global load and store of the wave, which would stall on the
+        // next tile's prefetch; the stage kernels exchange data between waves only through
+        // LDS, and their global accesses touch disjoint tiles.
+        __device__ __forceinline__ void lds_barrier()
+        {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        }
+
         // Orders this wave's LDS accesses against each other (lanes exchange data through LDS).
         __device__ __forceinline__ void wave_sync()
         {
@@ -583,7 +603,7 @@ namespace qsv
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
                     *reinterpret_cast<v4f *>(&tile[2 * threadIdx.x + i * 2 * kThreads]) = pf[i];
-                __syncthreads();
+                lds_barrier();
                 const uint64_t next = tl + gridDim.x;
                 if (next < ntiles)
                 {
@@ -594,7 +614,7 @@ namespace qsv
                 }
                 for (int bi = 0; bi < op.nblocks; ++bi)
                 {
-                    const StageBlock &B = op.blocks[bi];
+                    const __attribute__((address_space(4))) StageBlock &B = as_constant(op.blocks)[bi];
                     StageMatrix<KQ, X3> A;
                     A.load_image(images + bi * IMG, c, h);
 #pragma unroll
@@ -646,7 +666,7 @@ namespace qsv
                                 tile[cbase + o1] = v2f{d[mt][4 * q + 2], d[mt][4 * q + 3]};
                             }
                     }
-                    __syncthreads();
+                    lds_barrier();
                 }
                 const uint64_t base = expand(tl, tsub) + g_t;
 #pragma unroll
@@ -655,7 +675,7 @@ namespace qsv
                     const v4f v = *reinterpret_cast<const v4f *>(&tile[2 * threadIdx.x + i * 2 * kThreads]);
                     __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(s + (base + tile_offset(i))));
                 }
-                __syncthreads();
+                lds_barrier();
             }
         }
 
@@ -706,7 +726,7 @@ namespace qsv
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
                     tile[t + i * kThreads] = pf[i];
-                __syncthreads();
+                lds_barrier();
                 const uint64_t next = tl + gridDim.x;
                 if (next < ntiles)
                 {
@@ -717,8 +737,8 @@ namespace qsv
                 }
                 for (int bi = 0; bi < op.nblocks; ++bi)
                 {
-                    const StageBlock &B = op.blocks[bi];
-                    const double *__restrict__ Mx = op.M + B.matrix;
+                    const __attribute__((address_space(4))) StageBlock &B = as_constant(op.blocks)[bi];
+                    const __attribute__((address_space(4))) double *Mx = as_constant(op.M) + B.matrix;
 #pragma unroll
                     for (int cc = 0; cc < PER_THREAD; ++cc)
                     {
@@ -758,12 +778,12 @@ namespace qsv
                             tile[off[r]] = v2d{yr, yi};
                         }
                     }
-                    __syncthreads();
+                    lds_barrier();
                 }
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
                     __builtin_nontemporal_store(tile[t + i * kThreads], reinterpret_cast<v2d *>(s + (base + tile_offset(i))));
-                __syncthreads();
+                lds_barrier();
             }
         }
     } // namespace
