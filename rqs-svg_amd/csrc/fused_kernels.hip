@@ -191,6 +191,15 @@ namespace qsv
                     }
             }
         }
+        // LDS slot of tile index i in the stage kernels.  Blocks gather their columns at strides
+        // 2^b (b = the tile bits below the block's lowest qubit) and would hit one bank from
+        // all lanes; XOR-ing index bits 1..4 with bits 5..8 and 9..12 spreads such strides
+        // over the banks.  Bit 0 is kept, so amplitudes 2j and 2j + 1 stay one aligned 16-B
+        // pair for the tile's global loads and stores.
+        __device__ __forceinline__ int swz64(int i) { return i ^ ((((i >> 5) ^ (i >> 9)) & 15) << 1); }
+        // complex128 (16-B slots): XOR bits 0..3 with bits 4..7 and 8..11
+        __device__ __forceinline__ int swz128(int i) { return i ^ (((i >> 4) ^ (i >> 8)) & 15); }
+
         // Read-only data of a launch (block descriptors, complex128 matrices) read through the
         // constant address space: uniform loads from it become scalar loads, which neither
         // occupy vector registers nor wait behind the vector loads of the tile prefetch.
@@ -602,7 +611,7 @@ namespace qsv
             {
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
-                    *reinterpret_cast<v4f *>(&tile[2 * threadIdx.x + i * 2 * kThreads]) = pf[i];
+                    *reinterpret_cast<v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * kThreads)]) = pf[i];
                 lds_barrier();
                 const uint64_t next = tl + gridDim.x;
                 if (next < ntiles)
@@ -636,7 +645,7 @@ namespace qsv
                             for (int j = 0; j < KQ; ++j)
                                 if ((amp >> j) & 1)
                                     off += B.w[j];
-                            const v2f a = tile[cbase + off];
+                            const v2f a = tile[swz64(cbase + off)];
                             x[2 * e] = a.x;
                             x[2 * e + 1] = a.y;
                         }
@@ -662,8 +671,8 @@ namespace qsv
                                     if (((e0 + 1) >> j) & 1)
                                         o1 += B.w[j];
                                 }
-                                tile[cbase + o0] = v2f{d[mt][4 * q + 0], d[mt][4 * q + 1]};
-                                tile[cbase + o1] = v2f{d[mt][4 * q + 2], d[mt][4 * q + 3]};
+                                tile[swz64(cbase + o0)] = v2f{d[mt][4 * q + 0], d[mt][4 * q + 1]};
+                                tile[swz64(cbase + o1)] = v2f{d[mt][4 * q + 2], d[mt][4 * q + 3]};
                             }
                     }
                     lds_barrier();
@@ -672,7 +681,7 @@ namespace qsv
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
                 {
-                    const v4f v = *reinterpret_cast<const v4f *>(&tile[2 * threadIdx.x + i * 2 * kThreads]);
+                    const v4f v = *reinterpret_cast<const v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * kThreads)]);
                     __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(s + (base + tile_offset(i))));
                 }
                 lds_barrier();
@@ -725,7 +734,7 @@ namespace qsv
                 const uint64_t base = expand(tl, tsub) + g_t;
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
-                    tile[t + i * kThreads] = pf[i];
+                    tile[swz128(t + i * kThreads)] = pf[i];
                 lds_barrier();
                 const uint64_t next = tl + gridDim.x;
                 if (next < ntiles)
@@ -761,7 +770,7 @@ namespace qsv
                         v2d x[K];
 #pragma unroll
                         for (int k = 0; k < K; ++k)
-                            x[k] = tile[off[k]];
+                            x[k] = tile[swz128(off[k])];
 #pragma unroll
                         for (int r = 0; r < K; ++r)
                         {
@@ -775,14 +784,14 @@ namespace qsv
                                 yi = __builtin_fma(mr, x[k].y, yi);
                                 yi = __builtin_fma(mi, x[k].x, yi);
                             }
-                            tile[off[r]] = v2d{yr, yi};
+                            tile[swz128(off[r])] = v2d{yr, yi};
                         }
                     }
                     lds_barrier();
                 }
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
-                    __builtin_nontemporal_store(tile[t + i * kThreads], reinterpret_cast<v2d *>(s + (base + tile_offset(i))));
+                    __builtin_nontemporal_store(tile[swz128(t + i * kThreads)], reinterpret_cast<v2d *>(s + (base + tile_offset(i))));
                 lds_barrier();
             }
         }
