@@ -137,6 +137,48 @@ namespace qsv
             }
         }
 
+        // 2x2 on a low target t < 6 with every control at qubit 6 or above: a wave takes
+        // 64 consecutive amplitudes (one per lane, fully coalesced) and each lane gets its
+        // partner amplitude (lane ^ 2^t) through a cross-lane shuffle, instead of loading
+        // pairs 2^t apart.  `csub` enumerates the 64-amplitude chunks: the controls' positions
+        // minus 6 and their set bits shifted down by 6.
+        template <int PER>
+        __global__ __launch_bounds__(kBlock) void k_pair_low(c128 *__restrict__ s, uint64_t nchunks, Subspace csub, int t, PairOp op)
+        {
+            constexpr int kWavesPerBlock = kBlock / 64;
+            const int lane = threadIdx.x & 63;
+            const int wave = threadIdx.x >> 6;
+            const bool hi = (lane >> t) & 1;
+            // this lane's amplitude reads as zero on a measured target's other branch
+            const bool zero = hi ? (op.zero_mask & 2) : (op.zero_mask & 1);
+            const uint64_t per_tile = uint64_t(kWavesPerBlock) * PER;
+            const uint64_t ntiles = (nchunks + per_tile - 1) / per_tile;
+            for (uint64_t tile = xcd_block(); tile < ntiles; tile += gridDim.x)
+            {
+                uint64_t idx[PER];
+                v2d a[PER];
+                bool ok[PER];
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                {
+                    const uint64_t chunk = tile * per_tile + uint64_t(j) * kWavesPerBlock + wave;
+                    ok[j] = chunk < nchunks; // uniform per wave
+                    idx[j] = ok[j] ? (expand(chunk, csub) << 6) + lane : 0;
+                    a[j] = (ok[j] && !zero) ? ld(s + idx[j]) : v2d{0.0, 0.0};
+                }
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                {
+                    if (!ok[j])
+                        continue;
+                    const v2d p{__shfl_xor(a[j].x, 1 << t), __shfl_xor(a[j].y, 1 << t)};
+                    // bit t clear: a is the |0> amplitude, p the |1> one; set: the reverse
+                    const v2d out = hi ? cfma(op.m[3], a[j], cmul(op.m[2], p)) : cfma(op.m[1], p, cmul(op.m[0], a[j]));
+                    st(s + idx[j], out);
+                }
+            }
+        }
+
         template <int PER>
         __global__ __launch_bounds__(kBlock) void k_phase(c128 *__restrict__ s, uint64_t nitems, Subspace sub, c128 phase)
         {
@@ -644,6 +686,32 @@ namespace qsv
     {
         uint64_t items = items_of(n, sub, "pair");
         check_offset(n, op.offA | op.offB, "pair");
+        // one target t < 6 (the pattern |0>, |1> on bit t) and all controls at 6 or above
+        int t = -1;
+        if (op.offA == 0 && op.offB != 0 && (op.offB & (op.offB - 1)) == 0 && op.offB < 64 && n >= 12)
+        {
+            t = __builtin_ctzll(op.offB);
+            Subspace csub{};
+            csub.set_bits = sub.set_bits >> 6;
+            bool fits = (sub.set_bits & 63) == 0;
+            bool has_t = false;
+            for (int k = 0; k < sub.nops && fits; ++k)
+            {
+                if (sub.ops[k] == t)
+                    has_t = true;
+                else if (sub.ops[k] < 6)
+                    fits = false;
+                else
+                    csub.ops[csub.nops++] = sub.ops[k] - 6;
+            }
+            if (fits && has_t && n - 6 - csub.nops >= 0)
+            {
+                const uint64_t nchunks = uint64_t(1) << (n - 6 - csub.nops);
+                hipLaunchKernelGGL(k_pair_low<2>, dim3(grid_for(nchunks, (kBlock / 64) * 2)), dim3(kBlock), 0, s, state, nchunks, csub, t, op);
+                after_launch("pair kernel (low target)");
+                return;
+            }
+        }
         if (items >= (uint64_t(1) << 16))
         {
             hipLaunchKernelGGL(k_pair<2>, dim3(grid_for(items, kBlock * 2)), dim3(kBlock), 0, s, state, items, sub, op);

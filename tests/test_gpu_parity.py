@@ -62,6 +62,37 @@ def test_each_gate(gpu, oracle, name):
         assert np.max(np.abs(g - want)) <= AMP_TOL
 
 
+ONE_TARGET_DENSE = ["h", "x", "y", "sx", "sxdg", "rx", "ry", "r", "u", "u2", "u3", "u4"]
+
+
+def test_low_target_pair_kernel(gpu, oracle):
+    # targets 0..5 with controls at 6 and above (positive and negative) run in the
+    # cross-lane kernel; a measured target zeroes one branch first
+    n = 14
+    rng = np.random.default_rng(77)
+    psi = random_state(n, 9)
+    circ = []
+    for i in range(120):
+        name = ONE_TARGET_DENSE[i % len(ONE_TARGET_DENSE)]
+        t = int(rng.integers(0, 6))
+        ctl = [int(q) for q in rng.choice(np.arange(6, n), size=int(rng.integers(0, 3)), replace=False)]
+        pos = [c for c in ctl if rng.random() < 0.5]
+        neg = [c for c in ctl if c not in pos]
+        params = tuple(rng.uniform(-math.pi, math.pi, size=pyqcs.GATE_PARAMS[name]))
+        circ.append((name, (t,), params, tuple(pos), tuple(neg)))
+    res = []
+    for lib in (gpu, oracle):
+        with pyqcs.Simulator(lib, n, num_clbits=2, seed=5) as sim:
+            sim.set_statevector(psi)
+            sim.run(circ[:60])
+            bits = [sim.measure_to_clbit(2, 0), sim.measure_to_clbit(9, 1)]
+            sim.run(circ[60:])
+            res.append((sim.statevector(), bits))
+    assert res[0][1] == res[1][1]
+    assert np.max(np.abs(res[0][0] - res[1][0])) <= AMP_TOL
+    assert fidelity(res[0][0], res[1][0]) >= 1 - FID_TOL
+
+
 def test_random_circuit_all_gates(gpu, oracle):
     n = 12
     circ = circuits.random_all_gates(n, 400, seed=17, max_ctrls=3)
