@@ -857,34 +857,6 @@ namespace qcore
         return out;
     }
 
-    void stage_image(const std::vector<cd> &M, int kq, bool x3, unsigned char *out)
-    {
-        const int K = 1 << kq, ROW = K + 4;
-        std::memset(out, 0, qsv::stage_image_bytes(kq));
-        if (!x3)
-        {
-            float *f = reinterpret_cast<float *>(out);
-            for (int r = 0; r < K; ++r)
-                for (int c = 0; c < K; ++c)
-                {
-                    f[2 * (r * ROW + c)] = float(M[size_t(r) * K + c].real());
-                    f[2 * (r * ROW + c) + 1] = float(M[size_t(r) * K + c].imag());
-                }
-            return;
-        }
-        uint32_t *hi = reinterpret_cast<uint32_t *>(out);
-        uint32_t *lo = hi + K * ROW;
-        for (int r = 0; r < K; ++r)
-            for (int c = 0; c < K; ++c)
-            {
-                const double v[2] = {M[size_t(r) * K + c].real(), M[size_t(r) * K + c].imag()};
-                uint16_t h[2], l[2];
-                split_bf16(v, 2, h, l);
-                hi[r * ROW + c] = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
-                lo[r * ROW + c] = uint32_t(l[0]) | (uint32_t(l[1]) << 16);
-            }
-    }
-
     PreparedStage prepare_stage(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int kq, int m, int n, std::vector<float> &mats, bool x3)
     {
         if ((int)st.tile.size() != m)
@@ -894,17 +866,12 @@ namespace qcore
         for (int gi : st.gates)
             sub.push_back(gates[gi]);
         const auto plan = plan_greedy(sub, kq);
-        const size_t img = size_t(qsv::stage_image_bytes(kq));
-        static_assert(sizeof(qsv::StageBlock) % 16 == 0, "StageBlock keeps 16-B alignment");
-        while (mats.size() % 4)
-            mats.push_back(0.f);
-        PreparedStage p;
-        p.image_offset = mats.size();
-        mats.resize(mats.size() + plan.size() * img / sizeof(float));
+        const int D = 2 << kq;
+        const size_t per = size_t(D) * D;
+        const int K = 1 << kq;
         std::vector<qsv::StageBlock> desc;
-        for (size_t bi = 0; bi < plan.size(); ++bi)
+        for (const auto &pb : plan)
         {
-            const auto &pb = plan[bi];
             if (pb.wide)
                 throw std::runtime_error("stage: wide gate inside a stage");
             BlockLayout lay;
@@ -913,8 +880,31 @@ namespace qcore
             for (int i = m - 1; i >= 0 && (int)lay.qubits.size() < kq; --i)
                 if (local_pos(lay.qubits, st.tile[i]) < 0)
                     lay.qubits.push_back(st.tile[i]);
-            stage_image(block_matrix(sub, pb, lay), kq, x3, reinterpret_cast<unsigned char *>(mats.data() + p.image_offset) + bi * img);
+            const auto M = block_matrix(sub, pb, lay);
+            std::vector<double> R(per);
+            for (int r = 0; r < K; ++r)
+                for (int c = 0; c < K; ++c)
+                {
+                    const cd z = M[size_t(r) * K + c];
+                    R[(2 * r) * D + 2 * c] = z.real();
+                    R[(2 * r) * D + 2 * c + 1] = -z.imag();
+                    R[(2 * r + 1) * D + 2 * c] = z.imag();
+                    R[(2 * r + 1) * D + 2 * c + 1] = z.real();
+                }
             qsv::StageBlock sb{};
+            sb.matrix = uint32_t(mats.size());
+            mats.resize(mats.size() + per);
+            if (x3)
+            {
+                std::vector<uint16_t> parts(2 * per);
+                split_bf16(R.data(), per, parts.data(), parts.data() + per);
+                std::memcpy(mats.data() + sb.matrix, parts.data(), 2 * per * sizeof(uint16_t));
+            }
+            else
+            {
+                for (size_t i = 0; i < per; ++i)
+                    mats[sb.matrix + i] = float(R[i]);
+            }
             std::vector<char> is_block(m, 0);
             for (int j = 0; j < kq; ++j)
             {
@@ -930,9 +920,13 @@ namespace qcore
                     sb.w[j++] = 1 << b;
             desc.push_back(sb);
         }
+        static_assert(sizeof(qsv::StageBlock) % 16 == 0, "StageBlock keeps 16-B alignment");
+        PreparedStage p;
         p.gates = (int)st.gates.size();
         p.nblocks = (int)desc.size();
         p.bytes = 2.0 * 8.0 * double(uint64_t(1) << n);
+        while (mats.size() % 4)
+            mats.push_back(0.f);
         p.desc_offset = mats.size();
         mats.resize(mats.size() + desc.size() * sizeof(qsv::StageBlock) / sizeof(float));
         std::memcpy(mats.data() + p.desc_offset, desc.data(), desc.size() * sizeof(qsv::StageBlock));
@@ -950,27 +944,11 @@ namespace qcore
         return p;
     }
 
-    int stage_launches(int nblocks, int kq, int m)
-    {
-        const int cap = qsv::stage_max_blocks(kq, m);
-        return (nblocks + cap - 1) / cap;
-    }
-
     void launch_prepared_stage(hipStream_t s, qsv::c64 *state, int n, PreparedStage &p, const float *mats_dev)
     {
-        // launches of at most stage_max_blocks blocks each, one sweep apiece
-        const int cap = qsv::stage_max_blocks(p.op.kq, p.op.m);
-        const size_t img = size_t(qsv::stage_image_bytes(p.op.kq));
-        const auto *desc = reinterpret_cast<const qsv::StageBlock *>(mats_dev + p.desc_offset);
-        const auto *images = reinterpret_cast<const unsigned char *>(mats_dev + p.image_offset);
-        for (int first = 0; first < p.nblocks; first += cap)
-        {
-            qsv::StageOp op = p.op;
-            op.nblocks = std::min(cap, p.nblocks - first);
-            op.blocks = desc + first;
-            op.images = images + size_t(first) * img;
-            qsv::launch_stage(s, state, n, p.tsub, op);
-        }
+        p.op.R = mats_dev;
+        p.op.blocks = reinterpret_cast<const qsv::StageBlock *>(mats_dev + p.desc_offset);
+        qsv::launch_stage(s, state, n, p.tsub, p.op);
     }
 
     PreparedStage128 prepare_stage128(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int n, std::vector<double> &mats, int kq)
@@ -1108,16 +1086,13 @@ namespace qcore
         {
             if (it.stage)
             {
-                // a complex64 stage with more blocks than one launch's LDS holds takes
-                // several launches, each a sweep
-                const int launches = f64_ ? 1 : stage_launches(it.st.nblocks, it.st.op.kq, it.st.op.m);
-                sim_.prof_begin(QCS_EXT_K_FUSED, launches * (f64_ ? it.st128.bytes : it.st.bytes));
+                sim_.prof_begin(QCS_EXT_K_FUSED, f64_ ? it.st128.bytes : it.st.bytes);
                 if (f64_)
                     launch_prepared_stage128(sim_.stream_, state128_, n, it.st128, static_cast<const double *>(rdev_));
                 else
                     launch_prepared_stage(sim_.stream_, state_, n, it.st, static_cast<const float *>(rdev_));
                 sim_.prof_end();
-                blocks_applied_ += launches;
+                blocks_applied_ += 1;
                 gates_applied_ += f64_ ? it.st128.gates : it.st.gates;
                 continue;
             }

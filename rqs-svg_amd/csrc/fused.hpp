@@ -91,20 +91,15 @@ namespace qcore
     {
         int gates = 0;
         int nblocks = 0;
-        double bytes = 0;        // per launch (one sweep)
-        size_t image_offset = 0; // floats: the blocks' LDS images
-        size_t desc_offset = 0;  // floats: StageBlock array
+        double bytes = 0;       // one sweep
+        size_t desc_offset = 0; // floats: StageBlock array
         qsv::Subspace tsub{};
         qsv::StageOp op{};
     };
     // Plans the stage's gates into blocks of at most kq qubits inside the tile and appends
-    // their LDS images (FP32, or bfloat16 halves when x3) and descriptors to `mats`.
+    // their matrices (FP32, or bfloat16 halves when x3) and descriptors to `mats`.
     PreparedStage prepare_stage(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int kq, int m, int n, std::vector<float> &mats, bool x3);
-    // One launch per stage_max_blocks blocks (each a sweep of the state).
     void launch_prepared_stage(hipStream_t s, qsv::c64 *state, int n, PreparedStage &p, const float *mats_dev);
-    int stage_launches(int nblocks, int kq, int m);
-    // The LDS image of one block matrix for the complex64 stage kernel (StageOp).
-    void stage_image(const std::vector<cd> &M, int kq, bool x3, unsigned char *out);
 
     // complex128 stages (tile of 2^12, FP64 blocks of kq = 3 or 4 qubits): matrices and
     // descriptors appended to `mats` as doubles.

@@ -468,90 +468,20 @@ namespace qsv
             }
         }
 
-        // A operands of one block read from its LDS image (the complex matrix, row-major,
-        // rows padded to K + 4 entries; stage_image_bytes).  Real row r = 2i + eps of the
-        // real-ified matrix is (Re, -Im) of complex row i for eps = 0 and (Im, Re) for eps = 1,
-        // so a lane builds its rows from the complex entries.
-        template <int KQ, bool X3>
-        struct StageMatrix;
-
-        template <int KQ>
-        struct StageMatrix<KQ, false> : BlockMatrix<KQ, false>
-        {
-            static constexpr int K = 1 << KQ, ROW = K + 4, MT = (2 << KQ) / 32, H = (2 << KQ) / 2;
-            __device__ __forceinline__ void load_image(const unsigned char *img, int c, int h)
-            {
-                const v2f *m = reinterpret_cast<const v2f *>(img);
-                const bool odd = c & 1;
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                {
-                    const int i = 16 * mt + (c >> 1);
-#pragma unroll
-                    for (int q = 0; q < H / 4; ++q)
-                    {
-                        // complex columns h*H/2 + 2q, + 2q + 1
-                        const v4f z = *reinterpret_cast<const v4f *>(&m[i * ROW + h * (H / 2) + 2 * q]);
-                        this->a[mt][4 * q + 0] = odd ? z.y : z.x;
-                        this->a[mt][4 * q + 1] = odd ? z.x : -z.y;
-                        this->a[mt][4 * q + 2] = odd ? z.w : z.z;
-                        this->a[mt][4 * q + 3] = odd ? z.z : -z.w;
-                    }
-                }
-            }
-        };
-
-        template <int KQ>
-        struct StageMatrix<KQ, true> : BlockMatrix<KQ, true>
-        {
-            static constexpr int K = 1 << KQ, ROW = K + 4, MT = (2 << KQ) / 32, H = (2 << KQ) / 2, ST = H / 8;
-            typedef unsigned v4u __attribute__((ext_vector_type(4)));
-            // (re, im) bfloat16 pair in one dword: eps = 0 -> (re, -im), eps = 1 -> (im, re)
-            static __device__ __forceinline__ v4u rows(v4u d, bool odd)
-            {
-                v4u r;
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    r[e] = odd ? ((d[e] >> 16) | (d[e] << 16)) : (d[e] ^ 0x80000000u);
-                return r;
-            }
-            __device__ __forceinline__ void load_image(const unsigned char *img, int c, int h)
-            {
-                const unsigned *hi = reinterpret_cast<const unsigned *>(img);
-                const unsigned *lo = hi + K * ROW;
-                const bool odd = c & 1;
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                {
-                    const int i = 16 * mt + (c >> 1);
-#pragma unroll
-                    for (int st = 0; st < ST; ++st)
-                    {
-                        const int j0 = h * (H / 2) + 4 * st;
-                        const v4u a = rows(*reinterpret_cast<const v4u *>(hi + i * ROW + j0), odd);
-                        const v4u b = rows(*reinterpret_cast<const v4u *>(lo + i * ROW + j0), odd);
-                        this->hi[mt][st] = __builtin_bit_cast(bf16x8, a);
-                        this->lo[mt][st] = __builtin_bit_cast(bf16x8, b);
-                    }
-                }
-            }
-        };
-
         // Stage kernel: one workgroup holds a tile of 2^M amplitudes in LDS -- the
         // amplitudes whose index bits outside the stage's tile qubits T are fixed -- and
-        // applies every block of the launch to it before writing it back, so a launch costs
+        // applies every block of the stage to it before writing it back, so a stage costs
         // one HBM sweep however many blocks it holds.  T[0..3] are qubits 0..3, so each
         // thread moves 2 adjacent amplitudes (16 B) per load/store instruction and 8 lanes
         // cover one 128-B line.  LDS index bit b <-> tile qubit T[b].
         //
-        // The launch's block matrices are copied into LDS once per workgroup (dynamic LDS
-        // after the tile); the workgroup then walks its tiles, and while the blocks of one
-        // tile run, the next tile's loads are already in flight in registers.  Per block,
-        // the 2^(M-KQ) columns (tile bits outside the block) are split into groups of 32 over
-        // the 4 waves; lane (h, c) gathers column c's half h from LDS (LDS weights of the
-        // block's bits and of the column bits come with the block), multiplies on MFMA as the
-        // block kernels do, and writes its rows back in place.  Workgroup barriers separate
-        // the blocks.
+        // Per block, the 2^(M-KQ) columns (tile bits outside the block) are split into
+        // groups of 32 over the 4 waves; lane (h, c) of a wave gathers column c's half h
+        // from LDS (LDS weights of the block's bits and of the column bits come with the
+        // block), multiplies on MFMA as the block kernels do, and writes its rows back in
+        // place.  Workgroup barriers (LDS-only) separate the blocks.  The block matrices come
+        // from global memory (L2) into registers per block; block descriptors through scalar
+        // loads.
         template <int KQ, int M, bool X3>
         __global__ __launch_bounds__(kThreads) void k_stage(c64 *__restrict__ s, uint64_t ntiles, Subspace tsub, StageOp op)
         {
@@ -563,25 +493,13 @@ namespace qsv
             constexpr int COLS = NT >> KQ;
             constexpr int PER_WAVE = COLS / 32 / kWaves;
             constexpr int LOADS = NT / (2 * kThreads);
-            constexpr int IMG = (1 << KQ) * ((1 << KQ) + 4) * 8;
             static_assert(PER_WAVE >= 1 && LOADS >= 1, "tile too small");
-            extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-            v2f *tile = reinterpret_cast<v2f *>(smem);
-            unsigned char *images = smem + NT * sizeof(v2f);
+            __shared__ __attribute__((aligned(16))) v2f tile[NT];
 
             const int lane = threadIdx.x & 63;
             const int wave = threadIdx.x >> 6;
             const int c = lane & 31;
             const int h = lane >> 5;
-
-            // the launch's block matrices into LDS (16 B per thread and step)
-            {
-                const v4f *src = reinterpret_cast<const v4f *>(op.images);
-                v4f *dst = reinterpret_cast<v4f *>(images);
-                const int nvec = op.nblocks * (IMG / 16);
-                for (int i = threadIdx.x; i < nvec; i += kThreads)
-                    dst[i] = src[i];
-            }
 
             // global offset of this thread's first amplitude (LDS index 2 * threadIdx.x)
             uint64_t g_t = 0;
@@ -589,43 +507,27 @@ namespace qsv
             for (int b = 1; b < 9 && b < M; ++b)
                 if (((2 * threadIdx.x) >> b) & 1)
                     g_t += op.gw[b];
-            auto tile_offset = [&](int i) {
-                uint64_t gi = 0;
-#pragma unroll
-                for (int b = 9; b < M; ++b)
-                    if (((i * 2 * kThreads) >> b) & 1)
-                        gi += op.gw[b];
-                return gi;
-            };
 
-            v4f pf[LOADS];
-            uint64_t tl = blockIdx.x;
-            if (tl < ntiles)
+            for (uint64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x)
             {
                 const uint64_t base = expand(tl, tsub) + g_t;
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
-                    pf[i] = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (base + tile_offset(i))));
-            }
-            for (; tl < ntiles; tl += gridDim.x)
-            {
-#pragma unroll
-                for (int i = 0; i < LOADS; ++i)
-                    *reinterpret_cast<v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * kThreads)]) = pf[i];
-                lds_barrier();
-                const uint64_t next = tl + gridDim.x;
-                if (next < ntiles)
                 {
-                    const uint64_t nbase = expand(next, tsub) + g_t;
+                    uint64_t gi = 0;
 #pragma unroll
-                    for (int i = 0; i < LOADS; ++i)
-                        pf[i] = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (nbase + tile_offset(i))));
+                    for (int b = 9; b < M; ++b)
+                        if (((i * 2 * kThreads) >> b) & 1)
+                            gi += op.gw[b];
+                    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (base + gi)));
+                    *reinterpret_cast<v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * kThreads)]) = v;
                 }
+                lds_barrier();
                 for (int bi = 0; bi < op.nblocks; ++bi)
                 {
                     const __attribute__((address_space(4))) StageBlock &B = as_constant(op.blocks)[bi];
-                    StageMatrix<KQ, X3> A;
-                    A.load_image(images + bi * IMG, c, h);
+                    BlockMatrix<KQ, X3> A;
+                    A.load(op.R + B.matrix, c, h);
 #pragma unroll
                     for (int g = 0; g < PER_WAVE; ++g)
                     {
@@ -677,12 +579,16 @@ namespace qsv
                     }
                     lds_barrier();
                 }
-                const uint64_t base = expand(tl, tsub) + g_t;
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
                 {
+                    uint64_t gi = 0;
+#pragma unroll
+                    for (int b = 9; b < M; ++b)
+                        if (((i * 2 * kThreads) >> b) & 1)
+                            gi += op.gw[b];
                     const v4f v = *reinterpret_cast<const v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * kThreads)]);
-                    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(s + (base + tile_offset(i))));
+                    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(s + (base + gi)));
                 }
                 lds_barrier();
             }
@@ -835,30 +741,6 @@ namespace qsv
     }
 
 
-    namespace
-    {
-        template <int KQ, int M, bool X3>
-        void launch_stage_t(hipStream_t st, c64 *state, uint64_t ntiles, const Subspace &tsub, const StageOp &op, uint64_t grid, size_t lds)
-        {
-            // dynamic LDS beyond 64 KiB (gfx950 has 160 KiB per CU); once per kernel instance
-            static const bool attr = [] {
-                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_stage<KQ, M, X3>), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-                return true;
-            }();
-            (void)attr;
-            hipLaunchKernelGGL((k_stage<KQ, M, X3>), dim3(grid), dim3(kThreads), lds, st, state, ntiles, tsub, op);
-        }
-    } // namespace
-
-    int stage_image_bytes(int kq) { return (1 << kq) * ((1 << kq) + 4) * 8; }
-
-    int stage_max_blocks(int kq, int m)
-    {
-        // 2^12 tiles leave room for two workgroups per CU (80 KiB each), 2^13 tiles for one
-        const int budget = (m == 12 ? 80 : 160) * 1024 - (8 << m);
-        return std::max(1, budget / stage_image_bytes(kq));
-    }
-
     void launch_stage(hipStream_t st, c64 *state, int n, const Subspace &tsub, const StageOp &op)
     {
         if (op.kq != 4 && op.kq != 5)
@@ -878,23 +760,22 @@ namespace qsv
                 throw std::runtime_error("stage: tile weights do not match the tile qubits");
         if (tsub.set_bits != 0)
             throw std::runtime_error("stage: tiles take no controls");
-        if (op.nblocks < 1 || op.nblocks > stage_max_blocks(op.kq, op.m) || op.blocks == nullptr || op.images == nullptr)
-            throw std::runtime_error("stage: block count out of range for one launch");
-        if (reinterpret_cast<uintptr_t>(op.images) % 16 != 0)
-            throw std::runtime_error("stage: matrix images must be 16-byte aligned");
+        if (op.nblocks < 1 || op.blocks == nullptr || op.R == nullptr)
+            throw std::runtime_error("stage: no blocks");
         const uint64_t ntiles = uint64_t(1) << (n - op.m);
-        const uint64_t grid = ntiles < 8192 ? ntiles : 8192;
-        const size_t lds = (size_t(8) << op.m) + size_t(op.nblocks) * stage_image_bytes(op.kq);
+        uint64_t grid = ntiles < 8192 ? ntiles : 8192;
+        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op); };
         if (op.kq == 5 && op.m == 13)
-            op.x3 ? launch_stage_t<5, 13, true>(st, state, ntiles, tsub, op, grid, lds) : launch_stage_t<5, 13, false>(st, state, ntiles, tsub, op, grid, lds);
+            op.x3 ? go(k_stage<5, 13, true>) : go(k_stage<5, 13, false>);
         else if (op.kq == 5)
-            op.x3 ? launch_stage_t<5, 12, true>(st, state, ntiles, tsub, op, grid, lds) : launch_stage_t<5, 12, false>(st, state, ntiles, tsub, op, grid, lds);
+            op.x3 ? go(k_stage<5, 12, true>) : go(k_stage<5, 12, false>);
         else if (op.m == 13)
-            op.x3 ? launch_stage_t<4, 13, true>(st, state, ntiles, tsub, op, grid, lds) : launch_stage_t<4, 13, false>(st, state, ntiles, tsub, op, grid, lds);
+            op.x3 ? go(k_stage<4, 13, true>) : go(k_stage<4, 13, false>);
         else
-            op.x3 ? launch_stage_t<4, 12, true>(st, state, ntiles, tsub, op, grid, lds) : launch_stage_t<4, 12, false>(st, state, ntiles, tsub, op, grid, lds);
+            op.x3 ? go(k_stage<4, 12, true>) : go(k_stage<4, 12, false>);
         check(hipGetLastError(), "stage kernel");
     }
+
 
     void launch_stage128(hipStream_t st, c128 *state, int n, const Subspace &tsub, const Stage128Op &op)
     {

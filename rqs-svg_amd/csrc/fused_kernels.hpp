@@ -39,17 +39,14 @@ namespace qsv
 
     // Stage: several blocks applied to LDS-resident tiles in one sweep.  The tile qubits T
     // (tsub.ops, ascending, m of them, T[0..3] = 0..3) are the stage's; gw[b] = 1 << T[b].
-    // w[j] (j < kq) is the LDS index weight (1 << tile bit position) of a block's j-th
-    // qubit and w[kq + j] that of its j-th column bit (the other tile bits, ascending);
-    // `matrix` is unused by the complex64 stage kernel.  images points to the blocks' LDS
-    // images, consecutive, stage_image_bytes(kq) each: the 2^kq x 2^kq complex matrix
-    // row-major with rows padded to 2^kq + 4 entries, an entry being (re, im) as two floats
-    // (x3 = 0) or, for x3 = 1, all bfloat16 high parts as (re, im) pairs in one dword
-    // followed by all low parts.  Device memory for blocks and images.
+    // Block i's matrix is at R + blocks[i].matrix (layout of BlockOp for x3); w[j] (j < kq)
+    // is the LDS index weight (1 << tile bit position) of the block's j-th qubit and
+    // w[kq + j] that of the j-th column bit (the other tile bits, ascending).  blocks
+    // points to device memory.
     struct StageBlock
     {
         int w[16];
-        uint32_t matrix; // complex128 stages: offset in doubles of the 16x16 matrix from M
+        uint32_t matrix; // offset in floats from StageOp::R (complex128 stages: doubles from M)
         uint32_t pad[3];
     };
     struct StageOp
@@ -57,13 +54,11 @@ namespace qsv
         int kq;
         int m;
         int x3;
-        int nblocks; // at most stage_max_blocks(kq, m) per launch
+        int nblocks;
         uint64_t gw[16];
         const StageBlock *blocks;
-        const void *images;
+        const float *R;
     };
-    int stage_image_bytes(int kq);
-    int stage_max_blocks(int kq, int m);
     void launch_stage(hipStream_t s, c64 *state, int n, const Subspace &tsub, const StageOp &op);
 
     // complex128 stage: tile of 2^12 amplitudes, blocks of kq = 3 or 4 qubits in FP64;
