@@ -204,6 +204,40 @@ namespace qsv
             }
         }
 
+        // Phase on a subspace with operand bits below 3 (amplitudes of interest 16-64 B
+        // apart): a wave takes 64 consecutive amplitudes, and the lanes whose low 6 index bits
+        // match the subspace's low pattern load, multiply and store; `csub` enumerates the
+        // 64-amplitude chunks through the operands at 6 and above.  Each wave instruction
+        // then covers 8 whole lines instead of spreading over 32-64.
+        template <int PER>
+        __global__ __launch_bounds__(kBlock) void k_phase_low(c128 *__restrict__ s, uint64_t nchunks, Subspace csub, unsigned low_mask, unsigned low_set, c128 phase)
+        {
+            constexpr int kWavesPerBlock = kBlock / 64;
+            const int lane = threadIdx.x & 63;
+            const int wave = threadIdx.x >> 6;
+            const bool mine = (unsigned(lane) & low_mask) == low_set;
+            const uint64_t per_tile = uint64_t(kWavesPerBlock) * PER;
+            const uint64_t ntiles = (nchunks + per_tile - 1) / per_tile;
+            for (uint64_t tile = xcd_block(); tile < ntiles; tile += gridDim.x)
+            {
+                uint64_t idx[PER];
+                v2d a[PER];
+                bool ok[PER];
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                {
+                    const uint64_t chunk = tile * per_tile + uint64_t(j) * kWavesPerBlock + wave;
+                    ok[j] = chunk < nchunks && mine;
+                    idx[j] = ok[j] ? (expand(chunk, csub) << 6) + lane : 0;
+                    a[j] = ok[j] ? ld(s + idx[j]) : v2d{0.0, 0.0};
+                }
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                    if (ok[j])
+                        st(s + idx[j], cmul(phase, a[j]));
+            }
+        }
+
         template <int PER>
         __global__ __launch_bounds__(kBlock) void k_diag(c128 *__restrict__ s, uint64_t nitems, Subspace sub, DiagOp op)
         {
@@ -726,6 +760,28 @@ namespace qsv
     void launch_phase(hipStream_t s, c128 *state, int n, const Subspace &sub, c128 phase)
     {
         uint64_t items = items_of(n, sub, "phase");
+        // an operand below qubit 3: chunks of 64 amplitudes with a per-lane low-bit pattern
+        if (n >= 12 && sub.nops > 0 && sub.ops[0] < 3)
+        {
+            Subspace csub{};
+            csub.set_bits = sub.set_bits >> 6;
+            unsigned low_mask = 0;
+            for (int k = 0; k < sub.nops; ++k)
+            {
+                if (sub.ops[k] < 6)
+                    low_mask |= 1u << sub.ops[k];
+                else
+                    csub.ops[csub.nops++] = sub.ops[k] - 6;
+            }
+            const unsigned low_set = unsigned(sub.set_bits & 63);
+            if (n - 6 - csub.nops >= 0 && (low_set & ~low_mask) == 0)
+            {
+                const uint64_t nchunks = uint64_t(1) << (n - 6 - csub.nops);
+                hipLaunchKernelGGL(k_phase_low<2>, dim3(grid_for(nchunks, (kBlock / 64) * 2)), dim3(kBlock), 0, s, state, nchunks, csub, low_mask, low_set, phase);
+                after_launch("phase kernel (low operands)");
+                return;
+            }
+        }
         if (items >= (uint64_t(1) << 16))
             hipLaunchKernelGGL(k_phase<2>, dim3(grid_for(items, kBlock * 2)), dim3(kBlock), 0, s, state, items, sub, phase);
         else

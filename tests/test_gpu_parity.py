@@ -93,6 +93,30 @@ def test_low_target_pair_kernel(gpu, oracle):
     assert fidelity(res[0][0], res[1][0]) >= 1 - FID_TOL
 
 
+def test_low_operand_phase_kernel(gpu, oracle):
+    # phase gates whose operands include qubits 0..2 run in the 64-amplitude chunk kernel;
+    # controls positive and negative, low and high
+    n = 14
+    rng = np.random.default_rng(78)
+    psi = random_state(n, 10)
+    circ = []
+    for i in range(120):
+        name = ["z", "s", "sdg", "t", "tdg", "p", "u1"][i % 7]
+        t = int(rng.integers(0, 3))
+        others = [q for q in range(n) if q != t]
+        ctl = [int(q) for q in rng.choice(others, size=int(rng.integers(0, 3)), replace=False)]
+        pos = [c for c in ctl if rng.random() < 0.5]
+        neg = [c for c in ctl if c not in pos]
+        params = tuple(rng.uniform(-math.pi, math.pi, size=pyqcs.GATE_PARAMS[name]))
+        circ.append((name, (t,), params, tuple(pos), tuple(neg)))
+    for i in range(20):  # CZ across low and high qubits, both orders
+        a, b = int(rng.integers(0, 3)), int(rng.integers(3, n))
+        circ.append(("z", (a,), (), (b,), ()) if i % 2 else ("z", (b,), (), (a,), ()))
+    g, o = run_both(gpu, oracle, n, circ, psi=psi)
+    assert np.max(np.abs(g - o)) <= AMP_TOL
+    assert fidelity(g, o) >= 1 - FID_TOL
+
+
 def test_random_circuit_all_gates(gpu, oracle):
     n = 12
     circ = circuits.random_all_gates(n, 400, seed=17, max_ctrls=3)
