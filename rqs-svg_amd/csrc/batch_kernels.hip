@@ -133,15 +133,25 @@ namespace qsv
             }
         }
 
-        __global__ void k_zsum_finish(const double *__restrict__ part, int nblocks, int n, double *__restrict__ out)
+        // One workgroup per qubit: thread j sums partials j, j + 256, ... in order, then a
+        // fixed tree over the threads (the order does not depend on scheduling).
+        __global__ __launch_bounds__(kZBlock) void k_zsum_finish(const double *__restrict__ part, int nblocks, int n, double *__restrict__ out)
         {
-            const int q = threadIdx.x;
-            if (q >= n)
-                return;
+            __shared__ double red[kZBlock];
+            const int q = blockIdx.x;
             double acc = 0.0;
-            for (int b = 0; b < nblocks; ++b)
+            for (int b = threadIdx.x; b < nblocks; b += kZBlock)
                 acc += part[uint64_t(b) * kZMax + q];
-            out[q] = acc;
+            red[threadIdx.x] = acc;
+            __syncthreads();
+            for (int st = kZBlock / 2; st >= 1; st /= 2)
+            {
+                if (threadIdx.x < st)
+                    red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + st];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0)
+                out[q] = red[0];
         }
 
         __global__ void k_zero_counter(unsigned *c) { *c = 0u; }
@@ -224,7 +234,7 @@ namespace qsv
         const int g = zsum_blocks(n);
         hipLaunchKernelGGL(k_zsum_partial, dim3(g), dim3(kZBlock), 0, s, state, n, partials);
         check(hipGetLastError(), "zsum kernel");
-        hipLaunchKernelGGL(k_zsum_finish, dim3(1), dim3(64), 0, s, partials, g, n, out);
+        hipLaunchKernelGGL(k_zsum_finish, dim3(n), dim3(kZBlock), 0, s, partials, g, n, out);
         check(hipGetLastError(), "zsum kernel");
     }
 
