@@ -24,8 +24,8 @@ Also reported:
                    launch against a per-circuit plain-method loop, with max |d<Z>|;
   planner       -- the learned fusion planner: launch timings recorded at 26-32 qubits,
                    cost-model MLP trained on the GPU, then a 33-qubit QFT and a 33-qubit
-                   brick circuit of depth 40 (complex128 fused): one sweep per greedy
-                   block, the default stage plan, and the plan the model picks.
+                   brick circuit of depth 40 (complex128 fused): the greedy fusion
+                   fallback, the plan the model picks, and the stage heuristic.
 """
 import argparse
 import json
@@ -208,11 +208,12 @@ def batch_leg(count, seed=20250404, direct=False):
 def planner_leg(n, record_qubits, record_count, depth, k):
     """Learned fusion planner on complex128 fused blocks: record launch timings (single
     blocks and multi-block stages) on states of `record_qubits` qubits, train the cost-model
-    MLP on the GPU, then run a QFT and a random brick circuit at n qubits three ways: one
-    sweep per greedy block, the default plan (stages on 2^12-amplitude LDS tiles) and the
-    plan the model picks among both kinds (kept only when predicted 3 % faster than the
-    default).  The complex128 results must agree (<Z_q> in FP64); agreement with the FP64
-    oracle is covered by the tests at smaller n."""
+    MLP on the GPU, then run a QFT and a random brick circuit at n qubits three ways: the
+    greedy fusion fallback (one sweep per block), the plan the model picks among block plans
+    and stage plans (kept only when predicted 3 % faster than the greedy plan), and the
+    stage heuristic alone (stages on 2^12-amplitude LDS tiles) for reference.  The
+    complex128 results must agree (<Z_q> in FP64); agreement with the FP64 oracle is covered
+    by the tests at smaller n."""
     lib = pyqcs.gpu_library()
     F = pyqcs.planner_num_features(lib)
     Xs, ys = [], []
@@ -248,7 +249,10 @@ def planner_leg(n, record_qubits, record_count, depth, k):
         for name, circ in (("qft", circuits.qft(n)), ("brick", circuits.brick(n, depth, seed=CIRCUIT_SEED + 5))):
             res = {"gates": len(circ)}
             zs = {}
-            for mode, stage_bits, use_model in (("per_block", 0, False), ("default", 12, False), ("learned", 12, True)):
+            # greedy: the greedy fusion fallback (one sweep per block); learned: the model
+            # chooses among block plans and stage plans, judged by its prediction for the
+            # greedy plan; stages: the stage heuristic alone, for reference
+            for mode, stage_bits, use_model in (("greedy", 0, False), ("learned", 0, True), ("stages", 12, False)):
                 sim.set_stage_bits(stage_bits)
                 pyqcs.set_planner(sim, model if use_model else None)
                 sim.call("set_zero_state")
@@ -265,14 +269,14 @@ def planner_leg(n, record_qubits, record_count, depth, k):
                 res[f"{mode}_probability_total"] = total
                 if use_model:
                     st = pyqcs.planner_stats(sim)
-                    res["pred_default_us"] = st["pred_greedy_us"]
+                    res["pred_greedy_us"] = st["pred_greedy_us"]
                     res["pred_chosen_us"] = st["pred_chosen_us"]
                     res["learned_choices"] = st["learned_choices"]
                     res["flushes"] = st["flushes"]
                 log(f"planner leg: {name} {mode}: {dt:.3f}s, {b1 - b0} sweeps")
-            res["learned_vs_default"] = res["default_s"] / res["learned_s"]
-            res["default_vs_per_block"] = res["per_block_s"] / res["default_s"]
-            res["max_abs_diff_expz"] = float(max(np.max(np.abs(zs[a] - zs["per_block"])) for a in ("default", "learned")))
+            res["learned_vs_greedy"] = res["greedy_s"] / res["learned_s"]
+            res["stages_vs_greedy"] = res["greedy_s"] / res["stages_s"]
+            res["max_abs_diff_expz"] = float(max(np.max(np.abs(zs[a] - zs["greedy"])) for a in ("stages", "learned")))
             out[name] = res
     finally:
         pyqcs.set_planner(sim, None)
