@@ -84,6 +84,7 @@ namespace qsv
         }
 
         typedef float v2f __attribute__((ext_vector_type(2)));
+        typedef float v4f __attribute__((ext_vector_type(4)));
 
         // Amplitude load/store in FP64 registers for both storage types; complex64 values are
         // widened exactly (float -> double) and rounded to nearest on the way back.
@@ -505,25 +506,58 @@ namespace qsv
             if (group >= ngroups) // whole wave leaves together; no workgroup barrier below
                 return;
             double *tl = tiles[wave];
-            const int run = lane >> 4, col = lane & 15;
             double acc = 0.0;
+            // 8-byte entries (complex64 amplitudes, level sums) go two per lane so that each
+            // load moves 16 B: lane = (run, column pair); 16-byte amplitudes one per lane
+            constexpr bool PAIRS = !(FROM_STATE && sizeof(A) == 16);
 #pragma unroll
             for (int chunk = 0; chunk < 4; ++chunk)
             {
-#pragma unroll
-                for (int j = 0; j < 16; ++j)
+                if (PAIRS)
                 {
-                    const int r = 4 * j + run;
-                    const uint64_t e = group * 4096 + uint64_t(r) * 64 + 16 * chunk + col;
-                    double v = 0.0;
-                    if (e < count)
+                    const int run = lane >> 3, pc = lane & 7;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
                     {
-                        if (FROM_STATE)
-                            v = prob(lda(static_cast<const A *>(src) + e));
-                        else
-                            v = static_cast<const double *>(src)[e];
+                        const int r = 8 * j + run;
+                        const uint64_t e = group * 4096 + uint64_t(r) * 64 + 16 * chunk + 2 * pc;
+                        double v0 = 0.0, v1 = 0.0;
+                        if (e + 1 < count)
+                        {
+                            if (FROM_STATE)
+                            {
+                                const v4f z = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(static_cast<const A *>(src) + e));
+                                v0 = prob(v2d{double(z.x), double(z.y)});
+                                v1 = prob(v2d{double(z.z), double(z.w)});
+                            }
+                            else
+                            {
+                                const v2d z = *reinterpret_cast<const v2d *>(static_cast<const double *>(src) + e);
+                                v0 = z.x;
+                                v1 = z.y;
+                            }
+                        }
+                        else if (e < count)
+                        {
+                            v0 = FROM_STATE ? prob(lda(static_cast<const A *>(src) + e)) : static_cast<const double *>(src)[e];
+                        }
+                        tl[r * kFoldRow + 2 * pc] = v0;
+                        tl[r * kFoldRow + 2 * pc + 1] = v1;
                     }
-                    tl[r * kFoldRow + col] = v;
+                }
+                else
+                {
+                    const int run = lane >> 4, col = lane & 15;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j)
+                    {
+                        const int r = 4 * j + run;
+                        const uint64_t e = group * 4096 + uint64_t(r) * 64 + 16 * chunk + col;
+                        double v = 0.0;
+                        if (e < count)
+                            v = prob(lda(static_cast<const A *>(src) + e));
+                        tl[r * kFoldRow + col] = v;
+                    }
                 }
                 wave_sync();
 #pragma unroll
