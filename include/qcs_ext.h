@@ -107,10 +107,11 @@ int qcs_ext_permute(qcs_simulator *sim, const bit_num_t *new_perm_p2l, bit_num_t
  * FP32).  Must be called before allocate_memory. */
 int qcs_ext_set_precision(qcs_simulator *sim, int mode, int max_block_qubits);
 
-/* Complex64 precision modes (1, 3): blocks of a flush are grouped into stages, each applied
- * to LDS-resident tiles of 2^tile_bits amplitudes in one sweep of the state (tile_bits 12
- * or 13, default 13; used when the state has at least tile_bits + 4 qubits), or launched one
- * sweep per block (tile_bits 0).  Any time; queued gates are flushed first. */
+/* Fused precision modes: blocks of a flush are grouped into stages, each applied to
+ * LDS-resident tiles of 2^tile_bits amplitudes in one sweep of the state (tile_bits 11, 12 or
+ * 13, default 13; complex128 tiles hold 2^12; used when the state has at least tile_bits + 4
+ * qubits), or launched one sweep per block (tile_bits 0).  Any time; queued gates are flushed
+ * first. */
 int qcs_ext_set_stage_bits(qcs_simulator *sim, int tile_bits);
 
 /* Apply all queued gates now (no-op in mode 0). */
@@ -139,7 +140,7 @@ int qcs_ext_batch_add_circuit(qcs_ext_batch *batch, bit_num_t num_qubits, int32_
  * block_size (256, 512, 1024) threads per workgroup; larger ones as fused blocks of
  * block_qubits (4, 5) qubits, spread over `streams` (1..4) graph branches, grouped into stages
  * on LDS tiles of 2^stage_bits amplitudes when they have at least stage_bits + 4 qubits
- * (stage_bits 12 or 13; 0: one launch per block). */
+ * (stage_bits 11, 12 or 13; 0: one launch per block). */
 int qcs_ext_batch_set_tuning(qcs_ext_batch *batch, int lds_max_qubits, int block_size, int block_qubits, int streams, int stage_bits);
 /* Searches the tuning parameters on this batch; writes the chosen 5 values to tuning. */
 int qcs_ext_batch_autotune(qcs_ext_batch *batch, int *tuning);

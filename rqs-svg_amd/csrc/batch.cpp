@@ -123,8 +123,8 @@ namespace qcore
             throw std::runtime_error("batch: block qubits must be 4 or 5");
         if (t.streams < 1 || t.streams > 4)
             throw std::runtime_error("batch: streams must be in [1, 4]");
-        if (t.stage_bits != 0 && t.stage_bits != 12 && t.stage_bits != 13)
-            throw std::runtime_error("batch: stage bits must be 0, 12 or 13");
+        if (t.stage_bits != 0 && (t.stage_bits < 11 || t.stage_bits > 13))
+            throw std::runtime_error("batch: stage bits must be 0, 11, 12 or 13");
         release();
         tuning_ = t;
     }

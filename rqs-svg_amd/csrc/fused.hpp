@@ -137,14 +137,14 @@ namespace qcore
         void flush();
         void drop() { queue_.clear(); }
         void swap_physical(int a, int b);
-        // LDS tile bits of the stage kernel (12 or 13), or 0 for one launch per block.
+        // LDS tile bits of the stage kernel (11, 12 or 13), or 0 for one launch per block.
         void set_stage_bits(int m);
         int stage_bits() const { return stage_bits_; }
         // tile bits of the stage kernel for this state (complex128 tiles hold 2^12), 0 if
         // per-block launches are used; stages need at least 16 tiles
         int tile_bits() const
         {
-            const int m = f64_ ? std::min(stage_bits_, 12) : stage_bits_;
+            const int m = f64_ ? 12 : stage_bits_; // complex128 tiles are 2^12
             return (stage_bits_ > 0 && sim_.n_ >= m + 4) ? m : 0;
         }
         bool uses_stages() const { return tile_bits() > 0; }

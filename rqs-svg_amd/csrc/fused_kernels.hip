@@ -32,6 +32,15 @@ namespace qsv
 
         constexpr int kWaves = 4;
         constexpr int kThreads = 64 * kWaves;
+#ifndef QSV_STAGE_WAVES_13
+#define QSV_STAGE_WAVES_13 4
+#endif
+#ifndef QSV_STAGE_WAVES_12
+#define QSV_STAGE_WAVES_12 4
+#endif
+#ifndef QSV_STAGE_WAVES_11
+#define QSV_STAGE_WAVES_11 2
+#endif
 #ifndef QSV_BLOCK_PREFETCH
 #define QSV_BLOCK_PREFETCH 1
 #endif
@@ -468,6 +477,17 @@ namespace qsv
             }
         }
 
+        // Barrier among the W waves of a stage workgroup: a single wave only orders its own
+        // LDS accesses.
+        template <int W>
+        __device__ __forceinline__ void stage_barrier()
+        {
+            if (W == 1)
+                wave_sync();
+            else
+                lds_barrier();
+        }
+
         // Stage kernel: one workgroup holds a tile of 2^M amplitudes in LDS -- the
         // amplitudes whose index bits outside the stage's tile qubits T are fixed -- and
         // applies every block of the stage to it before writing it back, so a stage costs
@@ -482,8 +502,8 @@ namespace qsv
         // place.  Workgroup barriers (LDS-only) separate the blocks.  The block matrices come
         // from global memory (L2) into registers per block; block descriptors through scalar
         // loads.
-        template <int KQ, int M, bool X3>
-        __global__ __launch_bounds__(kThreads) void k_stage(c64 *__restrict__ s, uint64_t ntiles, Subspace tsub, StageOp op)
+        template <int KQ, int M, bool X3, int W>
+        __global__ __launch_bounds__(64 * W) void k_stage(c64 *__restrict__ s, uint64_t ntiles, Subspace tsub, StageOp op)
         {
             constexpr int D = 2 << KQ;
             constexpr int MT = D / 32;
@@ -491,8 +511,10 @@ namespace qsv
             constexpr int AMPS = H / 2;
             constexpr int NT = 1 << M;
             constexpr int COLS = NT >> KQ;
-            constexpr int PER_WAVE = COLS / 32 / kWaves;
-            constexpr int LOADS = NT / (2 * kThreads);
+            constexpr int NTH = 64 * W; // threads per workgroup
+            constexpr int PER_WAVE = COLS / 32 / W;
+            constexpr int LOADS = NT / (2 * NTH);
+            constexpr int TB = 1 + __builtin_ctz(NTH); // tile bits set by the thread index
             static_assert(PER_WAVE >= 1 && LOADS >= 1, "tile too small");
             __shared__ __attribute__((aligned(16))) v2f tile[NT];
 
@@ -504,7 +526,7 @@ namespace qsv
             // global offset of this thread's first amplitude (LDS index 2 * threadIdx.x)
             uint64_t g_t = 0;
 #pragma unroll
-            for (int b = 1; b < 9 && b < M; ++b)
+            for (int b = 1; b < TB && b < M; ++b)
                 if (((2 * threadIdx.x) >> b) & 1)
                     g_t += op.gw[b];
 
@@ -516,13 +538,13 @@ namespace qsv
                 {
                     uint64_t gi = 0;
 #pragma unroll
-                    for (int b = 9; b < M; ++b)
-                        if (((i * 2 * kThreads) >> b) & 1)
+                    for (int b = TB; b < M; ++b)
+                        if (((i * 2 * NTH) >> b) & 1)
                             gi += op.gw[b];
                     const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (base + gi)));
-                    *reinterpret_cast<v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * kThreads)]) = v;
+                    *reinterpret_cast<v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * NTH)]) = v;
                 }
-                lds_barrier();
+                stage_barrier<W>();
                 for (int bi = 0; bi < op.nblocks; ++bi)
                 {
                     const __attribute__((address_space(4))) StageBlock &B = as_constant(op.blocks)[bi];
@@ -577,20 +599,20 @@ namespace qsv
                                 tile[swz64(cbase + o1)] = v2f{d[mt][4 * q + 2], d[mt][4 * q + 3]};
                             }
                     }
-                    lds_barrier();
+                    stage_barrier<W>();
                 }
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
                 {
                     uint64_t gi = 0;
 #pragma unroll
-                    for (int b = 9; b < M; ++b)
-                        if (((i * 2 * kThreads) >> b) & 1)
+                    for (int b = TB; b < M; ++b)
+                        if (((i * 2 * NTH) >> b) & 1)
                             gi += op.gw[b];
-                    const v4f v = *reinterpret_cast<const v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * kThreads)]);
+                    const v4f v = *reinterpret_cast<const v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * NTH)]);
                     __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(s + (base + gi)));
                 }
-                lds_barrier();
+                stage_barrier<W>();
             }
         }
 
@@ -745,8 +767,8 @@ namespace qsv
     {
         if (op.kq != 4 && op.kq != 5)
             throw std::runtime_error("stage: kq must be 4 or 5");
-        if (op.m != 12 && op.m != 13)
-            throw std::runtime_error("stage: tile bits must be 12 or 13");
+        if (op.m < 11 || op.m > 13)
+            throw std::runtime_error("stage: tile bits must be 11, 12 or 13");
         if (n <= op.m || tsub.nops != op.m)
             throw std::runtime_error("stage: the tile must be a proper part of the state");
         for (int k = 0; k < tsub.nops; ++k)
@@ -763,16 +785,22 @@ namespace qsv
         if (op.nblocks < 1 || op.blocks == nullptr || op.R == nullptr)
             throw std::runtime_error("stage: no blocks");
         const uint64_t ntiles = uint64_t(1) << (n - op.m);
-        uint64_t grid = ntiles < 8192 ? ntiles : 8192;
-        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op); };
+        const uint64_t grid = ntiles < 8192 ? ntiles : 8192;
+        // waves per workgroup by tile size: smaller tiles, fewer waves, more workgroups per CU
+        auto go = [&](auto kern, int waves) { hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), 0, st, state, ntiles, tsub, op); };
+        constexpr int W13 = QSV_STAGE_WAVES_13, W12 = QSV_STAGE_WAVES_12, W11 = QSV_STAGE_WAVES_11;
         if (op.kq == 5 && op.m == 13)
-            op.x3 ? go(k_stage<5, 13, true>) : go(k_stage<5, 13, false>);
+            op.x3 ? go(k_stage<5, 13, true, W13>, W13) : go(k_stage<5, 13, false, W13>, W13);
+        else if (op.kq == 5 && op.m == 12)
+            op.x3 ? go(k_stage<5, 12, true, W12>, W12) : go(k_stage<5, 12, false, W12>, W12);
         else if (op.kq == 5)
-            op.x3 ? go(k_stage<5, 12, true>) : go(k_stage<5, 12, false>);
+            op.x3 ? go(k_stage<5, 11, true, W11>, W11) : go(k_stage<5, 11, false, W11>, W11);
         else if (op.m == 13)
-            op.x3 ? go(k_stage<4, 13, true>) : go(k_stage<4, 13, false>);
+            op.x3 ? go(k_stage<4, 13, true, W13>, W13) : go(k_stage<4, 13, false, W13>, W13);
+        else if (op.m == 12)
+            op.x3 ? go(k_stage<4, 12, true, W12>, W12) : go(k_stage<4, 12, false, W12>, W12);
         else
-            op.x3 ? go(k_stage<4, 12, true>) : go(k_stage<4, 12, false>);
+            op.x3 ? go(k_stage<4, 11, true, W11>, W11) : go(k_stage<4, 11, false, W11>, W11);
         check(hipGetLastError(), "stage kernel");
     }
 
