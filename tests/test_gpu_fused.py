@@ -242,10 +242,10 @@ def test_stages_vs_block_launches(gpu, precision):
 
 def test_stage_bits_validation(gpu):
     with pyqcs.Simulator(gpu, 8, precision=1) as sim:
-        for bad in (5, 11, 14, -1):
+        for bad in (5, 10, 14, -1):
             assert sim.lib.qcs_ext_set_stage_bits(sim.h, bad) == 0
-        sim.set_stage_bits(0)
-        sim.set_stage_bits(12)
+        for good in (0, 11, 12, 13):
+            sim.set_stage_bits(good)
 
 
 @pytest.mark.parametrize("k", [3, 4])
