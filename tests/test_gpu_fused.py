@@ -196,7 +196,7 @@ def test_fused_complex128_qft_and_measure(gpu, oracle):
 
 
 @pytest.mark.parametrize("precision", [1, 3])
-@pytest.mark.parametrize("m", [12, 13])
+@pytest.mark.parametrize("m", [11, 12, 13])
 def test_stages_match_oracle(gpu, oracle, precision, m):
     # stages (several blocks per sweep on LDS-resident tiles), with gates wider than a
     # block (up to 7 qubits) falling back to single launches
