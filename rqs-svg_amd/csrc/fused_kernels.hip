@@ -35,11 +35,14 @@ namespace qsv
 #ifndef QSV_STAGE_WAVES_13
 #define QSV_STAGE_WAVES_13 4
 #endif
+// waves per stage workgroup by tile bits (measured on 32-qubit brick circuits: 2^13 tiles
+// with 4 waves 23.2 ms per stage; 2^12 with 2 waves 21.7 ms (4 waves: 27.5 ms); 2^11 with
+// 1 wave 19.6 ms (2 waves: 25.1 ms))
 #ifndef QSV_STAGE_WAVES_12
-#define QSV_STAGE_WAVES_12 4
+#define QSV_STAGE_WAVES_12 2
 #endif
 #ifndef QSV_STAGE_WAVES_11
-#define QSV_STAGE_WAVES_11 2
+#define QSV_STAGE_WAVES_11 1
 #endif
 #ifndef QSV_BLOCK_PREFETCH
 #define QSV_BLOCK_PREFETCH 1
