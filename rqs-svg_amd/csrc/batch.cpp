@@ -433,7 +433,7 @@ namespace qcore
         try_set([](BatchTuning &t, int v) { t.block_size = v; }, {256, 512, 1024});
         try_set([](BatchTuning &t, int v) { t.block_qubits = v; }, {4, 5});
         try_set([](BatchTuning &t, int v) { t.streams = v; }, {1, 2, 4});
-        try_set([](BatchTuning &t, int v) { t.stage_bits = v; }, {0, 12, 13});
+        try_set([](BatchTuning &t, int v) { t.stage_bits = v; }, {0, 11, 12, 13});
         set_tuning(best);
         return best;
     }
