@@ -117,8 +117,8 @@ class TestGpuCli:
         ghz_checks(rows, 20, 24)
 
     def test_same_outcomes_as_oracle_cli(self, plugin_dir):
-        # one measurement per sample decides every clbit of a GHZ state; both engines draw
-        # the same SplitMix64-extended mt19937_64 stream from the same seed
+        # one measurement per sample decides every clbit of a GHZ state; both libraries draw
+        # from std::mt19937_64 seeded with the same value, one draw per undecided measurement
         if not ORACLE_CLI.exists():
             pytest.skip("oracle CLI not built")
         a = parse(run_cli(GPU_CLI, ["-s", 16, "--seed", 3, plugin_dir / "ghz_measure.so"], 12).stdout)
