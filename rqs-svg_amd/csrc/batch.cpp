@@ -1,16 +1,20 @@
 // Batching form host side: partition, plan, capture into one hipGraph, autotune.
 //
 // Circuits of up to `lds_max_qubits` qubits go to the persistent LDS-resident kernel (one
-// workgroup per circuit, work-counter driven); larger ones keep a complex64 state in HBM
-// and run as fused k-qubit blocks.  Everything -- counter reset, the persistent kernel,
-// the block sequences spread over several streams, the <Z> reductions -- is captured once
-// into a hipGraph and replayed with a single launch.
+// workgroup per circuit, work-counter driven); larger ones keep a complex64 state in HBM.
+// Those of at least stage_bits + 4 qubits run as stages on LDS tiles, batched across
+// circuits: one launch per stage round covers the r-th stage of every such circuit.  The
+// rest run as fused k-qubit blocks, one launch each, spread over several streams.
+// Everything -- counter reset, the persistent kernel, the fills, the stage rounds, the
+// block sequences, the <Z> reductions -- is captured once into a hipGraph and replayed
+// with a single launch.
 #include "batch.hpp"
 #include "simulator.hpp"
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <functional>
 #include <map>
 #include <memory>
@@ -190,24 +194,24 @@ namespace qcore
             small.push_back(bc);
             small_max_n = std::max(small_max_n, C.n);
         }
-        // ---- large circuits: fused blocks on HBM states ----
-        struct LargeItem
-        {
-            bool stage;
-            PreparedStage st;
-            PreparedBlock b;
-        };
-        struct LargeRun
+        // ---- large circuits: complex64 states in HBM ----
+        // Circuits of >= stage_bits + 4 qubits run in stage rounds: launch r applies stage r
+        // of every such circuit (tiles numbered across the circuits), after one basis-state
+        // fill for all of them and before one <Z> reduction for all of them.  The others run
+        // their fused blocks one launch each, spread over the streams.
+        struct Staged
         {
             int ci;
-            uint64_t state_offset;
-            size_t partial_offset;
-            std::vector<LargeItem> items;
+            std::vector<PreparedStage> stages;
         };
-        std::vector<LargeRun> runs;
+        struct Blocked
+        {
+            int ci;
+            std::vector<PreparedBlock> blocks;
+        };
+        std::vector<Staged> staged;
+        std::vector<Blocked> blocked;
         std::vector<float> mats;
-        uint64_t state_total = 0;
-        size_t partial_total = 0;
         for (int ci : large)
         {
             const Circuit &C = circuits_[ci];
@@ -225,31 +229,43 @@ namespace qcore
             const int kq = (C.n >= 5 && T.block_qubits == 5) ? 5 : 4;
             if (C.n < kq)
                 throw std::runtime_error("batch: circuit too small for the HBM path");
-            LargeRun lr{ci, state_total, partial_total, {}};
-            if (T.stage_bits > 0 && C.n >= T.stage_bits + 4)
+            if (T.stage_bits > 0 && C.n >= T.stage_bits + 4 && !rg.empty())
             {
-                for (const auto &st : plan_stages(rg, C.n, T.stage_bits, kq))
+                const auto plan = plan_stages(rg, C.n, T.stage_bits, kq);
+                // batched gates span at most 2 qubits, so every stage has a tile
+                bool all_tiled = true;
+                for (const auto &st : plan)
+                    all_tiled = all_tiled && !st.tile.empty();
+                if (all_tiled)
                 {
-                    if (!st.tile.empty())
-                    {
-                        lr.items.push_back(LargeItem{true, prepare_stage(rg, st, kq, T.stage_bits, C.n, mats, false), {}});
-                        continue;
-                    }
-                    std::vector<ResolvedGate> one{rg[st.gates[0]]};
-                    for (auto &b : prepare_blocks(one, plan_greedy(one, kq), kq, C.n, mats))
-                        lr.items.push_back(LargeItem{false, {}, b});
+                    Staged sc{ci, {}};
+                    for (const auto &st : plan)
+                        sc.stages.push_back(prepare_stage(rg, st, kq, T.stage_bits, C.n, mats, false));
+                    staged.push_back(std::move(sc));
+                    continue;
                 }
             }
-            else
-            {
-                for (auto &b : prepare_blocks(rg, plan_greedy(rg, kq), kq, C.n, mats))
-                    lr.items.push_back(LargeItem{false, {}, b});
-            }
-            state_total += uint64_t(1) << C.n;
-            partial_total += size_t(qsv::zsum_blocks(C.n)) * 32;
-            runs.push_back(std::move(lr));
+            blocked.push_back(Blocked{ci, prepare_blocks(rg, plan_greedy(rg, kq), kq, C.n, mats)});
         }
-        // ---- device buffers ----
+        // largest first: round launches hand out the biggest states' tiles first, and with
+        // sizes descending every state starts at a multiple of its own size
+        std::stable_sort(staged.begin(), staged.end(), [&](const Staged &a, const Staged &b) { return circuits_[a.ci].n > circuits_[b.ci].n; });
+        std::stable_sort(blocked.begin(), blocked.end(), [&](const Blocked &a, const Blocked &b) { return circuits_[a.ci].n > circuits_[b.ci].n; });
+        uint64_t state_total = 0;
+        size_t partial_total = 0;
+        std::vector<uint64_t> state_off, partial_off; // staged circuits, then blocked ones
+        auto place = [&](int ci) {
+            state_off.push_back(state_total);
+            partial_off.push_back(partial_total);
+            state_total += uint64_t(1) << circuits_[ci].n;
+            partial_total += size_t(qsv::zsum_blocks(circuits_[ci].n)) * 32;
+        };
+        for (const auto &sc : staged)
+            place(sc.ci);
+        const uint64_t staged_amps = state_total;
+        for (const auto &bc : blocked)
+            place(bc.ci);
+        // ---- device buffers (addresses go into the task arrays below) ----
         auto upload = [](void **dst, const void *src, size_t bytes) {
             if (bytes == 0)
                 return;
@@ -258,13 +274,75 @@ namespace qcore
         };
         upload((void **)&d_circs_, small.data(), small.size() * sizeof(qsv::BatchCircuit));
         upload((void **)&d_ops_, ops.data(), ops.size() * sizeof(qsv::BatchOp));
-        upload((void **)&d_mats_, mats.data(), mats.size() * sizeof(float));
         qsv::check(hipMalloc((void **)&d_out_, std::max(1, total_qubits_) * sizeof(double)), "hipMalloc batch");
         qsv::check(hipMalloc((void **)&d_counter_, sizeof(unsigned)), "hipMalloc batch");
         if (state_total)
             qsv::check(hipMalloc((void **)&d_states_, state_total * sizeof(qsv::c64)), "hipMalloc batch states");
         if (partial_total)
             qsv::check(hipMalloc((void **)&d_partials_, partial_total * sizeof(double)), "hipMalloc batch");
+        // ---- staged circuits: task arrays of the rounds, the fill and the reduction ----
+        struct Round
+        {
+            size_t task_offset;
+            int ntasks;
+            uint64_t tiles;
+        };
+        std::vector<Round> rounds;
+        size_t zsum_offset = 0, starts_offset = 0;
+        uint32_t zsum_blocks_total = 0, zsum_qubits_total = 0;
+        const int kq_staged = T.block_qubits;
+        if (!staged.empty())
+        {
+            size_t nrounds = 0;
+            for (const auto &sc : staged)
+                nrounds = std::max(nrounds, sc.stages.size());
+            for (size_t r = 0; r < nrounds; ++r)
+            {
+                std::vector<qsv::StageTask> tasks;
+                uint64_t tiles = 0;
+                for (size_t i = 0; i < staged.size(); ++i)
+                {
+                    if (r >= staged[i].stages.size())
+                        continue;
+                    const PreparedStage &p = staged[i].stages[r];
+                    if (p.kq != kq_staged)
+                        throw std::runtime_error("batch: stage round with mixed block sizes");
+                    tasks.push_back(stage_task(p, d_states_ + state_off[i], tiles));
+                    tiles += uint64_t(1) << (p.n - p.m);
+                }
+                rounds.push_back(Round{append_stage_tasks(mats, tasks), (int)tasks.size(), tiles});
+            }
+            std::vector<qsv::ZsumTask> zt;
+            std::vector<uint64_t> starts;
+            for (size_t i = 0; i < staged.size(); ++i)
+            {
+                const int n = circuits_[staged[i].ci].n;
+                qsv::ZsumTask z{};
+                z.state = d_states_ + state_off[i];
+                z.part = d_partials_ + partial_off[i];
+                z.out = d_out_ + circuits_[staged[i].ci].out_offset;
+                z.block_begin = zsum_blocks_total;
+                z.qubit_begin = zsum_qubits_total;
+                z.n = n;
+                z.nblocks = qsv::zsum_blocks(n);
+                zsum_blocks_total += uint32_t(z.nblocks);
+                zsum_qubits_total += uint32_t(n);
+                zt.push_back(z);
+                starts.push_back(state_off[i]);
+            }
+            static_assert(sizeof(qsv::ZsumTask) % 16 == 0, "ZsumTask keeps 16-B alignment");
+            auto append = [&](const void *src, size_t bytes) {
+                while (mats.size() % 4)
+                    mats.push_back(0.f);
+                const size_t off = mats.size();
+                mats.resize(off + (bytes + sizeof(float) - 1) / sizeof(float));
+                std::memcpy(mats.data() + off, src, bytes);
+                return off;
+            };
+            zsum_offset = append(zt.data(), zt.size() * sizeof(qsv::ZsumTask));
+            starts_offset = append(starts.data(), starts.size() * sizeof(uint64_t));
+        }
+        upload((void **)&d_mats_, mats.data(), mats.size() * sizeof(float));
         int grid = 0;
         if (!small.empty())
         {
@@ -287,35 +365,41 @@ namespace qcore
         }
         hipStream_t s0 = streams_[0];
         const int nsmall = (int)small.size();
+        const int nstaged = (int)staged.size();
+        std::vector<uint64_t> blocked_state(state_off.begin() + nstaged, state_off.end());
+        std::vector<uint64_t> blocked_partial(partial_off.begin() + nstaged, partial_off.end());
         // the whole batch as launches on the streams: captured once into the graph below,
         // or issued directly by run_direct
-        work_ = [this, runs = std::move(runs), grid, nsmall, small_max_n, ns, block_size = T.block_size]() {
+        work_ = [this, blocked = std::move(blocked), blocked_state, blocked_partial, rounds, nstaged, staged_amps, kq_staged,
+                 stage_bits = T.stage_bits, zsum_offset, starts_offset, zsum_blocks_total, zsum_qubits_total, grid, nsmall,
+                 small_max_n, ns, block_size = T.block_size]() mutable {
             hipStream_t s0 = streams_[0];
             qsv::check(hipEventRecord(fork_join_[0], s0), "hipEventRecord");
             for (int i = 1; i < ns; ++i)
                 qsv::check(hipStreamWaitEvent(streams_[i], fork_join_[0], 0), "hipStreamWaitEvent");
             if (nsmall > 0)
                 qsv::launch_batch_lds(s0, block_size, grid, small_max_n, d_circs_, nsmall, d_ops_, d_out_, d_counter_);
-            // large circuits, biggest first, over the streams (the LDS kernel occupies s0)
-            std::vector<int> order(runs.size());
-            for (size_t i = 0; i < order.size(); ++i)
-                order[i] = (int)i;
-            std::sort(order.begin(), order.end(), [&](int a, int b) { return circuits_[runs[a].ci].n > circuits_[runs[b].ci].n; });
-            for (size_t oi = 0; oi < order.size(); ++oi)
+            // the LDS kernel occupies s0; the stage rounds take the next stream, the blocked
+            // circuits (biggest first) the ones after it
+            int next = 0;
+            auto stream = [&]() { return streams_[ns == 1 ? 0 : 1 + (next++) % (ns - 1)]; };
+            if (nstaged > 0)
             {
-                auto lr = runs[order[oi]];
-                const Circuit &C = circuits_[lr.ci];
-                hipStream_t s = streams_[ns == 1 ? 0 : 1 + oi % (ns - 1)];
-                qsv::c64 *st = d_states_ + lr.state_offset;
+                hipStream_t s = stream();
+                qsv::launch_basis_states(s, d_states_, staged_amps, reinterpret_cast<const uint64_t *>(d_mats_ + starts_offset), nstaged);
+                for (const auto &r : rounds)
+                    launch_stage_tasks(s, kq_staged, stage_bits, false, d_mats_, r.task_offset, r.ntasks, r.tiles);
+                qsv::launch_zsum_tasks(s, reinterpret_cast<const qsv::ZsumTask *>(d_mats_ + zsum_offset), nstaged, zsum_blocks_total, zsum_qubits_total);
+            }
+            for (size_t i = 0; i < blocked.size(); ++i)
+            {
+                const Circuit &C = circuits_[blocked[i].ci];
+                hipStream_t s = stream();
+                qsv::c64 *st = d_states_ + blocked_state[i];
                 qsv::launch_fill_zero_c64(s, st, uint64_t(1) << C.n);
-                for (auto &it : lr.items)
-                {
-                    if (it.stage)
-                        launch_prepared_stage(s, st, C.n, it.st, d_mats_);
-                    else
-                        launch_prepared(s, st, C.n, it.b, d_mats_);
-                }
-                qsv::launch_zsum(s, st, C.n, d_partials_ + lr.partial_offset, d_out_ + C.out_offset);
+                for (auto &b : blocked[i].blocks)
+                    launch_prepared(s, st, C.n, b, d_mats_);
+                qsv::launch_zsum(s, st, C.n, d_partials_ + blocked_partial[i], d_out_ + C.out_offset);
             }
             for (int i = 1; i < ns; ++i)
             {

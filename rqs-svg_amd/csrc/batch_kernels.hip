@@ -97,15 +97,16 @@ namespace qsv
         constexpr int kZBlock = 256;
         constexpr int kZMax = 32;
 
-        __global__ __launch_bounds__(kZBlock) void k_zsum_partial(const c64 *__restrict__ s, int n, double *__restrict__ part)
+        // Partial sums of workgroup `blk` of `nblk` over a state of n qubits (one block of
+        // the grid-stride loop; the same order whatever the launch shape around it).
+        __device__ __forceinline__ void zsum_partial_block(const c64 *__restrict__ s, int n, uint64_t blk, uint64_t nblk, double *__restrict__ part, double *red)
         {
-            __shared__ double red[kZBlock];
             double acc[kZMax];
 #pragma unroll
             for (int q = 0; q < kZMax; ++q)
                 acc[q] = 0.0;
             const uint64_t N = uint64_t(1) << n;
-            for (uint64_t i = uint64_t(blockIdx.x) * kZBlock + threadIdx.x; i < N; i += uint64_t(gridDim.x) * kZBlock)
+            for (uint64_t i = blk * kZBlock + threadIdx.x; i < N; i += nblk * kZBlock)
             {
                 const v2f a = *reinterpret_cast<const v2f *>(s + i);
                 const double pr = __builtin_fma(double(a.x), double(a.x), double(a.y) * double(a.y));
@@ -127,18 +128,46 @@ namespace qsv
                         __syncthreads();
                     }
                     if (threadIdx.x == 0)
-                        part[uint64_t(blockIdx.x) * kZMax + q] = red[0];
+                        part[blk * kZMax + q] = red[0];
                     __syncthreads();
                 }
             }
         }
 
-        // One workgroup per qubit: thread j sums partials j, j + 256, ... in order, then a
-        // fixed tree over the threads (the order does not depend on scheduling).
-        __global__ __launch_bounds__(kZBlock) void k_zsum_finish(const double *__restrict__ part, int nblocks, int n, double *__restrict__ out)
+        __global__ __launch_bounds__(kZBlock) void k_zsum_partial(const c64 *__restrict__ s, int n, double *__restrict__ part)
         {
             __shared__ double red[kZBlock];
-            const int q = blockIdx.x;
+            zsum_partial_block(s, n, blockIdx.x, gridDim.x, part, red);
+        }
+
+        // The task of workgroup g: the last task whose begin (read by `begin`) is <= g.
+        template <class F>
+        __device__ __forceinline__ int find_task(int ntasks, uint32_t g, F begin)
+        {
+            int lo = 0, hi = ntasks - 1;
+            while (lo < hi)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if (begin(mid) <= g)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            return lo;
+        }
+
+        __global__ __launch_bounds__(kZBlock) void k_zsum_partial_tasks(const ZsumTask *__restrict__ tasks, int ntasks)
+        {
+            __shared__ double red[kZBlock];
+            const int t = find_task(ntasks, blockIdx.x, [&](int i) { return tasks[i].block_begin; });
+            const ZsumTask &T = tasks[t];
+            zsum_partial_block(T.state, T.n, blockIdx.x - T.block_begin, T.nblocks, T.part, red);
+        }
+
+        // One workgroup per qubit: thread j sums partials j, j + 256, ... in order, then a
+        // fixed tree over the threads (the order does not depend on scheduling).
+        __device__ __forceinline__ void zsum_finish_qubit(const double *__restrict__ part, int nblocks, int q, double *__restrict__ out, double *red)
+        {
             double acc = 0.0;
             for (int b = threadIdx.x; b < nblocks; b += kZBlock)
                 acc += part[uint64_t(b) * kZMax + q];
@@ -154,6 +183,20 @@ namespace qsv
                 out[q] = red[0];
         }
 
+        __global__ __launch_bounds__(kZBlock) void k_zsum_finish(const double *__restrict__ part, int nblocks, int n, double *__restrict__ out)
+        {
+            __shared__ double red[kZBlock];
+            zsum_finish_qubit(part, nblocks, blockIdx.x, out, red);
+        }
+
+        __global__ __launch_bounds__(kZBlock) void k_zsum_finish_tasks(const ZsumTask *__restrict__ tasks, int ntasks)
+        {
+            __shared__ double red[kZBlock];
+            const int t = find_task(ntasks, blockIdx.x, [&](int i) { return tasks[i].qubit_begin; });
+            const ZsumTask &T = tasks[t];
+            zsum_finish_qubit(T.part, T.nblocks, blockIdx.x - T.qubit_begin, T.out, red);
+        }
+
         __global__ void k_zero_counter(unsigned *c) { *c = 0u; }
 
         __global__ void k_fill_zero_state(c64 *s, uint64_t count)
@@ -161,6 +204,19 @@ namespace qsv
             uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
             for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
                 *reinterpret_cast<v2f *>(s + i) = v2f{i == 0 ? 1.f : 0.f, 0.f};
+        }
+        __global__ void k_fill_zero(c64 *s, uint64_t count)
+        {
+            uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+                *reinterpret_cast<v2f *>(s + i) = v2f{0.f, 0.f};
+        }
+
+        __global__ void k_set_ones(c64 *base, const uint64_t *__restrict__ starts, int nstarts)
+        {
+            const int i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i < nstarts)
+                *reinterpret_cast<v2f *>(base + starts[i]) = v2f{1.f, 0.f};
         }
     } // namespace
 
@@ -244,6 +300,29 @@ namespace qsv
         if (g > 65536)
             g = 65536;
         hipLaunchKernelGGL(k_fill_zero_state, dim3(g), dim3(256), 0, s, state, count);
+        check(hipGetLastError(), "fill kernel");
+    }
+
+    void launch_zsum_tasks(hipStream_t s, const ZsumTask *tasks, int ntasks, uint32_t total_blocks, uint32_t total_qubits)
+    {
+        if (tasks == nullptr || ntasks < 1 || total_blocks < 1 || total_qubits < 1)
+            throw std::runtime_error("zsum: no tasks");
+        hipLaunchKernelGGL(k_zsum_partial_tasks, dim3(total_blocks), dim3(kZBlock), 0, s, tasks, ntasks);
+        check(hipGetLastError(), "zsum kernel");
+        hipLaunchKernelGGL(k_zsum_finish_tasks, dim3(total_qubits), dim3(kZBlock), 0, s, tasks, ntasks);
+        check(hipGetLastError(), "zsum kernel");
+    }
+
+    void launch_basis_states(hipStream_t s, c64 *base, uint64_t count, const uint64_t *starts, int nstarts)
+    {
+        if (base == nullptr || starts == nullptr || count < 1 || nstarts < 1)
+            throw std::runtime_error("basis states: nothing to fill");
+        uint64_t g = (count + 255) / 256;
+        if (g > 65536)
+            g = 65536;
+        hipLaunchKernelGGL(k_fill_zero, dim3(g), dim3(256), 0, s, base, count);
+        check(hipGetLastError(), "fill kernel");
+        hipLaunchKernelGGL(k_set_ones, dim3((nstarts + 255) / 256), dim3(256), 0, s, base, starts, nstarts);
         check(hipGetLastError(), "fill kernel");
     }
 } // namespace qsv

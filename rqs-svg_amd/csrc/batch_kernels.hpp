@@ -35,4 +35,24 @@ namespace qsv
     int zsum_blocks(int n);
     void launch_zsum(hipStream_t s, const c64 *state, int n, double *partials, double *out);
     void launch_fill_zero_c64(hipStream_t s, c64 *state, uint64_t count);
+
+    // <Z_q> of several complex64 states in two launches, with the same sums in the same order
+    // as launch_zsum per state.  Task t owns workgroups [block_begin, block_begin + nblocks)
+    // of the partial launch (nblocks = zsum_blocks(n)) and [qubit_begin, qubit_begin + n) of
+    // the finish launch; part holds nblocks * 32 doubles, out n.  Tasks are in device memory
+    // with both begins ascending from 0.
+    struct alignas(16) ZsumTask
+    {
+        const c64 *state;
+        double *part;
+        double *out;
+        uint32_t block_begin;
+        uint32_t qubit_begin;
+        int n;
+        int nblocks;
+    };
+    void launch_zsum_tasks(hipStream_t s, const ZsumTask *tasks, int ntasks, uint32_t total_blocks, uint32_t total_qubits);
+    // |0...0> in several states: zeroes [base, base + count), then writes 1 at base + starts[i]
+    // for i < nstarts (starts in device memory, each < count).
+    void launch_basis_states(hipStream_t s, c64 *base, uint64_t count, const uint64_t *starts, int nstarts);
 } // namespace qsv

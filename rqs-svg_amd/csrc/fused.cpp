@@ -510,6 +510,7 @@ namespace qcore
                         else
                         {
                             L.st = prepare_stage(gs, st, kb, m, n, mats, x3_);
+                            attach_stage(L.st, state_, mats);
                             nblocks = L.st.nblocks;
                         }
                         stage_features(n, math, kb, st.tile, nblocks, (int)gs.size(), L.f);
@@ -544,7 +545,7 @@ namespace qcore
                     if (L.tile_bits && f64_)
                         launch_prepared_stage128(sim_.stream_, state128_, n, L.st128, devd);
                     else if (L.tile_bits)
-                        launch_prepared_stage(sim_.stream_, state_, n, L.st, devf);
+                        launch_prepared_stage(sim_.stream_, L.st, devf);
                     else if (f64_)
                         launch_prepared128(sim_.stream_, state128_, n, L.b128, devd);
                     else
@@ -924,6 +925,10 @@ namespace qcore
         PreparedStage p;
         p.gates = (int)st.gates.size();
         p.nblocks = (int)desc.size();
+        p.n = n;
+        p.kq = kq;
+        p.m = m;
+        p.x3 = x3;
         p.bytes = 2.0 * 8.0 * double(uint64_t(1) << n);
         while (mats.size() % 4)
             mats.push_back(0.f);
@@ -932,23 +937,57 @@ namespace qcore
         std::memcpy(mats.data() + p.desc_offset, desc.data(), desc.size() * sizeof(qsv::StageBlock));
         p.tsub.set_bits = 0;
         p.tsub.nops = m;
-        p.op.kq = kq;
-        p.op.m = m;
-        p.op.x3 = x3 ? 1 : 0;
-        p.op.nblocks = p.nblocks;
         for (int b = 0; b < m; ++b)
-        {
             p.tsub.ops[b] = st.tile[b];
-            p.op.gw[b] = uint64_t(1) << st.tile[b];
-        }
         return p;
     }
 
-    void launch_prepared_stage(hipStream_t s, qsv::c64 *state, int n, PreparedStage &p, const float *mats_dev)
+    qsv::StageTask stage_task(const PreparedStage &p, qsv::c64 *state, uint64_t tile_begin)
     {
-        p.op.R = mats_dev;
-        p.op.blocks = reinterpret_cast<const qsv::StageBlock *>(mats_dev + p.desc_offset);
-        qsv::launch_stage(s, state, n, p.tsub, p.op);
+        if (p.desc_offset > 0xffffffffu)
+            throw std::runtime_error("stage: matrix buffer beyond 2^32 floats");
+        qsv::StageTask t{};
+        t.state = state;
+        t.tile_begin = tile_begin;
+        for (int b = 0; b < p.m; ++b)
+            t.gw[b] = uint64_t(1) << p.tsub.ops[b];
+        t.tsub = p.tsub;
+        t.blocks = uint32_t(p.desc_offset);
+        t.nblocks = p.nblocks;
+        qsv::check_stage_task(t, p.n, p.kq, p.m);
+        return t;
+    }
+
+    size_t append_stage_tasks(std::vector<float> &mats, const std::vector<qsv::StageTask> &tasks)
+    {
+        static_assert(sizeof(qsv::StageTask) % 16 == 0, "StageTask keeps 16-B alignment");
+        if (mats.empty())
+            mats.resize(4, 0.f);
+        while (mats.size() % 4)
+            mats.push_back(0.f);
+        const size_t off = mats.size();
+        mats.resize(off + tasks.size() * sizeof(qsv::StageTask) / sizeof(float));
+        std::memcpy(mats.data() + off, tasks.data(), tasks.size() * sizeof(qsv::StageTask));
+        return off;
+    }
+
+    void attach_stage(PreparedStage &p, qsv::c64 *state, std::vector<float> &mats)
+    {
+        p.task_offset = append_stage_tasks(mats, {stage_task(p, state, 0)});
+    }
+
+    void launch_prepared_stage(hipStream_t s, const PreparedStage &p, const float *mats_dev)
+    {
+        if (p.task_offset == 0)
+            throw std::runtime_error("stage: launched without a task");
+        launch_stage_tasks(s, p.kq, p.m, p.x3, mats_dev, p.task_offset, 1, uint64_t(1) << (p.n - p.m));
+    }
+
+    void launch_stage_tasks(hipStream_t s, int kq, int m, bool x3, const float *mats_dev, size_t task_offset, int ntasks, uint64_t total_tiles)
+    {
+        if (task_offset % 4)
+            throw std::runtime_error("stage: misaligned task array");
+        qsv::launch_stage(s, kq, m, x3, mats_dev, reinterpret_cast<const qsv::StageTask *>(mats_dev + task_offset), ntasks, total_tiles);
     }
 
     PreparedStage128 prepare_stage128(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int n, std::vector<double> &mats, int kq)
@@ -1059,7 +1098,10 @@ namespace qcore
                 if (f64_)
                     it.st128 = prepare_stage128(gates, s, n, mats64, k);
                 else
+                {
                     it.st = prepare_stage(gates, s, k, m, n, mats, x3_);
+                    attach_stage(it.st, state_, mats);
+                }
             }
             else
             {
@@ -1090,7 +1132,7 @@ namespace qcore
                 if (f64_)
                     launch_prepared_stage128(sim_.stream_, state128_, n, it.st128, static_cast<const double *>(rdev_));
                 else
-                    launch_prepared_stage(sim_.stream_, state_, n, it.st, static_cast<const float *>(rdev_));
+                    launch_prepared_stage(sim_.stream_, it.st, static_cast<const float *>(rdev_));
                 sim_.prof_end();
                 blocks_applied_ += 1;
                 gates_applied_ += f64_ ? it.st128.gates : it.st.gates;

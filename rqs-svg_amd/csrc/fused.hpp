@@ -91,15 +91,26 @@ namespace qcore
     {
         int gates = 0;
         int nblocks = 0;
+        int n = 0, kq = 0, m = 0;
+        bool x3 = false;
         double bytes = 0;       // one sweep
         size_t desc_offset = 0; // floats: StageBlock array
+        size_t task_offset = 0; // floats: its one-state task (attach_stage), 0: none yet
         qsv::Subspace tsub{};
-        qsv::StageOp op{};
     };
     // Plans the stage's gates into blocks of at most kq qubits inside the tile and appends
     // their matrices (FP32, or bfloat16 halves when x3) and descriptors to `mats`.
     PreparedStage prepare_stage(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int kq, int m, int n, std::vector<float> &mats, bool x3);
-    void launch_prepared_stage(hipStream_t s, qsv::c64 *state, int n, PreparedStage &p, const float *mats_dev);
+    // The stage's task on `state` as the launch's tiles from tile_begin on (checked).
+    qsv::StageTask stage_task(const PreparedStage &p, qsv::c64 *state, uint64_t tile_begin);
+    // Appends tasks to `mats` 16-B aligned; returns their offset in floats (never 0).
+    size_t append_stage_tasks(std::vector<float> &mats, const std::vector<qsv::StageTask> &tasks);
+    // Appends the stage's one-state task on `state` (a device address) to `mats`.
+    void attach_stage(PreparedStage &p, qsv::c64 *state, std::vector<float> &mats);
+    // Launches an attached stage; mats_dev holds `mats` in device memory.
+    void launch_prepared_stage(hipStream_t s, const PreparedStage &p, const float *mats_dev);
+    // Launches ntasks tasks (stages of one kq, m and x3 on several states) at task_offset.
+    void launch_stage_tasks(hipStream_t s, int kq, int m, bool x3, const float *mats_dev, size_t task_offset, int ntasks, uint64_t total_tiles);
 
     // complex128 stages (tile of 2^12, FP64 blocks of kq = 3 or 4 qubits): matrices and
     // descriptors appended to `mats` as doubles.

@@ -504,9 +504,26 @@ namespace qsv
         // block), multiplies on MFMA as the block kernels do, and writes its rows back in
         // place.  Workgroup barriers (LDS-only) separate the blocks.  The block matrices come
         // from global memory (L2) into registers per block; block descriptors through scalar
-        // loads.
+        // loads.  One launch may serve several states (tasks): the launch's tiles are numbered
+        // across its tasks, and a workgroup finds the task of each tile by binary search over
+        // the tasks' first tile numbers (batched circuits use one launch per stage round).
+        typedef const __attribute__((address_space(4))) StageTask *TaskPtr;
+
+        // expand() reading the subspace through scalar loads
+        __device__ __forceinline__ uint64_t expand_task(uint64_t t, TaskPtr T)
+        {
+            const int nops = T->tsub.nops;
+            for (int k = 0; k < nops; ++k)
+            {
+                const int op = T->tsub.ops[k];
+                const uint64_t lo = t & ((uint64_t(1) << op) - 1);
+                t = lo | ((t ^ lo) << 1);
+            }
+            return t | T->tsub.set_bits;
+        }
+
         template <int KQ, int M, bool X3, int W>
-        __global__ __launch_bounds__(64 * W) void k_stage(c64 *__restrict__ s, uint64_t ntiles, Subspace tsub, StageOp op)
+        __global__ __launch_bounds__(64 * W) void k_stage(const float *__restrict__ mats, const StageTask *__restrict__ tasks, int ntasks, uint64_t total_tiles)
         {
             constexpr int D = 2 << KQ;
             constexpr int MT = D / 32;
@@ -525,17 +542,31 @@ namespace qsv
             const int wave = threadIdx.x >> 6;
             const int c = lane & 31;
             const int h = lane >> 5;
+            const TaskPtr all = as_constant(tasks);
 
-            // global offset of this thread's first amplitude (LDS index 2 * threadIdx.x)
-            uint64_t g_t = 0;
-#pragma unroll
-            for (int b = 1; b < TB && b < M; ++b)
-                if (((2 * threadIdx.x) >> b) & 1)
-                    g_t += op.gw[b];
-
-            for (uint64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x)
+            for (uint64_t g = blockIdx.x; g < total_tiles; g += gridDim.x)
             {
-                const uint64_t base = expand(tl, tsub) + g_t;
+                // the task owning global tile g: the last one with tile_begin <= g
+                int lo = 0, hi = ntasks - 1;
+                while (lo < hi)
+                {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (all[mid].tile_begin <= g)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                const TaskPtr T = all + lo;
+                c64 *__restrict__ s = T->state;
+                const uint64_t tl = g - T->tile_begin;
+
+                // global offset of this thread's first amplitude (LDS index 2 * threadIdx.x)
+                uint64_t g_t = 0;
+#pragma unroll
+                for (int b = 1; b < TB && b < M; ++b)
+                    if (((2 * threadIdx.x) >> b) & 1)
+                        g_t += T->gw[b];
+                const uint64_t base = expand_task(tl, T) + g_t;
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
                 {
@@ -543,20 +574,22 @@ namespace qsv
 #pragma unroll
                     for (int b = TB; b < M; ++b)
                         if (((i * 2 * NTH) >> b) & 1)
-                            gi += op.gw[b];
+                            gi += T->gw[b];
                     const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (base + gi)));
                     *reinterpret_cast<v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * NTH)]) = v;
                 }
                 stage_barrier<W>();
-                for (int bi = 0; bi < op.nblocks; ++bi)
+                const int nblocks = T->nblocks;
+                const __attribute__((address_space(4))) StageBlock *blocks = as_constant(reinterpret_cast<const StageBlock *>(mats + T->blocks));
+                for (int bi = 0; bi < nblocks; ++bi)
                 {
-                    const __attribute__((address_space(4))) StageBlock &B = as_constant(op.blocks)[bi];
+                    const __attribute__((address_space(4))) StageBlock &B = blocks[bi];
                     BlockMatrix<KQ, X3> A;
-                    A.load(op.R + B.matrix, c, h);
+                    A.load(mats + B.matrix, c, h);
 #pragma unroll
-                    for (int g = 0; g < PER_WAVE; ++g)
+                    for (int gr = 0; gr < PER_WAVE; ++gr)
                     {
-                        const int col = (wave * PER_WAVE + g) * 32 + c;
+                        const int col = (wave * PER_WAVE + gr) * 32 + c;
                         int cbase = 0;
 #pragma unroll
                         for (int j = 0; j < M - KQ; ++j)
@@ -611,7 +644,7 @@ namespace qsv
 #pragma unroll
                     for (int b = TB; b < M; ++b)
                         if (((i * 2 * NTH) >> b) & 1)
-                            gi += op.gw[b];
+                            gi += T->gw[b];
                     const v4f v = *reinterpret_cast<const v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * NTH)]);
                     __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(s + (base + gi)));
                 }
@@ -766,44 +799,53 @@ namespace qsv
     }
 
 
-    void launch_stage(hipStream_t st, c64 *state, int n, const Subspace &tsub, const StageOp &op)
+    void check_stage_task(const StageTask &t, int n, int kq, int m)
     {
-        if (op.kq != 4 && op.kq != 5)
+        if (kq != 4 && kq != 5)
             throw std::runtime_error("stage: kq must be 4 or 5");
-        if (op.m < 11 || op.m > 13)
+        if (m < 11 || m > 13)
             throw std::runtime_error("stage: tile bits must be 11, 12 or 13");
-        if (n <= op.m || tsub.nops != op.m)
+        if (n <= m || t.tsub.nops != m || n > kMaxQubits)
             throw std::runtime_error("stage: the tile must be a proper part of the state");
-        for (int k = 0; k < tsub.nops; ++k)
-            if (tsub.ops[k] < 0 || tsub.ops[k] >= n || (k > 0 && tsub.ops[k] <= tsub.ops[k - 1]))
+        for (int k = 0; k < t.tsub.nops; ++k)
+            if (t.tsub.ops[k] < 0 || t.tsub.ops[k] >= n || (k > 0 && t.tsub.ops[k] <= t.tsub.ops[k - 1]))
                 throw std::runtime_error("stage: tile qubits must be ascending and < n");
         for (int b = 0; b < 4; ++b)
-            if (tsub.ops[b] != b)
+            if (t.tsub.ops[b] != b)
                 throw std::runtime_error("stage: tile qubits must include qubits 0..3");
-        for (int b = 0; b < op.m; ++b)
-            if (op.gw[b] != (uint64_t(1) << tsub.ops[b]))
+        for (int b = 0; b < m; ++b)
+            if (t.gw[b] != (uint64_t(1) << t.tsub.ops[b]))
                 throw std::runtime_error("stage: tile weights do not match the tile qubits");
-        if (tsub.set_bits != 0)
+        if (t.tsub.set_bits != 0)
             throw std::runtime_error("stage: tiles take no controls");
-        if (op.nblocks < 1 || op.blocks == nullptr || op.R == nullptr)
-            throw std::runtime_error("stage: no blocks");
-        const uint64_t ntiles = uint64_t(1) << (n - op.m);
-        const uint64_t grid = ntiles < 8192 ? ntiles : 8192;
+        if (t.nblocks < 1 || t.state == nullptr || t.blocks % 4 != 0)
+            throw std::runtime_error("stage: task without blocks or state, or misaligned descriptors");
+    }
+
+    void launch_stage(hipStream_t st, int kq, int m, bool x3, const float *base, const StageTask *tasks, int ntasks, uint64_t total_tiles)
+    {
+        if (kq != 4 && kq != 5)
+            throw std::runtime_error("stage: kq must be 4 or 5");
+        if (m < 11 || m > 13)
+            throw std::runtime_error("stage: tile bits must be 11, 12 or 13");
+        if (base == nullptr || tasks == nullptr || ntasks < 1 || total_tiles < 1)
+            throw std::runtime_error("stage: no tasks");
+        const uint64_t grid = total_tiles < 8192 ? total_tiles : 8192;
         // waves per workgroup by tile size: smaller tiles, fewer waves, more workgroups per CU
-        auto go = [&](auto kern, int waves) { hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), 0, st, state, ntiles, tsub, op); };
+        auto go = [&](auto kern, int waves) { hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), 0, st, base, tasks, ntasks, total_tiles); };
         constexpr int W13 = QSV_STAGE_WAVES_13, W12 = QSV_STAGE_WAVES_12, W11 = QSV_STAGE_WAVES_11;
-        if (op.kq == 5 && op.m == 13)
-            op.x3 ? go(k_stage<5, 13, true, W13>, W13) : go(k_stage<5, 13, false, W13>, W13);
-        else if (op.kq == 5 && op.m == 12)
-            op.x3 ? go(k_stage<5, 12, true, W12>, W12) : go(k_stage<5, 12, false, W12>, W12);
-        else if (op.kq == 5)
-            op.x3 ? go(k_stage<5, 11, true, W11>, W11) : go(k_stage<5, 11, false, W11>, W11);
-        else if (op.m == 13)
-            op.x3 ? go(k_stage<4, 13, true, W13>, W13) : go(k_stage<4, 13, false, W13>, W13);
-        else if (op.m == 12)
-            op.x3 ? go(k_stage<4, 12, true, W12>, W12) : go(k_stage<4, 12, false, W12>, W12);
+        if (kq == 5 && m == 13)
+            x3 ? go(k_stage<5, 13, true, W13>, W13) : go(k_stage<5, 13, false, W13>, W13);
+        else if (kq == 5 && m == 12)
+            x3 ? go(k_stage<5, 12, true, W12>, W12) : go(k_stage<5, 12, false, W12>, W12);
+        else if (kq == 5)
+            x3 ? go(k_stage<5, 11, true, W11>, W11) : go(k_stage<5, 11, false, W11>, W11);
+        else if (m == 13)
+            x3 ? go(k_stage<4, 13, true, W13>, W13) : go(k_stage<4, 13, false, W13>, W13);
+        else if (m == 12)
+            x3 ? go(k_stage<4, 12, true, W12>, W12) : go(k_stage<4, 12, false, W12>, W12);
         else
-            op.x3 ? go(k_stage<4, 11, true, W11>, W11) : go(k_stage<4, 11, false, W11>, W11);
+            x3 ? go(k_stage<4, 11, true, W11>, W11) : go(k_stage<4, 11, false, W11>, W11);
         check(hipGetLastError(), "stage kernel");
     }
 

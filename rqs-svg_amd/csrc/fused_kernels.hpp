@@ -46,20 +46,26 @@ namespace qsv
     struct StageBlock
     {
         int w[16];
-        uint32_t matrix; // offset in floats from StageOp::R (complex128 stages: doubles from M)
+        uint32_t matrix; // offset in floats from the launch's matrix base (complex128 stages: doubles from M)
         uint32_t pad[3];
     };
-    struct StageOp
+    // One state's part of a stage launch (device memory, read by the kernel): its tiles are
+    // the launch's tiles [tile_begin, tile_begin + 2^(n - m)), its nblocks StageBlock
+    // descriptors start `blocks` floats after the launch's matrix base (a multiple of 4).
+    struct alignas(16) StageTask
     {
-        int kq;
-        int m;
-        int x3;
+        c64 *state;
+        uint64_t tile_begin;
+        uint64_t gw[16]; // global stride of tile bit b: 2^tile qubit b
+        Subspace tsub;   // the tile qubits, ascending (tiles enumerate the other qubits)
+        uint32_t blocks;
         int nblocks;
-        uint64_t gw[16];
-        const StageBlock *blocks;
-        const float *R;
     };
-    void launch_stage(hipStream_t s, c64 *state, int n, const Subspace &tsub, const StageOp &op);
+    // Host-side check of one task before upload (the kernel trusts the tasks).
+    void check_stage_task(const StageTask &t, int n, int kq, int m);
+    // tasks: device array of ntasks tasks with ascending tile_begin, the first 0, all checked
+    // with check_stage_task; total_tiles = the tiles of all tasks.
+    void launch_stage(hipStream_t s, int kq, int m, bool x3, const float *base, const StageTask *tasks, int ntasks, uint64_t total_tiles);
 
     // complex128 stage: tile of 2^12 amplitudes, blocks of kq = 3 or 4 qubits in FP64;
     // blocks[i].matrix is the offset in doubles of the 2^kq x 2^kq complex matrix (row-major,

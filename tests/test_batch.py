@@ -65,6 +65,37 @@ def test_gpu_batch_vs_oracle(gpu, oracle):
 
 
 @pytest.mark.gpu
+def test_gpu_batch_stage_rounds(gpu, oracle):
+    # circuits of 15-18 qubits with different depths: with 2^11-amplitude tiles all of them
+    # run in stage rounds (one launch per round across circuits, later rounds with fewer
+    # circuits); a gate-free 16-qubit circuit and small LDS circuits ride along
+    work = circuits.variational_batch(10, seed=21, nmin=15, nmax=18, dmin=1, dmax=7)
+    work += [(16, []), (6, [("h", 0, (), -1), ("x", 5, (), 0)])]
+    ref = pyqcs.Batch(oracle)
+    for n, g in work:
+        ref.add(n, g)
+    want, _ = ref.run()
+    results = {}
+    for tuning in [(14, 512, 5, 2, 11), (14, 512, 4, 4, 11), (14, 512, 5, 1, 0)]:
+        b = pyqcs.Batch(gpu)
+        for n, g in work:
+            b.add(n, g)
+        b.set_tuning(*tuning)
+        got, secs = b.run()
+        direct, _ = b.run_direct()
+        err = max(np.max(np.abs(x - y)) for x, y in zip(got, want))
+        print(f"tuning {tuning}: max |d<Z>| {err:.2e} device {secs * 1e3:.2f} ms nodes {b.info()['graph_nodes']}", flush=True)
+        assert err < 2e-5, tuning
+        assert all(np.array_equal(x, y) for x, y in zip(got, direct))
+        assert np.array_equal(got[-2], np.ones(16))  # |0...0>: <Z_q> = 1 exactly
+        results[tuning] = got
+        b.close()
+    # rounds and per-block launches agree to complex64 rounding
+    a, c = results[(14, 512, 5, 2, 11)], results[(14, 512, 5, 1, 0)]
+    assert max(np.max(np.abs(x - y)) for x, y in zip(a, c)) < 2e-5
+
+
+@pytest.mark.gpu
 def test_gpu_batch_autotune(gpu):
     work = circuits.variational_batch(24, seed=8, nmin=10, nmax=18, dmin=2, dmax=5)
     b = pyqcs.Batch(gpu)
