@@ -162,7 +162,7 @@ def fused_leg(n, depth, k, shots, variants=FUSED_VARIANTS):
     return out
 
 
-def batch_leg(count, seed=20250404, direct=False):
+def batch_leg(count, seed=20250404, direct=False, tuning=None):
     """512 variational circuits (12-22 qubits, 4-16 layers of RY/RZ + CNOT ladder): one
     autotuned hipGraph launch (complex64 states, FP64 <Z> reductions) against the per-circuit
     plain-method loop in C++ (complex128, one kernel per gate, one simulator per qubit
@@ -174,8 +174,15 @@ def batch_leg(count, seed=20250404, direct=False):
     for n, g in work:
         b.add(n, g)
     t0 = time.perf_counter()
-    # the autotuner times graph replays; a direct-launch (profiler) run keeps the defaults
-    tuning = b.autotune() if not direct else {"default": True}
+    # the autotuner times graph replays; a direct-launch (profiler) run takes the tuning
+    # given on the command line, or the defaults
+    if tuning is not None:
+        b.set_tuning(*tuning)
+        tuning = dict(zip(("lds_max_qubits", "block_size", "block_qubits", "streams", "stage_bits"), tuning))
+    elif direct:
+        tuning = {"default": True}
+    else:
+        tuning = b.autotune()
     t_tune = time.perf_counter() - t0
     b.run()
     runs = []
@@ -301,6 +308,8 @@ def main():
     ap.add_argument("--fused-k", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-circuits", type=int, default=512)
+    ap.add_argument("--batch-tuning", type=lambda s: tuple(int(v) for v in s.split(",")), default=None,
+                    help="lds_max_qubits,block_size,block_qubits,streams,stage_bits instead of the autotuner")
     ap.add_argument("--batch-no-graph", action="store_true",
                     help="issue the batch's launches directly instead of one graph launch "
                          "(same kernels; for profilers that do not follow graph launches)")
@@ -407,7 +416,7 @@ def main():
     if not args.no_fused:
         result.update(fused_leg(args.fused_qubits, args.fused_depth, args.fused_k, args.shots))
     if not args.no_batch:
-        result["batch"] = batch_leg(args.batch_circuits, direct=args.batch_no_graph)
+        result["batch"] = batch_leg(args.batch_circuits, direct=args.batch_no_graph, tuning=args.batch_tuning)
     if not args.no_planner:
         result["planner"] = planner_leg(args.planner_qubits, (26, 28, 30, 32), args.planner_record,
                                         args.planner_depth, args.planner_k)

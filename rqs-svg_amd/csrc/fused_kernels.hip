@@ -44,11 +44,18 @@ namespace qsv
 #ifndef QSV_STAGE_WAVES_11
 #define QSV_STAGE_WAVES_11 1
 #endif
+#ifndef QSV_STAGE_GROUPS_KQ5
+#define QSV_STAGE_GROUPS_KQ5 2
+#endif
+#ifndef QSV_STAGE_GROUPS_KQ4
+#define QSV_STAGE_GROUPS_KQ4 1
+#endif
 #ifndef QSV_BLOCK_PREFETCH
 #define QSV_BLOCK_PREFETCH 1
 #endif
 
         typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+        typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
         // The block matrix R as MFMA A-operands, held in registers for the whole launch, and
         // its product with a wave's 32 columns.  Lane (h, c) supplies rows 32*mt + c of R and
@@ -88,6 +95,18 @@ namespace qsv
                     for (int mt = 0; mt < MT; ++mt)
                         d[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt][st], x[st], d[mt], 0, 0, 0);
             }
+            // G column groups at once: G * MT independent accumulation chains
+            template <int G>
+            __device__ __forceinline__ void apply_n(const float (&x)[G][H], v16f (&d)[G][MT]) const
+            {
+#pragma unroll
+                for (int st = 0; st < H; ++st)
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                        for (int g = 0; g < G; ++g)
+                            d[g][mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt][st], x[g][st], d[g][mt], 0, 0, 0);
+            }
         };
 
         // BF16x3 split precision on v_mfma_f32_32x32x16_bf16: R = Rh + Rl and x = xh + xl
@@ -115,20 +134,31 @@ namespace qsv
                         lo[mt][st] = rl[v];
                     }
             }
+            // x[8*st .. 8*st + 7] as bfloat16 parts xh + xl
+            static __device__ __forceinline__ void split(const float *x, bf16x8 &xh, bf16x8 &xl)
+            {
+#pragma unroll
+                for (int j = 0; j < 8; j += 2)
+                {
+                    // two values per conversion instruction
+                    const v2f v = {x[j], x[j + 1]};
+                    const bf16x2 hb = __builtin_convertvector(v, bf16x2);
+                    const v2f hf = __builtin_convertvector(hb, v2f);
+                    const v2f r = {v.x - hf.x, v.y - hf.y};
+                    const bf16x2 lb = __builtin_convertvector(r, bf16x2);
+                    xh[j] = hb.x;
+                    xh[j + 1] = hb.y;
+                    xl[j] = lb.x;
+                    xl[j + 1] = lb.y;
+                }
+            }
             __device__ __forceinline__ void apply(const float (&x)[H], v16f (&d)[MT]) const
             {
 #pragma unroll
                 for (int st = 0; st < ST; ++st)
                 {
                     bf16x8 xh, xl;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j)
-                    {
-                        const float v = x[8 * st + j];
-                        const __bf16 t = (__bf16)v;
-                        xh[j] = t;
-                        xl[j] = (__bf16)(v - (float)t);
-                    }
+                    split(x + 8 * st, xh, xl);
 #pragma unroll
                     for (int mt = 0; mt < MT; ++mt)
                     {
@@ -136,6 +166,34 @@ namespace qsv
                         d[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[mt][st], xl, d[mt], 0, 0, 0);
                         d[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[mt][st], xh, d[mt], 0, 0, 0);
                     }
+                }
+            }
+            // G column groups at once: G * MT independent accumulation chains
+            template <int G>
+            __device__ __forceinline__ void apply_n(const float (&x)[G][H], v16f (&d)[G][MT]) const
+            {
+#pragma unroll
+                for (int st = 0; st < ST; ++st)
+                {
+                    bf16x8 xh[G], xl[G];
+#pragma unroll
+                    for (int g = 0; g < G; ++g)
+                        split(x[g] + 8 * st, xh[g], xl[g]);
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                        for (int g = 0; g < G; ++g)
+                            d[g][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo[mt][st], xh[g], d[g][mt], 0, 0, 0);
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                        for (int g = 0; g < G; ++g)
+                            d[g][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[mt][st], xl[g], d[g][mt], 0, 0, 0);
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                        for (int g = 0; g < G; ++g)
+                            d[g][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[mt][st], xh[g], d[g][mt], 0, 0, 0);
                 }
             }
         };
@@ -211,6 +269,11 @@ namespace qsv
         __device__ __forceinline__ int swz64(int i) { return i ^ ((((i >> 5) ^ (i >> 9)) & 15) << 1); }
         // complex128 (16-B slots): XOR bits 0..3 with bits 4..7 and 8..11
         __device__ __forceinline__ int swz128(int i) { return i ^ (((i >> 4) ^ (i >> 8)) & 15); }
+        // swz64 is linear over GF(2) (it XORs bits 1..4 with bits of i it leaves unchanged), so
+        // for indices built from disjoint bit sets, swz64(a | b) = swz64(a) ^ swz64(b): the
+        // stage kernel combines per-lane and uniform parts of an LDS address with one XOR.
+        // Byte offset of swz64(i) for 8-B slots.
+        __device__ __forceinline__ uint32_t swz64b(uint32_t i) { return uint32_t(swz64(int(i))) << 3; }
 
         // Read-only data of a launch (block descriptors, complex128 matrices) read through the
         // constant address space: uniform loads from it become scalar loads, which neither
@@ -536,16 +599,20 @@ namespace qsv
             constexpr int LOADS = NT / (2 * NTH);
             constexpr int TB = 1 + __builtin_ctz(NTH); // tile bits set by the thread index
             static_assert(PER_WAVE >= 1 && LOADS >= 1, "tile too small");
+            // column groups per pass (each adds MT accumulators and H floats per lane)
+            constexpr int GMAX = KQ == 5 ? QSV_STAGE_GROUPS_KQ5 : QSV_STAGE_GROUPS_KQ4;
+            constexpr int G = PER_WAVE < GMAX ? PER_WAVE : GMAX;
+            static_assert(PER_WAVE % G == 0, "groups per pass must divide the groups per wave");
             __shared__ __attribute__((aligned(16))) v2f tile[NT];
 
             const int lane = threadIdx.x & 63;
-            const int wave = threadIdx.x >> 6;
+            const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
             const int c = lane & 31;
             const int h = lane >> 5;
             const TaskPtr all = as_constant(tasks);
 
-            for (uint64_t g = blockIdx.x; g < total_tiles; g += gridDim.x)
-            {
+            // tile g: its task and this thread's first global amplitude (LDS index 2 * threadIdx.x)
+            auto locate = [&](uint64_t g, TaskPtr &T, uint64_t &base) {
                 // the task owning global tile g: the last one with tile_begin <= g
                 int lo = 0, hi = ntasks - 1;
                 while (lo < hi)
@@ -556,27 +623,43 @@ namespace qsv
                     else
                         hi = mid - 1;
                 }
-                const TaskPtr T = all + lo;
-                c64 *__restrict__ s = T->state;
-                const uint64_t tl = g - T->tile_begin;
-
-                // global offset of this thread's first amplitude (LDS index 2 * threadIdx.x)
+                T = all + lo;
                 uint64_t g_t = 0;
 #pragma unroll
                 for (int b = 1; b < TB && b < M; ++b)
-                    if (((2 * threadIdx.x) >> b) & 1)
-                        g_t += T->gw[b];
-                const uint64_t base = expand_task(tl, T) + g_t;
+                    g_t += uint64_t(-int64_t(((2 * threadIdx.x) >> b) & 1)) & T->gw[b];
+                base = expand_task(g - T->tile_begin, T) + g_t;
+            };
+            // global offset of the thread's i-th v4f of a tile (LDS index 2 * threadIdx.x + i * 2 * NTH)
+            auto offset = [&](TaskPtr T, int i) {
+                uint64_t gi = 0;
+#pragma unroll
+                for (int b = TB; b < M; ++b)
+                    if (((i * 2 * NTH) >> b) & 1)
+                        gi += T->gw[b];
+                return gi;
+            };
+            auto fetch = [&](TaskPtr T, uint64_t base, v4f *v) {
+                c64 *__restrict__ s = T->state;
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
+                    v[i] = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (base + offset(T, i))));
+            };
+
+            // one tile per workgroup (a loop only for launches beyond the grid limit): the
+            // load -> blocks -> store phases of different workgroups overlap on a CU, and no
+            // wave waits for its own stores before loading the next tile
+            for (uint64_t g = blockIdx.x; g < total_tiles; g += gridDim.x)
+            {
+                TaskPtr T;
+                uint64_t base;
+                locate(g, T, base);
                 {
-                    uint64_t gi = 0;
+                    v4f v[LOADS];
+                    fetch(T, base, v);
 #pragma unroll
-                    for (int b = TB; b < M; ++b)
-                        if (((i * 2 * NTH) >> b) & 1)
-                            gi += T->gw[b];
-                    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (base + gi)));
-                    *reinterpret_cast<v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * NTH)]) = v;
+                    for (int i = 0; i < LOADS; ++i)
+                        *reinterpret_cast<v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * NTH)]) = v[i];
                 }
                 stage_barrier<W>();
                 const int nblocks = T->nblocks;
@@ -586,69 +669,100 @@ namespace qsv
                     const __attribute__((address_space(4))) StageBlock &B = blocks[bi];
                     BlockMatrix<KQ, X3> A;
                     A.load(mats + B.matrix, c, h);
+                    // all of the matrix loads are issued here, together (one L2 round trip),
+                    // not moved next to their first uses
+                    __builtin_amdgcn_sched_barrier(0);
+                    // LDS byte offsets (swizzled) of the block bits (amplitude index bits) and
+                    // of the column bits; uniform, so kept in scalar registers
+                    uint32_t tb[KQ], cb[M - KQ];
 #pragma unroll
-                    for (int gr = 0; gr < PER_WAVE; ++gr)
+                    for (int j = 0; j < KQ; ++j)
+                        tb[j] = swz64b(uint32_t(B.w[j]));
+#pragma unroll
+                    for (int j = 0; j < M - KQ; ++j)
+                        cb[j] = swz64b(uint32_t(B.w[KQ + j]));
+                    // amplitude a of a column: XOR of tb[j] over the set bits j of a
+                    auto amp_off = [&](int a) {
+                        uint32_t o = 0;
+#pragma unroll
+                        for (int j = 0; j < KQ; ++j)
+                            if ((a >> j) & 1)
+                                o ^= tb[j];
+                        return o;
+                    };
+                    // the lane's column bits 0..4 (c) and its amplitude half h: gathers read
+                    // amplitudes h*AMPS + e, scatters write 16*mt + 4*q + 2*h (+1)
+                    uint32_t lanep = 0;
+#pragma unroll
+                    for (int j = 0; j < 5; ++j)
+                        lanep ^= uint32_t(-int((c >> j) & 1)) & cb[j];
+                    const uint32_t hmask = uint32_t(-h);
+                    const uint32_t g_lane = lanep ^ (hmask & amp_off(AMPS));
+                    const uint32_t s_lane = lanep ^ (hmask & amp_off(2));
+                    const char *tbytes = reinterpret_cast<const char *>(tile);
+                    char *wbytes = reinterpret_cast<char *>(tile);
+                    // G column groups per pass: their gathers are in flight together and their
+                    // products form independent MFMA chains
+#pragma unroll
+                    for (int gr0 = 0; gr0 < PER_WAVE; gr0 += G)
                     {
-                        const int col = (wave * PER_WAVE + gr) * 32 + c;
-                        int cbase = 0;
+                        uint32_t U[G];
+                        float x[G][H];
 #pragma unroll
-                        for (int j = 0; j < M - KQ; ++j)
-                            if ((col >> j) & 1)
-                                cbase += B.w[KQ + j];
-                        float x[H];
-#pragma unroll
-                        for (int e = 0; e < AMPS; ++e)
+                        for (int g = 0; g < G; ++g)
                         {
-                            const int amp = h * AMPS + e;
-                            int off = 0;
+                            // column bits 5.. of this group (uniform)
+                            const int grp = wave * PER_WAVE + gr0 + g;
+                            U[g] = 0;
 #pragma unroll
-                            for (int j = 0; j < KQ; ++j)
-                                if ((amp >> j) & 1)
-                                    off += B.w[j];
-                            const v2f a = tile[swz64(cbase + off)];
-                            x[2 * e] = a.x;
-                            x[2 * e + 1] = a.y;
+                            for (int j = 5; j < M - KQ; ++j)
+                                if ((grp >> (j - 5)) & 1)
+                                    U[g] ^= cb[j];
+                            const uint32_t gbase = g_lane ^ U[g];
+#pragma unroll
+                            for (int e = 0; e < AMPS; ++e)
+                            {
+                                const v2f a = *reinterpret_cast<const v2f *>(tbytes + (gbase ^ amp_off(e)));
+                                x[g][2 * e] = a.x;
+                                x[g][2 * e + 1] = a.y;
+                            }
                         }
-                        v16f d[MT];
+                        v16f d[G][MT];
 #pragma unroll
-                        for (int mt = 0; mt < MT; ++mt)
-                            d[mt] = v16f{0.f};
-                        A.apply(x, d);
+                        for (int g = 0; g < G; ++g)
+#pragma unroll
+                            for (int mt = 0; mt < MT; ++mt)
+                                d[g][mt] = v16f{0.f};
+                        A.template apply_n<G>(x, d);
                         // accumulator register r of tile mt holds real row 32*mt + 8*(r/4) + 4*h + r%4,
                         // i.e. amplitudes 16*mt + 4*(r/4) + 2*h and the next one
 #pragma unroll
-                        for (int mt = 0; mt < MT; ++mt)
+                        for (int g = 0; g < G; ++g)
+                        {
+                            const uint32_t sbase = s_lane ^ U[g];
 #pragma unroll
-                            for (int q = 0; q < 4; ++q)
-                            {
-                                const int e0 = 16 * mt + 4 * q + 2 * h;
-                                int o0 = 0, o1 = 0;
+                            for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-                                for (int j = 0; j < KQ; ++j)
+                                for (int q = 0; q < 4; ++q)
                                 {
-                                    if ((e0 >> j) & 1)
-                                        o0 += B.w[j];
-                                    if (((e0 + 1) >> j) & 1)
-                                        o1 += B.w[j];
+                                    const uint32_t o0 = sbase ^ amp_off(16 * mt + 4 * q);
+                                    *reinterpret_cast<v2f *>(wbytes + o0) = v2f{d[g][mt][4 * q + 0], d[g][mt][4 * q + 1]};
+                                    *reinterpret_cast<v2f *>(wbytes + (o0 ^ tb[0])) = v2f{d[g][mt][4 * q + 2], d[g][mt][4 * q + 3]};
                                 }
-                                tile[swz64(cbase + o0)] = v2f{d[mt][4 * q + 0], d[mt][4 * q + 1]};
-                                tile[swz64(cbase + o1)] = v2f{d[mt][4 * q + 2], d[mt][4 * q + 3]};
-                            }
+                        }
                     }
                     stage_barrier<W>();
                 }
+                c64 *__restrict__ s = T->state;
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
                 {
-                    uint64_t gi = 0;
-#pragma unroll
-                    for (int b = TB; b < M; ++b)
-                        if (((i * 2 * NTH) >> b) & 1)
-                            gi += T->gw[b];
                     const v4f v = *reinterpret_cast<const v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * NTH)]);
-                    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(s + (base + gi)));
+                    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(s + (base + offset(T, i))));
                 }
-                stage_barrier<W>();
+                // the LDS tile is reused only when this workgroup takes another tile
+                if (g + gridDim.x < total_tiles)
+                    stage_barrier<W>();
             }
         }
 
@@ -830,7 +944,9 @@ namespace qsv
             throw std::runtime_error("stage: tile bits must be 11, 12 or 13");
         if (base == nullptr || tasks == nullptr || ntasks < 1 || total_tiles < 1)
             throw std::runtime_error("stage: no tasks");
-        const uint64_t grid = total_tiles < 8192 ? total_tiles : 8192;
+        // one workgroup per tile (measured on 32-qubit brick stages: 19.5 ms per 2^13-tile
+        // stage against 23.4 ms with 8192 workgroups looping over the tiles)
+        const uint64_t grid = total_tiles < 0x7fffffffu ? total_tiles : 0x7fffffffu;
         // waves per workgroup by tile size: smaller tiles, fewer waves, more workgroups per CU
         auto go = [&](auto kern, int waves) { hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), 0, st, base, tasks, ntasks, total_tiles); };
         constexpr int W13 = QSV_STAGE_WAVES_13, W12 = QSV_STAGE_WAVES_12, W11 = QSV_STAGE_WAVES_11;
