@@ -32,20 +32,25 @@ namespace qsv
 
         constexpr int kWaves = 4;
         constexpr int kThreads = 64 * kWaves;
+// waves per stage workgroup by tile bits and arithmetic (FP32 or BF16x3 MFMA), measured on
+// 32-qubit brick circuits
 #ifndef QSV_STAGE_WAVES_13
 #define QSV_STAGE_WAVES_13 4
 #endif
-// waves per stage workgroup by tile bits (measured on 32-qubit brick circuits: 2^13 tiles
-// with 4 waves 23.2 ms per stage; 2^12 with 2 waves 21.7 ms (4 waves: 27.5 ms); 2^11 with
-// 1 wave 19.6 ms (2 waves: 25.1 ms))
+#ifndef QSV_STAGE_WAVES_13_X3
+#define QSV_STAGE_WAVES_13_X3 2
+#endif
 #ifndef QSV_STAGE_WAVES_12
 #define QSV_STAGE_WAVES_12 2
+#endif
+#ifndef QSV_STAGE_WAVES_12_X3
+#define QSV_STAGE_WAVES_12_X3 1
 #endif
 #ifndef QSV_STAGE_WAVES_11
 #define QSV_STAGE_WAVES_11 1
 #endif
 #ifndef QSV_STAGE_GROUPS_KQ5
-#define QSV_STAGE_GROUPS_KQ5 2
+#define QSV_STAGE_GROUPS_KQ5 1
 #endif
 #ifndef QSV_STAGE_GROUPS_KQ4
 #define QSV_STAGE_GROUPS_KQ4 1
@@ -554,6 +559,116 @@ namespace qsv
                 lds_barrier();
         }
 
+        // One block of a stage on the tile in LDS: CWAVES waves (index `wave`) share the
+        // 2^(M-KQ) columns, 32 per MFMA column group.  The caller separates blocks with a
+        // workgroup barrier.
+        template <int KQ, int M, bool X3, int CWAVES>
+        __device__ __forceinline__ void stage_block_with(v2f *tile, const BlockMatrix<KQ, X3> &A, const __attribute__((address_space(4))) StageBlock &B, int wave, int c, int h)
+        {
+            constexpr int D = 2 << KQ;
+            constexpr int MT = D / 32;
+            constexpr int H = D / 2;
+            constexpr int AMPS = H / 2;
+            constexpr int NT = 1 << M;
+            constexpr int COLS = NT >> KQ;
+            constexpr int PER_WAVE = COLS / 32 / CWAVES;
+            static_assert(PER_WAVE >= 1, "tile too small");
+            // column groups per pass (each adds MT accumulators and H floats per lane)
+            constexpr int GMAX = KQ == 5 ? QSV_STAGE_GROUPS_KQ5 : QSV_STAGE_GROUPS_KQ4;
+            constexpr int G = PER_WAVE < GMAX ? PER_WAVE : GMAX;
+            static_assert(PER_WAVE % G == 0, "groups per pass must divide the groups per wave");
+            // LDS byte offsets (swizzled) of the block bits (amplitude index bits) and
+            // of the column bits; uniform, so kept in scalar registers
+            uint32_t tb[KQ], cb[M - KQ];
+#pragma unroll
+            for (int j = 0; j < KQ; ++j)
+                tb[j] = swz64b(uint32_t(B.w[j]));
+#pragma unroll
+            for (int j = 0; j < M - KQ; ++j)
+                cb[j] = swz64b(uint32_t(B.w[KQ + j]));
+            // amplitude a of a column: XOR of tb[j] over the set bits j of a
+            auto amp_off = [&](int a) {
+                uint32_t o = 0;
+#pragma unroll
+                for (int j = 0; j < KQ; ++j)
+                    if ((a >> j) & 1)
+                        o ^= tb[j];
+                return o;
+            };
+            // the lane's column bits 0..4 (c) and its amplitude half h: gathers read
+            // amplitudes h*AMPS + e, scatters write 16*mt + 4*q + 2*h (+1)
+            uint32_t lanep = 0;
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                lanep ^= uint32_t(-int((c >> j) & 1)) & cb[j];
+            const uint32_t hmask = uint32_t(-h);
+            const uint32_t g_lane = lanep ^ (hmask & amp_off(AMPS));
+            const uint32_t s_lane = lanep ^ (hmask & amp_off(2));
+            const char *tbytes = reinterpret_cast<const char *>(tile);
+            char *wbytes = reinterpret_cast<char *>(tile);
+            // G column groups per pass: their gathers are in flight together and their
+            // products form independent MFMA chains
+#pragma unroll
+            for (int gr0 = 0; gr0 < PER_WAVE; gr0 += G)
+            {
+                uint32_t U[G];
+                float x[G][H];
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                {
+                    // column bits 5.. of this group (uniform)
+                    const int grp = wave * PER_WAVE + gr0 + g;
+                    U[g] = 0;
+#pragma unroll
+                    for (int j = 5; j < M - KQ; ++j)
+                        if ((grp >> (j - 5)) & 1)
+                            U[g] ^= cb[j];
+                    const uint32_t gbase = g_lane ^ U[g];
+#pragma unroll
+                    for (int e = 0; e < AMPS; ++e)
+                    {
+                        const v2f a = *reinterpret_cast<const v2f *>(tbytes + (gbase ^ amp_off(e)));
+                        x[g][2 * e] = a.x;
+                        x[g][2 * e + 1] = a.y;
+                    }
+                }
+                v16f d[G][MT];
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                        d[g][mt] = v16f{0.f};
+                A.template apply_n<G>(x, d);
+                // accumulator register r of tile mt holds real row 32*mt + 8*(r/4) + 4*h + r%4,
+                // i.e. amplitudes 16*mt + 4*(r/4) + 2*h and the next one
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                {
+                    const uint32_t sbase = s_lane ^ U[g];
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                        {
+                            const uint32_t o0 = sbase ^ amp_off(16 * mt + 4 * q);
+                            *reinterpret_cast<v2f *>(wbytes + o0) = v2f{d[g][mt][4 * q + 0], d[g][mt][4 * q + 1]};
+                            *reinterpret_cast<v2f *>(wbytes + (o0 ^ tb[0])) = v2f{d[g][mt][4 * q + 2], d[g][mt][4 * q + 3]};
+                        }
+                }
+            }
+        }
+
+        template <int KQ, int M, bool X3, int CWAVES>
+        __device__ __forceinline__ void stage_block(v2f *tile, const float *__restrict__ mats, const __attribute__((address_space(4))) StageBlock &B, int wave, int c, int h)
+        {
+            BlockMatrix<KQ, X3> A;
+            A.load(mats + B.matrix, c, h);
+            // all of the matrix loads are issued here, together (one L2 round trip),
+            // not moved next to their first uses
+            __builtin_amdgcn_sched_barrier(0);
+            stage_block_with<KQ, M, X3, CWAVES>(tile, A, B, wave, c, h);
+        }
+
         // Stage kernel: one workgroup holds a tile of 2^M amplitudes in LDS -- the
         // amplitudes whose index bits outside the stage's tile qubits T are fixed -- and
         // applies every block of the stage to it before writing it back, so a stage costs
@@ -585,35 +700,21 @@ namespace qsv
             return t | T->tsub.set_bits;
         }
 
-        template <int KQ, int M, bool X3, int W>
-        __global__ __launch_bounds__(64 * W) void k_stage(const float *__restrict__ mats, const StageTask *__restrict__ tasks, int ntasks, uint64_t total_tiles)
+        // Moves tiles between HBM and LDS: NTHM threads (index tm) move 16 B (2 amplitudes)
+        // per v4f; the thread's i-th v4f is LDS index 2*tm + i*2*NTHM, whose tile bits 1.. are
+        // set by tm (bits 1 .. TB-1) and i (bits TB ..).
+        template <int M, int NTHM>
+        struct TileIO
         {
-            constexpr int D = 2 << KQ;
-            constexpr int MT = D / 32;
-            constexpr int H = D / 2;
-            constexpr int AMPS = H / 2;
-            constexpr int NT = 1 << M;
-            constexpr int COLS = NT >> KQ;
-            constexpr int NTH = 64 * W; // threads per workgroup
-            constexpr int PER_WAVE = COLS / 32 / W;
-            constexpr int LOADS = NT / (2 * NTH);
-            constexpr int TB = 1 + __builtin_ctz(NTH); // tile bits set by the thread index
-            static_assert(PER_WAVE >= 1 && LOADS >= 1, "tile too small");
-            // column groups per pass (each adds MT accumulators and H floats per lane)
-            constexpr int GMAX = KQ == 5 ? QSV_STAGE_GROUPS_KQ5 : QSV_STAGE_GROUPS_KQ4;
-            constexpr int G = PER_WAVE < GMAX ? PER_WAVE : GMAX;
-            static_assert(PER_WAVE % G == 0, "groups per pass must divide the groups per wave");
-            __shared__ __attribute__((aligned(16))) v2f tile[NT];
-
-            const int lane = threadIdx.x & 63;
-            const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-            const int c = lane & 31;
-            const int h = lane >> 5;
-            const TaskPtr all = as_constant(tasks);
-
-            // tile g: its task and this thread's first global amplitude (LDS index 2 * threadIdx.x)
-            auto locate = [&](uint64_t g, TaskPtr &T, uint64_t &base) {
-                // the task owning global tile g: the last one with tile_begin <= g
+            static constexpr int LOADS = (1 << M) / (2 * NTHM);
+            static constexpr int TB = 1 + __builtin_ctz(NTHM);
+            static_assert(LOADS >= 1, "tile too small");
+            TaskPtr all;
+            int ntasks;
+            int tm;
+            // the task owning global tile g (the last one with tile_begin <= g)
+            __device__ __forceinline__ TaskPtr task(uint64_t g) const
+            {
                 int lo = 0, hi = ntasks - 1;
                 while (lo < hi)
                 {
@@ -623,28 +724,61 @@ namespace qsv
                     else
                         hi = mid - 1;
                 }
-                T = all + lo;
+                return all + lo;
+            }
+            // tile g: its task and this thread's first global amplitude
+            __device__ __forceinline__ void locate(uint64_t g, TaskPtr &T, uint64_t &base) const
+            {
+                T = task(g);
                 uint64_t g_t = 0;
 #pragma unroll
                 for (int b = 1; b < TB && b < M; ++b)
-                    g_t += uint64_t(-int64_t(((2 * threadIdx.x) >> b) & 1)) & T->gw[b];
+                    g_t += uint64_t(-int64_t(((2 * tm) >> b) & 1)) & T->gw[b];
                 base = expand_task(g - T->tile_begin, T) + g_t;
-            };
-            // global offset of the thread's i-th v4f of a tile (LDS index 2 * threadIdx.x + i * 2 * NTH)
-            auto offset = [&](TaskPtr T, int i) {
+            }
+            __device__ __forceinline__ uint64_t offset(TaskPtr T, int i) const
+            {
                 uint64_t gi = 0;
 #pragma unroll
                 for (int b = TB; b < M; ++b)
-                    if (((i * 2 * NTH) >> b) & 1)
+                    if (((i * 2 * NTHM) >> b) & 1)
                         gi += T->gw[b];
                 return gi;
-            };
-            auto fetch = [&](TaskPtr T, uint64_t base, v4f *v) {
-                c64 *__restrict__ s = T->state;
+            }
+            __device__ __forceinline__ void fetch(TaskPtr T, uint64_t base, v4f *v) const
+            {
+                const c64 *__restrict__ s = T->state;
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
                     v[i] = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (base + offset(T, i))));
-            };
+            }
+            __device__ __forceinline__ void put(v2f *tile, const v4f *v) const
+            {
+#pragma unroll
+                for (int i = 0; i < LOADS; ++i)
+                    *reinterpret_cast<v4f *>(&tile[swz64(2 * tm + i * 2 * NTHM)]) = v[i];
+            }
+            __device__ __forceinline__ void store(const v2f *tile, TaskPtr T, uint64_t base) const
+            {
+                c64 *__restrict__ s = T->state;
+#pragma unroll
+                for (int i = 0; i < LOADS; ++i)
+                {
+                    const v4f v = *reinterpret_cast<const v4f *>(&tile[swz64(2 * tm + i * 2 * NTHM)]);
+                    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(s + (base + offset(T, i))));
+                }
+            }
+        };
+
+        template <int KQ, int M, bool X3, int W>
+        __global__ __launch_bounds__(64 * W) void k_stage(const float *__restrict__ mats, const StageTask *__restrict__ tasks, int ntasks, uint64_t total_tiles)
+        {
+            __shared__ __attribute__((aligned(16))) v2f tile[1 << M];
+            const int lane = threadIdx.x & 63;
+            const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+            const int c = lane & 31;
+            const int h = lane >> 5;
+            const TileIO<M, 64 * W> io{as_constant(tasks), ntasks, int(threadIdx.x)};
 
             // one tile per workgroup (a loop only for launches beyond the grid limit): the
             // load -> blocks -> store phases of different workgroups overlap on a CU, and no
@@ -653,113 +787,21 @@ namespace qsv
             {
                 TaskPtr T;
                 uint64_t base;
-                locate(g, T, base);
+                io.locate(g, T, base);
                 {
-                    v4f v[LOADS];
-                    fetch(T, base, v);
-#pragma unroll
-                    for (int i = 0; i < LOADS; ++i)
-                        *reinterpret_cast<v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * NTH)]) = v[i];
+                    v4f v[TileIO<M, 64 * W>::LOADS];
+                    io.fetch(T, base, v);
+                    io.put(tile, v);
                 }
                 stage_barrier<W>();
                 const int nblocks = T->nblocks;
                 const __attribute__((address_space(4))) StageBlock *blocks = as_constant(reinterpret_cast<const StageBlock *>(mats + T->blocks));
                 for (int bi = 0; bi < nblocks; ++bi)
                 {
-                    const __attribute__((address_space(4))) StageBlock &B = blocks[bi];
-                    BlockMatrix<KQ, X3> A;
-                    A.load(mats + B.matrix, c, h);
-                    // all of the matrix loads are issued here, together (one L2 round trip),
-                    // not moved next to their first uses
-                    __builtin_amdgcn_sched_barrier(0);
-                    // LDS byte offsets (swizzled) of the block bits (amplitude index bits) and
-                    // of the column bits; uniform, so kept in scalar registers
-                    uint32_t tb[KQ], cb[M - KQ];
-#pragma unroll
-                    for (int j = 0; j < KQ; ++j)
-                        tb[j] = swz64b(uint32_t(B.w[j]));
-#pragma unroll
-                    for (int j = 0; j < M - KQ; ++j)
-                        cb[j] = swz64b(uint32_t(B.w[KQ + j]));
-                    // amplitude a of a column: XOR of tb[j] over the set bits j of a
-                    auto amp_off = [&](int a) {
-                        uint32_t o = 0;
-#pragma unroll
-                        for (int j = 0; j < KQ; ++j)
-                            if ((a >> j) & 1)
-                                o ^= tb[j];
-                        return o;
-                    };
-                    // the lane's column bits 0..4 (c) and its amplitude half h: gathers read
-                    // amplitudes h*AMPS + e, scatters write 16*mt + 4*q + 2*h (+1)
-                    uint32_t lanep = 0;
-#pragma unroll
-                    for (int j = 0; j < 5; ++j)
-                        lanep ^= uint32_t(-int((c >> j) & 1)) & cb[j];
-                    const uint32_t hmask = uint32_t(-h);
-                    const uint32_t g_lane = lanep ^ (hmask & amp_off(AMPS));
-                    const uint32_t s_lane = lanep ^ (hmask & amp_off(2));
-                    const char *tbytes = reinterpret_cast<const char *>(tile);
-                    char *wbytes = reinterpret_cast<char *>(tile);
-                    // G column groups per pass: their gathers are in flight together and their
-                    // products form independent MFMA chains
-#pragma unroll
-                    for (int gr0 = 0; gr0 < PER_WAVE; gr0 += G)
-                    {
-                        uint32_t U[G];
-                        float x[G][H];
-#pragma unroll
-                        for (int g = 0; g < G; ++g)
-                        {
-                            // column bits 5.. of this group (uniform)
-                            const int grp = wave * PER_WAVE + gr0 + g;
-                            U[g] = 0;
-#pragma unroll
-                            for (int j = 5; j < M - KQ; ++j)
-                                if ((grp >> (j - 5)) & 1)
-                                    U[g] ^= cb[j];
-                            const uint32_t gbase = g_lane ^ U[g];
-#pragma unroll
-                            for (int e = 0; e < AMPS; ++e)
-                            {
-                                const v2f a = *reinterpret_cast<const v2f *>(tbytes + (gbase ^ amp_off(e)));
-                                x[g][2 * e] = a.x;
-                                x[g][2 * e + 1] = a.y;
-                            }
-                        }
-                        v16f d[G][MT];
-#pragma unroll
-                        for (int g = 0; g < G; ++g)
-#pragma unroll
-                            for (int mt = 0; mt < MT; ++mt)
-                                d[g][mt] = v16f{0.f};
-                        A.template apply_n<G>(x, d);
-                        // accumulator register r of tile mt holds real row 32*mt + 8*(r/4) + 4*h + r%4,
-                        // i.e. amplitudes 16*mt + 4*(r/4) + 2*h and the next one
-#pragma unroll
-                        for (int g = 0; g < G; ++g)
-                        {
-                            const uint32_t sbase = s_lane ^ U[g];
-#pragma unroll
-                            for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                                for (int q = 0; q < 4; ++q)
-                                {
-                                    const uint32_t o0 = sbase ^ amp_off(16 * mt + 4 * q);
-                                    *reinterpret_cast<v2f *>(wbytes + o0) = v2f{d[g][mt][4 * q + 0], d[g][mt][4 * q + 1]};
-                                    *reinterpret_cast<v2f *>(wbytes + (o0 ^ tb[0])) = v2f{d[g][mt][4 * q + 2], d[g][mt][4 * q + 3]};
-                                }
-                        }
-                    }
+                    stage_block<KQ, M, X3, W>(tile, mats, blocks[bi], wave, c, h);
                     stage_barrier<W>();
                 }
-                c64 *__restrict__ s = T->state;
-#pragma unroll
-                for (int i = 0; i < LOADS; ++i)
-                {
-                    const v4f v = *reinterpret_cast<const v4f *>(&tile[swz64(2 * threadIdx.x + i * 2 * NTH)]);
-                    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(s + (base + offset(T, i))));
-                }
+                io.store(tile, T, base);
                 // the LDS tile is reused only when this workgroup takes another tile
                 if (g + gridDim.x < total_tiles)
                     stage_barrier<W>();
@@ -944,22 +986,26 @@ namespace qsv
             throw std::runtime_error("stage: tile bits must be 11, 12 or 13");
         if (base == nullptr || tasks == nullptr || ntasks < 1 || total_tiles < 1)
             throw std::runtime_error("stage: no tasks");
-        // one workgroup per tile (measured on 32-qubit brick stages: 19.5 ms per 2^13-tile
-        // stage against 23.4 ms with 8192 workgroups looping over the tiles)
+        // one workgroup per tile: the load -> blocks -> store phases of the workgroups on a CU
+        // overlap, and no wave waits for its own stores before loading another tile
+        // (measured on 32-qubit brick stages, 2^13 tiles: 19.5 ms per stage against 23.4 ms
+        // with 8192 workgroups looping over the tiles; a persistent form that loads the next
+        // tile during the last block measured 21.1 ms)
         const uint64_t grid = total_tiles < 0x7fffffffu ? total_tiles : 0x7fffffffu;
         // waves per workgroup by tile size: smaller tiles, fewer waves, more workgroups per CU
         auto go = [&](auto kern, int waves) { hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), 0, st, base, tasks, ntasks, total_tiles); };
         constexpr int W13 = QSV_STAGE_WAVES_13, W12 = QSV_STAGE_WAVES_12, W11 = QSV_STAGE_WAVES_11;
+        constexpr int X13 = QSV_STAGE_WAVES_13_X3, X12 = QSV_STAGE_WAVES_12_X3;
         if (kq == 5 && m == 13)
-            x3 ? go(k_stage<5, 13, true, W13>, W13) : go(k_stage<5, 13, false, W13>, W13);
+            x3 ? go(k_stage<5, 13, true, X13>, X13) : go(k_stage<5, 13, false, W13>, W13);
         else if (kq == 5 && m == 12)
-            x3 ? go(k_stage<5, 12, true, W12>, W12) : go(k_stage<5, 12, false, W12>, W12);
+            x3 ? go(k_stage<5, 12, true, X12>, X12) : go(k_stage<5, 12, false, W12>, W12);
         else if (kq == 5)
             x3 ? go(k_stage<5, 11, true, W11>, W11) : go(k_stage<5, 11, false, W11>, W11);
         else if (m == 13)
-            x3 ? go(k_stage<4, 13, true, W13>, W13) : go(k_stage<4, 13, false, W13>, W13);
+            x3 ? go(k_stage<4, 13, true, X13>, X13) : go(k_stage<4, 13, false, W13>, W13);
         else if (m == 12)
-            x3 ? go(k_stage<4, 12, true, W12>, W12) : go(k_stage<4, 12, false, W12>, W12);
+            x3 ? go(k_stage<4, 12, true, X12>, X12) : go(k_stage<4, 12, false, W12>, W12);
         else
             x3 ? go(k_stage<4, 11, true, W11>, W11) : go(k_stage<4, 11, false, W11>, W11);
         check(hipGetLastError(), "stage kernel");
