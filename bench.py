@@ -191,11 +191,15 @@ def batch_leg(count, seed=20250404, direct=False, tuning=None):
         z_batch, dev = b.run_direct() if direct else b.run()
         runs.append((time.perf_counter() - t0, dev))
     info = b.info()
+    traffic = b.traffic()
     b.close()
     wall = min(r[0] for r in runs)
     dev = min(r[1] for r in runs)
+    gbps = traffic["hbm_bytes"] / dev / 1e9
     log(f"batch leg: {count} circuits, {gates} gates, graph {dev * 1e3:.2f} ms device "
         f"({wall * 1e3:.2f} ms wall), tuning {tuning} found in {t_tune:.1f}s")
+    log(f"batch leg: {traffic['hbm_bytes'] / 1e9:.1f} GB algorithmic HBM traffic in "
+        f"{traffic['kernel_launches']} launches: {gbps:.0f} GB/s over the whole run")
     b = pyqcs.Batch(lib)
     for n, g in work:
         b.add(n, g)
@@ -209,6 +213,8 @@ def batch_leg(count, seed=20250404, direct=False, tuning=None):
             "plain_loop_s": t_plain, "speedup_wall": t_plain / wall,
             "gates_per_s": gates / wall, "max_abs_diff_expz": err,
             "tuning": tuning, "autotune_s": t_tune, "graph_nodes": info["graph_nodes"],
+            "hbm_bytes": traffic["hbm_bytes"], "kernel_launches": traffic["kernel_launches"],
+            "hbm_GBps_whole_run": gbps, "hbm_frac_whole_run": gbps / 8000.0,
             "launch": "direct" if direct else "graph"}
 
 

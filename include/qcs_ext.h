@@ -147,6 +147,12 @@ int qcs_ext_batch_autotune(qcs_ext_batch *batch, int *tuning);
 /* expz receives sum(num_qubits) doubles, circuit by circuit; seconds = device time. */
 int qcs_ext_batch_run(qcs_ext_batch *batch, double *expz, uint64_t count, double *seconds);
 int qcs_ext_batch_info(qcs_ext_batch *batch, int *num_circuits, int *total_qubits, uint64_t *graph_nodes);
+/* Algorithmic HBM bytes of one run and its kernel launches (libqcs.so; prepares the batch):
+ * every fused-block or stage sweep of a complex64 state in HBM reads and writes it (16 B
+ * per amplitude), every basis-state fill writes it (8 B), every <Z> reduction reads it
+ * (8 B); circuits in the LDS-resident kernel move nothing through HBM.  The oracle writes 0
+ * for both. */
+int qcs_ext_batch_traffic(qcs_ext_batch *batch, double *hbm_bytes, uint64_t *kernel_launches);
 /* The same batch by the plain method, circuit after circuit (complex128, one kernel per
  * gate, one simulator per distinct qubit count, in C++): the reference's way of running
  * such a workload, as a baseline.  Same outputs as qcs_ext_batch_run (up to the complex64

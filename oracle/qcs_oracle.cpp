@@ -1529,6 +1529,15 @@ extern "C"
             put(nodes, 0);
         });
     }
+    int qcs_ext_batch_traffic(qcs_ext_batch *b, double *bytes, uint64_t *launches)
+    {
+        return guard([&] {
+            if (!b)
+                throw std::runtime_error("batch is NULL");
+            put(bytes, 0.0);
+            put(launches, 0);
+        });
+    }
     int qcs_ext_mlp_create(qcs_ext_mlp **out, int F, uint64_t seed)
     {
         return guard([&] {

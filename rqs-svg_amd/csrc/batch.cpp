@@ -343,6 +343,24 @@ namespace qcore
             starts_offset = append(starts.data(), starts.size() * sizeof(uint64_t));
         }
         upload((void **)&d_mats_, mats.data(), mats.size() * sizeof(float));
+        // algorithmic traffic and launches of one run (see qcs_ext_batch_traffic)
+        hbm_bytes_ = 0;
+        launches_ = small.empty() ? 0 : 2;
+        if (!staged.empty())
+        {
+            for (const auto &sc : staged)
+            {
+                const double amps = double(uint64_t(1) << circuits_[sc.ci].n);
+                hbm_bytes_ += 16.0 * amps * double(sc.stages.size()) + 8.0 * amps + 8.0 * amps;
+            }
+            launches_ += 2 + rounds.size() + 2;
+        }
+        for (const auto &bc : blocked)
+        {
+            const double amps = double(uint64_t(1) << circuits_[bc.ci].n);
+            hbm_bytes_ += 16.0 * amps * double(bc.blocks.size()) + 8.0 * amps + 8.0 * amps;
+            launches_ += 1 + bc.blocks.size() + 2;
+        }
         int grid = 0;
         if (!small.empty())
         {

@@ -55,6 +55,17 @@ namespace qcore
         // Coordinate search over the tuning parameters on this batch; keeps the best.
         BatchTuning autotune();
         uint64_t graph_nodes() const { return graph_nodes_; }
+        // algorithmic HBM bytes and kernel launches of one run (prepares the batch)
+        double hbm_bytes()
+        {
+            prepare();
+            return hbm_bytes_;
+        }
+        uint64_t kernel_launches()
+        {
+            prepare();
+            return launches_;
+        }
 
     private:
         struct Circuit
@@ -78,6 +89,8 @@ namespace qcore
         hipGraph_t graph_ = nullptr;
         hipGraphExec_t exec_ = nullptr;
         uint64_t graph_nodes_ = 0;
+        double hbm_bytes_ = 0;
+        uint64_t launches_ = 0;
         // device buffers
         qsv::BatchCircuit *d_circs_ = nullptr;
         qsv::BatchOp *d_ops_ = nullptr;

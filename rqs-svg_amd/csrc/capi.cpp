@@ -649,4 +649,13 @@ extern "C"
             put(graph_nodes, b->core->graph_nodes());
         });
     }
+    int qcs_ext_batch_traffic(qcs_ext_batch *b, double *hbm_bytes, uint64_t *kernel_launches)
+    {
+        return guard([&] {
+            if (!b || !b->core)
+                throw std::runtime_error("batch is NULL");
+            put(hbm_bytes, b->core->hbm_bytes());
+            put(kernel_launches, b->core->kernel_launches());
+        });
+    }
 }

@@ -153,6 +153,7 @@ class Library:
         lib.qcs_ext_batch_run_plain.argtypes = [vp, vp, u64, P(ctypes.c_double)]
         lib.qcs_ext_batch_run_direct.argtypes = [vp, vp, u64, P(ctypes.c_double)]
         lib.qcs_ext_batch_info.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int), P(u64)]
+        lib.qcs_ext_batch_traffic.argtypes = [vp, P(ctypes.c_double), P(u64)]
 
     def check(self, rc, what):
         if rc != 1:
@@ -537,6 +538,12 @@ class Batch:
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
         self.L.check(self.lib.qcs_ext_batch_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "batch_info")
         return {"circuits": a.value, "total_qubits": b.value, "graph_nodes": c.value}
+
+    def traffic(self):
+        """Algorithmic HBM bytes and kernel launches of one run."""
+        by, n = ctypes.c_double(), ctypes.c_uint64()
+        self.L.check(self.lib.qcs_ext_batch_traffic(self.h, ctypes.byref(by), ctypes.byref(n)), "batch_traffic")
+        return {"hbm_bytes": by.value, "kernel_launches": n.value}
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value is not None:

@@ -96,6 +96,29 @@ def test_gpu_batch_stage_rounds(gpu, oracle):
 
 
 @pytest.mark.gpu
+def test_gpu_batch_traffic(gpu, oracle):
+    # a gate-free 16-qubit circuit in HBM: one fill (8 B per amplitude) and one <Z>
+    # reduction (8 B), 3 launches; an LDS-resident circuit: no HBM bytes, 2 launches
+    b = pyqcs.Batch(gpu)
+    b.add(16, [])
+    b.add(5, [("h", 0, (), -1)])
+    assert b.traffic() == {"hbm_bytes": 16.0 * 2 ** 16, "kernel_launches": 5}
+    b.close()
+    # staged circuits: bytes grow by one sweep (16 B per amplitude) per stage round
+    b = pyqcs.Batch(gpu)
+    b.add(16, [("ry", q, (0.1 * q,), -1) for q in range(16)])
+    b.set_tuning(14, 512, 5, 1, 11)
+    t = b.traffic()
+    sweeps = (t["hbm_bytes"] - 16.0 * 2 ** 16) / (16.0 * 2 ** 16)
+    assert sweeps == int(sweeps) and sweeps >= 1
+    assert t["kernel_launches"] == 4 + int(sweeps)  # fill pair, rounds, reduction pair
+    b.close()
+    o = pyqcs.Batch(oracle)
+    o.add(4, [])
+    assert o.traffic() == {"hbm_bytes": 0.0, "kernel_launches": 0}
+
+
+@pytest.mark.gpu
 def test_gpu_batch_autotune(gpu):
     work = circuits.variational_batch(24, seed=8, nmin=10, nmax=18, dmin=2, dmax=5)
     b = pyqcs.Batch(gpu)
