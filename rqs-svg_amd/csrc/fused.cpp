@@ -327,7 +327,7 @@ namespace qcore
             if (n >= m + 4)
                 tiles.push_back(m);
         std::vector<PlanOption> opts;
-        opts.push_back(PlanOption{m0, m0 > 0 ? stage_k() : k_, 64, 0});
+        opts.push_back(PlanOption{m0, m0 > 0 ? stage_k() : k_, 64, 0, kTilesAuto});
         for (const auto &o : candidate_options(kq_, tiles, stage_block_sizes()))
             if (o.k <= k_ || o.stage_bits > 0)
                 opts.push_back(o);
@@ -349,7 +349,7 @@ namespace qcore
                     add_block(gates, b);
                 continue;
             }
-            for (const auto &st : plan_stages(gates, n, o.stage_bits, o.k))
+            for (const auto &st : plan_stages(gates, n, o.stage_bits, o.k, o.tiles))
             {
                 std::vector<ResolvedGate> sub;
                 for (int gi : st.gates)
@@ -390,7 +390,7 @@ namespace qcore
         }
         const PlanOption &o = opts[best];
         if (o.stage_bits > 0)
-            apply_stages(gates, o.stage_bits, o.k);
+            apply_stages(gates, o.stage_bits, o.k, o.tiles);
         else
             apply_blocks(gates, plan_with(gates, o));
     }
@@ -795,67 +795,123 @@ namespace qcore
         qsv::launch_block128(s, state, n, b.sub, b.op, b.staged);
     }
 
-    std::vector<PlannedStage> plan_stages(const std::vector<ResolvedGate> &gates, int n, int m, int k)
+    namespace
     {
-        if (m < 8 || m > n)
-            throw std::runtime_error("stages: tile bits out of range");
-        std::vector<PlannedStage> out;
-        std::vector<int> rem(gates.size());
-        std::iota(rem.begin(), rem.end(), 0);
-        while (!rem.empty())
+        // One stage taken from `rem` with the tile starting as `start` (grown up to m qubits
+        // when `grow`): the gates taken, the gates deferred, the tile set.
+        struct StageTake
         {
-            std::vector<char> in_tile(n, 0), blocked(n, 0);
+            std::vector<int> taken, deferred;
+            std::vector<char> in_tile;
+        };
+        StageTake take_stage(const std::vector<std::vector<int>> &qubits, const std::vector<int> &rem, int n, int m, int k, const std::vector<int> &start, bool grow)
+        {
+            StageTake r;
+            r.in_tile.assign(n, 0);
+            std::vector<char> blocked(n, 0);
             int tile_size = 0;
-            for (int q = 0; q < 4; ++q)
-                in_tile[q] = 1, ++tile_size;
-            PlannedStage st;
-            std::vector<int> deferred;
+            for (int q : start)
+                if (!r.in_tile[q])
+                    r.in_tile[q] = 1, ++tile_size;
             for (int gi : rem)
             {
-                const auto qs = gate_qubits(gates[gi]);
+                const auto &qs = qubits[gi];
                 bool take = (int)qs.size() <= k;
-                int grow = 0;
+                int need = 0;
                 for (int q : qs)
                 {
                     if (blocked[q])
                         take = false;
-                    grow += in_tile[q] ? 0 : 1;
+                    need += r.in_tile[q] ? 0 : 1;
                 }
-                if (take && tile_size + grow > m)
+                if (take && need > 0 && (!grow || tile_size + need > m))
                     take = false;
                 if (take)
                 {
                     for (int q : qs)
-                        if (!in_tile[q])
-                            in_tile[q] = 1, ++tile_size;
-                    st.gates.push_back(gi);
+                        if (!r.in_tile[q])
+                            r.in_tile[q] = 1, ++tile_size;
+                    r.taken.push_back(gi);
                 }
                 else
                 {
-                    deferred.push_back(gi);
+                    r.deferred.push_back(gi);
                     for (int q : qs)
                         blocked[q] = 1;
                 }
             }
-            if (st.gates.empty())
-            { // the first remaining gate is wider than k: it runs on its own
-                st.gates.push_back(deferred.front());
-                deferred.erase(deferred.begin());
-            }
-            else
-            {
-                // pad the tile with the lowest free qubits (longer contiguous runs)
-                for (int q = 4; q < n && tile_size < m; ++q)
-                    if (!in_tile[q])
-                        in_tile[q] = 1, ++tile_size;
-                for (int q = 0; q < n; ++q)
-                    if (in_tile[q])
-                        st.tile.push_back(q);
-            }
-            out.push_back(std::move(st));
-            rem.swap(deferred);
+            return r;
         }
-        return out;
+
+        std::vector<PlannedStage> plan_stages_with(const std::vector<std::vector<int>> &qubits, int n, int m, int k, bool windows)
+        {
+            std::vector<PlannedStage> out;
+            std::vector<int> rem(qubits.size());
+            std::iota(rem.begin(), rem.end(), 0);
+            const std::vector<int> low = {0, 1, 2, 3};
+            while (!rem.empty())
+            {
+                StageTake best = take_stage(qubits, rem, n, m, k, low, true);
+                bool grown = true;
+                if (windows)
+                {
+                    for (int a = 4; a + (m - 4) <= n; ++a)
+                    {
+                        std::vector<int> w = low;
+                        for (int q = a; q < a + m - 4; ++q)
+                            w.push_back(q);
+                        StageTake c = take_stage(qubits, rem, n, m, k, w, false);
+                        if (c.taken.size() > best.taken.size())
+                        {
+                            best = std::move(c);
+                            grown = false;
+                        }
+                    }
+                }
+                PlannedStage st;
+                if (best.taken.empty())
+                { // the first remaining gate is wider than k: it runs on its own
+                    st.gates.push_back(rem.front());
+                    rem.erase(rem.begin());
+                    out.push_back(std::move(st));
+                    continue;
+                }
+                st.gates = best.taken;
+                if (grown)
+                {
+                    // pad the tile with the lowest free qubits (longer contiguous runs)
+                    int tile_size = 0;
+                    for (int q = 0; q < n; ++q)
+                        tile_size += best.in_tile[q];
+                    for (int q = 4; q < n && tile_size < m; ++q)
+                        if (!best.in_tile[q])
+                            best.in_tile[q] = 1, ++tile_size;
+                }
+                for (int q = 0; q < n; ++q)
+                    if (best.in_tile[q])
+                        st.tile.push_back(q);
+                out.push_back(std::move(st));
+                rem.swap(best.deferred);
+            }
+            return out;
+        }
+    } // namespace
+
+    std::vector<PlannedStage> plan_stages(const std::vector<ResolvedGate> &gates, int n, int m, int k, int tiles)
+    {
+        if (m < 8 || m > n)
+            throw std::runtime_error("stages: tile bits out of range");
+        if (tiles != kTilesGrow && tiles != kTilesWindow && tiles != kTilesAuto)
+            throw std::runtime_error("stages: unknown tile choice");
+        std::vector<std::vector<int>> qubits;
+        qubits.reserve(gates.size());
+        for (const auto &g : gates)
+            qubits.push_back(gate_qubits(g));
+        if (tiles != kTilesAuto)
+            return plan_stages_with(qubits, n, m, k, tiles == kTilesWindow);
+        auto grow = plan_stages_with(qubits, n, m, k, false);
+        auto window = plan_stages_with(qubits, n, m, k, true);
+        return window.size() < grow.size() ? window : grow;
     }
 
     PreparedStage prepare_stage(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int kq, int m, int n, std::vector<float> &mats, bool x3)
@@ -1073,10 +1129,10 @@ namespace qcore
         stage_bits_ = m;
     }
 
-    void FusedEngine::apply_stages(const std::vector<ResolvedGate> &gates, int m, int k)
+    void FusedEngine::apply_stages(const std::vector<ResolvedGate> &gates, int m, int k, int tiles)
     {
         const int n = sim_.n_;
-        const auto stages = plan_stages(gates, n, m, k);
+        const auto stages = plan_stages(gates, n, m, k, tiles);
         // one matrix buffer for the whole flush: FP32 (complex64 modes) or FP64 entries
         std::vector<float> mats;
         std::vector<double> mats64;

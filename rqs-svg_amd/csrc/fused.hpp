@@ -84,7 +84,17 @@ namespace qcore
         std::vector<int> tile;  // m tile qubits, ascending (empty: one wide gate)
         std::vector<int> gates; // indices into the gate list, in application order
     };
-    std::vector<PlannedStage> plan_stages(const std::vector<ResolvedGate> &gates, int n, int m, int k);
+    // How a stage's tile is chosen.  Grow: qubits 0..3, then the qubits of the gates taken
+    // in order until m are reached (free places padded with the lowest free qubits).
+    // Window: per stage, the candidate among Grow and every {0..3} + m-4 consecutive qubits
+    // that takes the most gates.  Auto: both plans, the one with fewer stages (Grow on a tie).
+    enum StageTiles
+    {
+        kTilesGrow = 0,
+        kTilesWindow = 1,
+        kTilesAuto = 2
+    };
+    std::vector<PlannedStage> plan_stages(const std::vector<ResolvedGate> &gates, int n, int m, int k, int tiles = kTilesAuto);
 
     // A stage ready to launch: its blocks' matrices and descriptors in a matrix buffer.
     struct PreparedStage
@@ -185,7 +195,7 @@ namespace qcore
 
     private:
         void apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan);
-        void apply_stages(const std::vector<ResolvedGate> &gates, int m, int k);
+        void apply_stages(const std::vector<ResolvedGate> &gates, int m, int k, int tiles = kTilesAuto);
         void reserve_matrices(size_t bytes);
 
         Simulator &sim_;

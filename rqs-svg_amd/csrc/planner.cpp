@@ -91,7 +91,8 @@ namespace qcore
                     out.push_back(PlanOption{0, k, w, rule});
         for (int m : tile_bits)
             for (int k : stage_k)
-                out.push_back(PlanOption{m, k, 64, 0});
+                for (int tiles : {0, 1})
+                    out.push_back(PlanOption{m, k, 64, 0, tiles});
         return out;
     }
 

@@ -57,9 +57,11 @@ namespace qcore
         int k;          // qubits per block
         int window;     // blocks examined after the dependency frontier (block plans)
         int rule;       // 0: least growth, 1: fullest block, 2: earliest block that fits
+        int tiles = kTilesAuto; // stage plans: tile choice (StageTiles)
     };
     std::vector<PlannedBlock> plan_with(const std::vector<ResolvedGate> &gates, const PlanOption &opt);
     // Block plans: k from max(2, kq - 2) to kq x windows {4, 64} x rules {0, 1, 2}; stage
-    // plans: each tile size in `tile_bits` x each block size in `stage_k`.
+    // plans: each tile size in `tile_bits` x each block size in `stage_k` x tile choice
+    // {grow, window}.
     std::vector<PlanOption> candidate_options(int kq, const std::vector<int> &tile_bits, const std::vector<int> &stage_k);
 } // namespace qcore
