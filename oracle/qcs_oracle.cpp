@@ -1388,8 +1388,8 @@ extern "C"
     { // the oracle has no fused path; the setting is validated and ignored
         return guard([&] {
             (void)core(s);
-            if (m != 0 && (m < 11 || m > 13))
-                throw std::runtime_error("stage tile bits must be 0, 11, 12 or 13");
+            if (m != 0 && (m < 11 || m > 14))
+                throw std::runtime_error("stage tile bits must be 0, 11, 12, 13 or 14");
         });
     }
     int qcs_ext_set_precision(qcs_simulator *s, int mode, int k)

@@ -323,7 +323,7 @@ namespace qcore
         const int n = sim_.n_;
         const float math = math_feature();
         std::vector<int> tiles;
-        for (int m : f64_ ? std::vector<int>{12} : std::vector<int>{11, 12, 13})
+        for (int m : f64_ ? std::vector<int>{12} : std::vector<int>{11, 12, 13, 14})
             if (n >= m + 4)
                 tiles.push_back(m);
         std::vector<PlanOption> opts;
@@ -417,7 +417,7 @@ namespace qcore
         };
         auto unif = [&]() { return double(rnd() >> 11) * 0x1.0p-53; };
         std::vector<int> tiles;
-        for (int m : f64_ ? std::vector<int>{12} : std::vector<int>{11, 12, 13})
+        for (int m : f64_ ? std::vector<int>{12} : std::vector<int>{11, 12, 13, 14})
             if (n >= m + 4)
                 tiles.push_back(m);
         // 1..24 random gates (U3, or CX/CZ with probability 0.3) on the qubit set qs
@@ -1123,8 +1123,8 @@ namespace qcore
 
     void FusedEngine::set_stage_bits(int m)
     {
-        if (m != 0 && (m < 11 || m > 13))
-            throw std::runtime_error("stage tile bits must be 0 (per-block launches), 11, 12 or 13");
+        if (m != 0 && (m < 11 || m > 14))
+            throw std::runtime_error("stage tile bits must be 0 (per-block launches), 11, 12, 13 or 14");
         flush();
         stage_bits_ = m;
     }

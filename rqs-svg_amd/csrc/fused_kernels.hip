@@ -34,6 +34,12 @@ namespace qsv
         constexpr int kThreads = 64 * kWaves;
 // waves per stage workgroup by tile bits and arithmetic (FP32 or BF16x3 MFMA), measured on
 // 32-qubit brick circuits
+#ifndef QSV_STAGE_WAVES_14
+#define QSV_STAGE_WAVES_14 4
+#endif
+#ifndef QSV_STAGE_WAVES_14_X3
+#define QSV_STAGE_WAVES_14_X3 4
+#endif
 #ifndef QSV_STAGE_WAVES_13
 #define QSV_STAGE_WAVES_13 4
 #endif
@@ -989,8 +995,8 @@ namespace qsv
     {
         if (kq != 4 && kq != 5)
             throw std::runtime_error("stage: kq must be 4 or 5");
-        if (m < 11 || m > 13)
-            throw std::runtime_error("stage: tile bits must be 11, 12 or 13");
+        if (m < 11 || m > 14)
+            throw std::runtime_error("stage: tile bits must be 11, 12, 13 or 14");
         if (n <= m || t.tsub.nops != m || n > kMaxQubits)
             throw std::runtime_error("stage: the tile must be a proper part of the state");
         for (int k = 0; k < t.tsub.nops; ++k)
@@ -1012,8 +1018,8 @@ namespace qsv
     {
         if (kq != 4 && kq != 5)
             throw std::runtime_error("stage: kq must be 4 or 5");
-        if (m < 11 || m > 13)
-            throw std::runtime_error("stage: tile bits must be 11, 12 or 13");
+        if (m < 11 || m > 14)
+            throw std::runtime_error("stage: tile bits must be 11, 12, 13 or 14");
         if (base == nullptr || tasks == nullptr || ntasks < 1 || total_tiles < 1)
             throw std::runtime_error("stage: no tasks");
         // one workgroup per tile: the load -> blocks -> store phases of the workgroups on a CU
@@ -1026,7 +1032,12 @@ namespace qsv
         auto go = [&](auto kern, int waves) { hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), 0, st, base, tasks, ntasks, total_tiles); };
         constexpr int W13 = QSV_STAGE_WAVES_13, W12 = QSV_STAGE_WAVES_12, W11 = QSV_STAGE_WAVES_11;
         constexpr int X13 = QSV_STAGE_WAVES_13_X3, X12 = QSV_STAGE_WAVES_12_X3;
-        if (kq == 5 && m == 13)
+        constexpr int W14 = QSV_STAGE_WAVES_14, X14 = QSV_STAGE_WAVES_14_X3;
+        if (kq == 5 && m == 14)
+            x3 ? go(k_stage<5, 14, true, X14>, X14) : go(k_stage<5, 14, false, W14>, W14);
+        else if (m == 14)
+            x3 ? go(k_stage<4, 14, true, X14>, X14) : go(k_stage<4, 14, false, W14>, W14);
+        else if (kq == 5 && m == 13)
             x3 ? go(k_stage<5, 13, true, X13>, X13) : go(k_stage<5, 13, false, W13>, W13);
         else if (kq == 5 && m == 12)
             x3 ? go(k_stage<5, 12, true, X12>, X12) : go(k_stage<5, 12, false, W12>, W12);

@@ -129,8 +129,8 @@ namespace qcore
 
     void Simulator::set_stage_bits(int m)
     {
-        if (m != 0 && (m < 11 || m > 13))
-            throw std::runtime_error("stage tile bits must be 0 (one sweep per block), 11, 12 or 13");
+        if (m != 0 && (m < 11 || m > 14))
+            throw std::runtime_error("stage tile bits must be 0 (one sweep per block), 11, 12, 13 or 14");
         stage_bits_ = m;
         if (fused_)
             fused_->set_stage_bits(m);

@@ -196,7 +196,7 @@ def test_fused_complex128_qft_and_measure(gpu, oracle):
 
 
 @pytest.mark.parametrize("precision", [1, 3])
-@pytest.mark.parametrize("m", [11, 12, 13])
+@pytest.mark.parametrize("m", [11, 12, 13, 14])
 def test_stages_match_oracle(gpu, oracle, precision, m):
     # stages (several blocks per sweep on LDS-resident tiles), with gates wider than a
     # block (up to 7 qubits) falling back to single launches
@@ -242,9 +242,9 @@ def test_stages_vs_block_launches(gpu, precision):
 
 def test_stage_bits_validation(gpu):
     with pyqcs.Simulator(gpu, 8, precision=1) as sim:
-        for bad in (5, 10, 14, -1):
+        for bad in (5, 10, 15, -1):
             assert sim.lib.qcs_ext_set_stage_bits(sim.h, bad) == 0
-        for good in (0, 11, 12, 13):
+        for good in (0, 11, 12, 13, 14):
             sim.set_stage_bits(good)
 
 
