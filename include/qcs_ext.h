@@ -109,9 +109,9 @@ int qcs_ext_set_precision(qcs_simulator *sim, int mode, int max_block_qubits);
 
 /* Fused precision modes: blocks of a flush are grouped into stages, each applied to
  * LDS-resident tiles of 2^tile_bits amplitudes in one sweep of the state (tile_bits 11, 12,
- * 13 or 14, default 13; complex128 tiles hold 2^12; used when the state has at least
- * tile_bits + 4 qubits), or launched one sweep per block (tile_bits 0).  Any time; queued gates are flushed
- * first. */
+ * 13 or 14, default 13; complex128 tiles hold 2^12 and take 3-qubit FP64 blocks; used when
+ * the state has at least tile_bits + 4 qubits), or launched one sweep per block (tile_bits
+ * 0).  Any time; queued gates are flushed first. */
 int qcs_ext_set_stage_bits(qcs_simulator *sim, int tile_bits);
 
 /* Apply all queued gates now (no-op in mode 0). */

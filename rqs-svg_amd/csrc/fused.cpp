@@ -804,7 +804,10 @@ namespace qcore
             std::vector<int> taken, deferred;
             std::vector<char> in_tile;
         };
-        StageTake take_stage(const std::vector<std::vector<int>> &qubits, const std::vector<int> &rem, int n, int m, int k, const std::vector<int> &start, bool grow)
+        // Gates are taken in order; a gate may pass a deferred gate only if they commute:
+        // disjoint qubits, or both diagonal.  blocked[q]: 0 no deferred gate on q, 1 only
+        // diagonal ones, 2 some other gate.
+        StageTake take_stage(const std::vector<std::vector<int>> &qubits, const std::vector<char> &diag, const std::vector<int> &rem, int n, int m, int k, const std::vector<int> &start, bool grow)
         {
             StageTake r;
             r.in_tile.assign(n, 0);
@@ -816,11 +819,12 @@ namespace qcore
             for (int gi : rem)
             {
                 const auto &qs = qubits[gi];
+                const char level = diag[gi] ? 1 : 2;
                 bool take = (int)qs.size() <= k;
                 int need = 0;
                 for (int q : qs)
                 {
-                    if (blocked[q])
+                    if (blocked[q] > (diag[gi] ? 1 : 0))
                         take = false;
                     need += r.in_tile[q] ? 0 : 1;
                 }
@@ -837,13 +841,13 @@ namespace qcore
                 {
                     r.deferred.push_back(gi);
                     for (int q : qs)
-                        blocked[q] = 1;
+                        blocked[q] = std::max(blocked[q], level);
                 }
             }
             return r;
         }
 
-        std::vector<PlannedStage> plan_stages_with(const std::vector<std::vector<int>> &qubits, int n, int m, int k, bool windows)
+        std::vector<PlannedStage> plan_stages_with(const std::vector<std::vector<int>> &qubits, const std::vector<char> &diag, int n, int m, int k, bool windows)
         {
             std::vector<PlannedStage> out;
             std::vector<int> rem(qubits.size());
@@ -851,7 +855,7 @@ namespace qcore
             const std::vector<int> low = {0, 1, 2, 3};
             while (!rem.empty())
             {
-                StageTake best = take_stage(qubits, rem, n, m, k, low, true);
+                StageTake best = take_stage(qubits, diag, rem, n, m, k, low, true);
                 bool grown = true;
                 if (windows)
                 {
@@ -860,7 +864,7 @@ namespace qcore
                         std::vector<int> w = low;
                         for (int q = a; q < a + m - 4; ++q)
                             w.push_back(q);
-                        StageTake c = take_stage(qubits, rem, n, m, k, w, false);
+                        StageTake c = take_stage(qubits, diag, rem, n, m, k, w, false);
                         if (c.taken.size() > best.taken.size())
                         {
                             best = std::move(c);
@@ -904,13 +908,20 @@ namespace qcore
         if (tiles != kTilesGrow && tiles != kTilesWindow && tiles != kTilesAuto)
             throw std::runtime_error("stages: unknown tile choice");
         std::vector<std::vector<int>> qubits;
+        std::vector<char> diag;
         qubits.reserve(gates.size());
         for (const auto &g : gates)
+        {
             qubits.push_back(gate_qubits(g));
+            // diagonal in the computational basis (controls keep it so); a gate on a measured
+            // target is kept in place
+            const Kind kd = g.spec.kind;
+            diag.push_back(g.mmask == 0 && (kd == Kind::GlobalPhase || kd == Kind::PhaseOne || kd == Kind::Diag1 || kd == Kind::Diag2 || kd == Kind::Identity));
+        }
         if (tiles != kTilesAuto)
-            return plan_stages_with(qubits, n, m, k, tiles == kTilesWindow);
-        auto grow = plan_stages_with(qubits, n, m, k, false);
-        auto window = plan_stages_with(qubits, n, m, k, true);
+            return plan_stages_with(qubits, diag, n, m, k, tiles == kTilesWindow);
+        auto grow = plan_stages_with(qubits, diag, n, m, k, false);
+        auto window = plan_stages_with(qubits, diag, n, m, k, true);
         return window.size() < grow.size() ? window : grow;
     }
 

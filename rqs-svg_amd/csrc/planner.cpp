@@ -70,6 +70,7 @@ namespace qcore
             f[5 + i] = (m - 5 + i >= 0 ? tile[m - 5 + i] : n) / 32.f;
         f[13] = gates / 128.f;
         f[14] = (n - m) / 32.f;
+        f[15] = float(nblocks << kq) / 256.f; // arithmetic per amplitude, relative (blocks x 2^k)
         f[16] = nblocks / 16.f;
         f[17] = m / 16.f;
         f[18] = nblocks ? float(gates) / nblocks / 32.f : 0.f;
