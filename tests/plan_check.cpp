@@ -1,0 +1,147 @@
+// Checks the stage planner (qcore::plan_stages) on seeded random gate lists, on the CPU:
+// every gate lands in exactly one stage, stages keep the gates' order, a stage's gates lie
+// inside its tile (m qubits, ascending, 0..3 included), a stage without a tile holds one
+// gate wider than the block size, and any two gates whose order the plan swaps commute
+// (disjoint qubits, or both diagonal without measured targets).  Prints one line per case
+// and "ok" at the end; exits 1 at the first violation.
+// Usage: plan_check <cases> <seed>
+#include "fused.hpp"
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+using namespace qcore;
+
+namespace
+{
+    uint64_t mix(uint64_t &s)
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+
+    bool diagonal(const ResolvedGate &g)
+    {
+        const Kind k = g.spec.kind;
+        return g.mmask == 0 && (k == Kind::GlobalPhase || k == Kind::PhaseOne || k == Kind::Diag1 || k == Kind::Diag2 || k == Kind::Identity);
+    }
+
+    std::vector<int> qubits_of(const ResolvedGate &g)
+    {
+        std::vector<int> q = g.tphys;
+        q.insert(q.end(), g.pphys.begin(), g.pphys.end());
+        q.insert(q.end(), g.nphys.begin(), g.nphys.end());
+        return q;
+    }
+
+    [[noreturn]] void fail(const std::string &what, int c)
+    {
+        std::printf("FAIL case %d: %s\n", c, what.c_str());
+        std::exit(1);
+    }
+} // namespace
+
+int main(int argc, char **argv)
+{
+    const int cases = argc > 1 ? std::atoi(argv[1]) : 50;
+    uint64_t seed = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 1;
+    // gate names by target count; p, rz, rzz, z, s, t are diagonal, the rest not
+    const char *one[] = {"h", "x", "z", "s", "t", "p", "rz", "rx", "u3", "sx"};
+    const char *two[] = {"swap", "rzz", "rxx", "ecr", "iswap"};
+    for (int c = 0; c < cases; ++c)
+    {
+        const int n = 14 + int(mix(seed) % 10);
+        const int m = 11 + int(mix(seed) % 3);
+        const int k = 3 + int(mix(seed) % 3);
+        const int ngates = 20 + int(mix(seed) % 200);
+        std::vector<ResolvedGate> gates;
+        for (int i = 0; i < ngates; ++i)
+        {
+            ResolvedGate r;
+            const int nt = (mix(seed) % 5 == 0) ? 2 : 1;
+            const char *name = nt == 1 ? one[mix(seed) % 10] : two[mix(seed) % 5];
+            double p[4] = {0.3, 0.7, 1.1, 0.2};
+            r.spec = make_gate(name, p);
+            std::vector<int> pool(n);
+            for (int q = 0; q < n; ++q)
+                pool[q] = q;
+            for (int q = n - 1; q > 0; --q)
+                std::swap(pool[q], pool[mix(seed) % (q + 1)]);
+            int used = 0;
+            for (int t = 0; t < nt; ++t)
+                r.tphys.push_back(pool[used++]);
+            const int nc = int(mix(seed) % 4); // 0..3 controls: some gates are wider than k
+            for (int j = 0; j < nc; ++j)
+                (mix(seed) & 1 ? r.pphys : r.nphys).push_back(pool[used++]);
+            if (mix(seed) % 17 == 0)
+                r.mmask = 1; // a measured target: stays in order
+            gates.push_back(r);
+        }
+        for (int tiles : {kTilesGrow, kTilesWindow, kTilesAuto})
+        {
+            const auto plan = plan_stages(gates, n, m, k, tiles);
+            std::vector<int> stage_of(gates.size(), -1);
+            for (int s = 0; s < (int)plan.size(); ++s)
+            {
+                const auto &st = plan[s];
+                if (st.gates.empty())
+                    fail("empty stage", c);
+                for (size_t i = 0; i < st.gates.size(); ++i)
+                {
+                    const int gi = st.gates[i];
+                    if (gi < 0 || gi >= (int)gates.size() || stage_of[gi] != -1)
+                        fail("gate missing, repeated or out of range", c);
+                    stage_of[gi] = s;
+                    if (i > 0 && st.gates[i] <= st.gates[i - 1])
+                        fail("gates out of order inside a stage", c);
+                }
+                if (st.tile.empty())
+                {
+                    if (st.gates.size() != 1 || (int)qubits_of(gates[st.gates[0]]).size() <= k)
+                        fail("a stage without a tile must hold one gate wider than k", c);
+                    continue;
+                }
+                if ((int)st.tile.size() != m || !std::is_sorted(st.tile.begin(), st.tile.end()) ||
+                    std::adjacent_find(st.tile.begin(), st.tile.end()) != st.tile.end() || st.tile.back() >= n)
+                    fail("tile not m ascending qubits of the state", c);
+                for (int q = 0; q < 4; ++q)
+                    if (st.tile[q] != q)
+                        fail("tile without qubits 0..3", c);
+                for (int gi : st.gates)
+                {
+                    const auto qs = qubits_of(gates[gi]);
+                    if ((int)qs.size() > k)
+                        fail("gate wider than k inside a tiled stage", c);
+                    for (int q : qs)
+                        if (!std::binary_search(st.tile.begin(), st.tile.end(), q))
+                            fail("gate qubit outside its stage's tile", c);
+                }
+            }
+            for (size_t gi = 0; gi < gates.size(); ++gi)
+                if (stage_of[gi] < 0)
+                    fail("gate in no stage", c);
+            // order swaps: a before b in the list but b in an earlier stage
+            for (size_t a = 0; a < gates.size(); ++a)
+                for (size_t b = a + 1; b < gates.size(); ++b)
+                {
+                    if (stage_of[b] >= stage_of[a])
+                        continue;
+                    const auto qa = qubits_of(gates[a]), qb = qubits_of(gates[b]);
+                    bool share = false;
+                    for (int q : qa)
+                        share = share || std::find(qb.begin(), qb.end(), q) != qb.end();
+                    if (share && !(diagonal(gates[a]) && diagonal(gates[b])))
+                        fail("two gates that do not commute were reordered", c);
+                }
+            std::printf("case %d n %d m %d k %d gates %d tiles %d: %zu stages\n", c, n, m, k, ngates, tiles, plan.size());
+        }
+    }
+    std::printf("ok\n");
+    return 0;
+}
