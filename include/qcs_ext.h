@@ -149,8 +149,9 @@ int qcs_ext_batch_run(qcs_ext_batch *batch, double *expz, uint64_t count, double
 int qcs_ext_batch_info(qcs_ext_batch *batch, int *num_circuits, int *total_qubits, uint64_t *graph_nodes);
 /* Algorithmic HBM bytes of one run and its kernel launches (libqcs.so; prepares the batch):
  * every fused-block or stage sweep of a complex64 state in HBM reads and writes it (16 B
- * per amplitude), every basis-state fill writes it (8 B), every <Z> reduction reads it
- * (8 B); circuits in the LDS-resident kernel move nothing through HBM.  The oracle writes 0
+ * per amplitude) except a circuit's first stage round, which writes it from |0...0>
+ * without reading (8 B), every basis-state fill writes it (8 B), every <Z> reduction reads
+ * it (8 B); circuits in the LDS-resident kernel move nothing through HBM.  The oracle writes 0
  * for both. */
 int qcs_ext_batch_traffic(qcs_ext_batch *batch, double *hbm_bytes, uint64_t *kernel_launches);
 /* The same batch by the plain method, circuit after circuit (complex128, one kernel per

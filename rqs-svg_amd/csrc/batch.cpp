@@ -5,9 +5,9 @@
 // Those of at least stage_bits + 4 qubits run as stages on LDS tiles, batched across
 // circuits: one launch per stage round covers the r-th stage of every such circuit.  The
 // rest run as fused k-qubit blocks, one launch each, spread over several streams.
-// Everything -- counter reset, the persistent kernel, the fills, the stage rounds, the
-// block sequences, the <Z> reductions -- is captured once into a hipGraph and replayed
-// with a single launch.
+// Everything -- counter reset, the persistent kernel, the stage rounds, the fills and
+// block sequences of the smaller HBM circuits, the <Z> reductions -- is captured once into
+// a hipGraph and replayed with a single launch.
 #include "batch.hpp"
 #include "simulator.hpp"
 
@@ -262,7 +262,6 @@ namespace qcore
         };
         for (const auto &sc : staged)
             place(sc.ci);
-        const uint64_t staged_amps = state_total;
         for (const auto &bc : blocked)
             place(bc.ci);
         // ---- device buffers (addresses go into the task arrays below) ----
@@ -288,7 +287,7 @@ namespace qcore
             uint64_t tiles;
         };
         std::vector<Round> rounds;
-        size_t zsum_offset = 0, starts_offset = 0;
+        size_t zsum_offset = 0;
         uint32_t zsum_blocks_total = 0, zsum_qubits_total = 0;
         const int kq_staged = T.block_qubits;
         if (!staged.empty())
@@ -308,12 +307,13 @@ namespace qcore
                     if (p.kq != kq_staged)
                         throw std::runtime_error("batch: stage round with mixed block sizes");
                     tasks.push_back(stage_task(p, d_states_ + state_off[i], tiles));
+                    if (r == 0)
+                        tasks.back().flags = qsv::kStageZeroState; // the first stage starts from |0...0>
                     tiles += uint64_t(1) << (p.n - p.m);
                 }
                 rounds.push_back(Round{append_stage_tasks(mats, tasks), (int)tasks.size(), tiles});
             }
             std::vector<qsv::ZsumTask> zt;
-            std::vector<uint64_t> starts;
             for (size_t i = 0; i < staged.size(); ++i)
             {
                 const int n = circuits_[staged[i].ci].n;
@@ -328,7 +328,6 @@ namespace qcore
                 zsum_blocks_total += uint32_t(z.nblocks);
                 zsum_qubits_total += uint32_t(n);
                 zt.push_back(z);
-                starts.push_back(state_off[i]);
             }
             static_assert(sizeof(qsv::ZsumTask) % 16 == 0, "ZsumTask keeps 16-B alignment");
             auto append = [&](const void *src, size_t bytes) {
@@ -340,7 +339,6 @@ namespace qcore
                 return off;
             };
             zsum_offset = append(zt.data(), zt.size() * sizeof(qsv::ZsumTask));
-            starts_offset = append(starts.data(), starts.size() * sizeof(uint64_t));
         }
         upload((void **)&d_mats_, mats.data(), mats.size() * sizeof(float));
         // algorithmic traffic and launches of one run (see qcs_ext_batch_traffic)
@@ -350,10 +348,11 @@ namespace qcore
         {
             for (const auto &sc : staged)
             {
+                // the first stage writes without reading; the reduction reads once
                 const double amps = double(uint64_t(1) << circuits_[sc.ci].n);
-                hbm_bytes_ += 16.0 * amps * double(sc.stages.size()) + 8.0 * amps + 8.0 * amps;
+                hbm_bytes_ += 16.0 * amps * double(sc.stages.size()) - 8.0 * amps + 8.0 * amps;
             }
-            launches_ += 2 + rounds.size() + 2;
+            launches_ += rounds.size() + 2;
         }
         for (const auto &bc : blocked)
         {
@@ -388,8 +387,8 @@ namespace qcore
         std::vector<uint64_t> blocked_partial(partial_off.begin() + nstaged, partial_off.end());
         // the whole batch as launches on the streams: captured once into the graph below,
         // or issued directly by run_direct
-        work_ = [this, blocked = std::move(blocked), blocked_state, blocked_partial, rounds, nstaged, staged_amps, kq_staged,
-                 stage_bits = T.stage_bits, zsum_offset, starts_offset, zsum_blocks_total, zsum_qubits_total, grid, nsmall,
+        work_ = [this, blocked = std::move(blocked), blocked_state, blocked_partial, rounds, nstaged, kq_staged,
+                 stage_bits = T.stage_bits, zsum_offset, zsum_blocks_total, zsum_qubits_total, grid, nsmall,
                  small_max_n, ns, block_size = T.block_size]() mutable {
             hipStream_t s0 = streams_[0];
             qsv::check(hipEventRecord(fork_join_[0], s0), "hipEventRecord");
@@ -404,7 +403,6 @@ namespace qcore
             if (nstaged > 0)
             {
                 hipStream_t s = stream();
-                qsv::launch_basis_states(s, d_states_, staged_amps, reinterpret_cast<const uint64_t *>(d_mats_ + starts_offset), nstaged);
                 for (const auto &r : rounds)
                     launch_stage_tasks(s, kq_staged, stage_bits, false, d_mats_, r.task_offset, r.ntasks, r.tiles);
                 qsv::launch_zsum_tasks(s, reinterpret_cast<const qsv::ZsumTask *>(d_mats_ + zsum_offset), nstaged, zsum_blocks_total, zsum_qubits_total);

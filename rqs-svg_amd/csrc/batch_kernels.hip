@@ -205,19 +205,6 @@ namespace qsv
             for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
                 *reinterpret_cast<v2f *>(s + i) = v2f{i == 0 ? 1.f : 0.f, 0.f};
         }
-        __global__ void k_fill_zero(c64 *s, uint64_t count)
-        {
-            uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-            for (; i < count; i += uint64_t(gridDim.x) * blockDim.x)
-                *reinterpret_cast<v2f *>(s + i) = v2f{0.f, 0.f};
-        }
-
-        __global__ void k_set_ones(c64 *base, const uint64_t *__restrict__ starts, int nstarts)
-        {
-            const int i = blockIdx.x * blockDim.x + threadIdx.x;
-            if (i < nstarts)
-                *reinterpret_cast<v2f *>(base + starts[i]) = v2f{1.f, 0.f};
-        }
     } // namespace
 
     size_t batch_lds_bytes(int max_n) { return (size_t(1) << max_n) * sizeof(c64); }
@@ -313,16 +300,4 @@ namespace qsv
         check(hipGetLastError(), "zsum kernel");
     }
 
-    void launch_basis_states(hipStream_t s, c64 *base, uint64_t count, const uint64_t *starts, int nstarts)
-    {
-        if (base == nullptr || starts == nullptr || count < 1 || nstarts < 1)
-            throw std::runtime_error("basis states: nothing to fill");
-        uint64_t g = (count + 255) / 256;
-        if (g > 65536)
-            g = 65536;
-        hipLaunchKernelGGL(k_fill_zero, dim3(g), dim3(256), 0, s, base, count);
-        check(hipGetLastError(), "fill kernel");
-        hipLaunchKernelGGL(k_set_ones, dim3((nstarts + 255) / 256), dim3(256), 0, s, base, starts, nstarts);
-        check(hipGetLastError(), "fill kernel");
-    }
 } // namespace qsv

@@ -52,7 +52,4 @@ namespace qsv
         int nblocks;
     };
     void launch_zsum_tasks(hipStream_t s, const ZsumTask *tasks, int ntasks, uint32_t total_blocks, uint32_t total_qubits);
-    // |0...0> in several states: zeroes [base, base + count), then writes 1 at base + starts[i]
-    // for i < nstarts (starts in device memory, each < count).
-    void launch_basis_states(hipStream_t s, c64 *base, uint64_t count, const uint64_t *starts, int nstarts);
 } // namespace qsv

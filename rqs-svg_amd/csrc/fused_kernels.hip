@@ -784,6 +784,14 @@ namespace qsv
             __device__ __forceinline__ void fetch(TaskPtr T, uint64_t base, v4f *v) const
             {
                 const c64 *__restrict__ s = T->state;
+                if (T->flags & kStageZeroState)
+                {
+                    // |0...0>: amplitude 1 at index 0 (the first of this v4f's pair)
+#pragma unroll
+                    for (int i = 0; i < LOADS; ++i)
+                        v[i] = v4f{base + offset(T, i) == 0 ? 1.f : 0.f, 0.f, 0.f, 0.f};
+                    return;
+                }
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
                     v[i] = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (base + offset(T, i))));

@@ -60,7 +60,10 @@ namespace qsv
         Subspace tsub;   // the tile qubits, ascending (tiles enumerate the other qubits)
         uint32_t blocks;
         int nblocks;
+        uint32_t flags; // kStageZeroState: the state is |0...0>; the tiles are made, not loaded
+        uint32_t pad;
     };
+    constexpr uint32_t kStageZeroState = 1;
     // Host-side check of one task before upload (the kernel trusts the tasks).
     void check_stage_task(const StageTask &t, int n, int kq, int m);
     // tasks: device array of ntasks tasks with ascending tile_begin, the first 0, all checked
