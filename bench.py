@@ -162,60 +162,72 @@ def fused_leg(n, depth, k, shots, variants=FUSED_VARIANTS):
     return out
 
 
-def batch_leg(count, seed=20250404, direct=False, tuning=None):
+def batch_leg(count, seed=20250404, direct=False, tuning=None, precisions=(1, 3)):
     """512 variational circuits (12-22 qubits, 4-16 layers of RY/RZ + CNOT ladder): one
     autotuned hipGraph launch (complex64 states, FP64 <Z> reductions) against the per-circuit
     plain-method loop in C++ (complex128, one kernel per gate, one simulator per qubit
-    count; qcs_ext_batch_run_plain)."""
+    count; qcs_ext_batch_run_plain).  The block products of the circuits in HBM run on FP32
+    MFMA (the top-level figures) and on BF16x3 split-precision MFMA ("bf16x3"), each with its
+    own tuning and its own accuracy against the plain loop."""
     lib = pyqcs.gpu_library()
     work = circuits.variational_batch(count, seed=seed)
     gates = sum(len(g) for _, g in work)
     b = pyqcs.Batch(lib)
     for n, g in work:
         b.add(n, g)
-    t0 = time.perf_counter()
-    # the autotuner times graph replays; a direct-launch (profiler) run takes the tuning
-    # given on the command line, or the defaults
-    if tuning is not None:
-        b.set_tuning(*tuning)
-        tuning = dict(zip(("lds_max_qubits", "block_size", "block_qubits", "streams", "stage_bits"), tuning))
-    elif direct:
-        tuning = {"default": True}
-    else:
-        tuning = b.autotune()
-    t_tune = time.perf_counter() - t0
-    b.run()
-    runs = []
-    for _ in range(5):
-        t0 = time.perf_counter()
-        z_batch, dev = b.run_direct() if direct else b.run()
-        runs.append((time.perf_counter() - t0, dev))
-    info = b.info()
-    traffic = b.traffic()
-    b.close()
-    wall = min(r[0] for r in runs)
-    dev = min(r[1] for r in runs)
-    gbps = traffic["hbm_bytes"] / dev / 1e9
-    log(f"batch leg: {count} circuits, {gates} gates, graph {dev * 1e3:.2f} ms device "
-        f"({wall * 1e3:.2f} ms wall), tuning {tuning} found in {t_tune:.1f}s")
-    log(f"batch leg: {traffic['hbm_bytes'] / 1e9:.1f} GB algorithmic HBM traffic in "
-        f"{traffic['kernel_launches']} launches: {gbps:.0f} GB/s over the whole run")
-    b = pyqcs.Batch(lib)
-    for n, g in work:
-        b.add(n, g)
     z_plain, t_plain = b.run_plain()
     b.close()
-    err = max(float(abs(x - y).max()) for x, y in zip(z_batch, z_plain))
-    log(f"batch leg: plain loop {t_plain:.3f}s, max |d<Z>| {err:.2e}")
-    return {"config": f"{count} variational circuits, 12-22 qubits, 4-16 layers RY+RZ on every "
-                      f"qubit + CNOT ladder, {gates} gates; <Z_i> of every qubit",
-            "circuits": count, "gates": gates, "graph_device_s": dev, "graph_wall_s": wall,
-            "plain_loop_s": t_plain, "speedup_wall": t_plain / wall,
-            "gates_per_s": gates / wall, "max_abs_diff_expz": err,
-            "tuning": tuning, "autotune_s": t_tune, "graph_nodes": info["graph_nodes"],
-            "hbm_bytes": traffic["hbm_bytes"], "kernel_launches": traffic["kernel_launches"],
-            "hbm_GBps_whole_run": gbps, "hbm_frac_whole_run": gbps / 8000.0,
-            "launch": "direct" if direct else "graph"}
+    log(f"batch leg: {count} circuits, {gates} gates; plain loop {t_plain:.3f}s")
+    legs = {}
+    for mode in precisions:
+        b = pyqcs.Batch(lib)
+        for n, g in work:
+            b.add(n, g)
+        b.set_precision(mode)
+        t0 = time.perf_counter()
+        # the autotuner times graph replays; a direct-launch (profiler) run takes the tuning
+        # given on the command line, or the defaults
+        if tuning is not None:
+            b.set_tuning(*tuning)
+            tun = dict(zip(("lds_max_qubits", "block_size", "block_qubits", "streams", "stage_bits"), tuning))
+        elif direct:
+            tun = {"default": True}
+        else:
+            tun = b.autotune()
+        t_tune = time.perf_counter() - t0
+        b.run()
+        runs = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            z_batch, dev = b.run_direct() if direct else b.run()
+            runs.append((time.perf_counter() - t0, dev))
+        info = b.info()
+        traffic = b.traffic()
+        b.close()
+        wall = min(r[0] for r in runs)
+        dev = min(r[1] for r in runs)
+        gbps = traffic["hbm_bytes"] / dev / 1e9
+        err = max(float(abs(x - y).max()) for x, y in zip(z_batch, z_plain))
+        name = {1: "FP32", 3: "BF16x3"}[mode]
+        log(f"batch leg ({name} MFMA): graph {dev * 1e3:.2f} ms device ({wall * 1e3:.2f} ms wall), "
+            f"{t_plain / wall:.1f}x the plain loop, max |d<Z>| {err:.2e}; tuning {tun} found in {t_tune:.1f}s")
+        log(f"batch leg ({name} MFMA): {traffic['hbm_bytes'] / 1e9:.1f} GB algorithmic HBM traffic in "
+            f"{traffic['kernel_launches']} launches: {gbps:.0f} GB/s over the whole run")
+        legs[mode] = {"mfma": name, "graph_device_s": dev, "graph_wall_s": wall,
+                      "speedup_wall": t_plain / wall, "gates_per_s": gates / wall,
+                      "max_abs_diff_expz": err, "tuning": tun, "autotune_s": t_tune,
+                      "graph_nodes": info["graph_nodes"], "hbm_bytes": traffic["hbm_bytes"],
+                      "kernel_launches": traffic["kernel_launches"], "hbm_GBps_whole_run": gbps,
+                      "hbm_frac_whole_run": gbps / 8000.0}
+    first = legs[precisions[0]]
+    out = {"config": f"{count} variational circuits, 12-22 qubits, 4-16 layers RY+RZ on every "
+                     f"qubit + CNOT ladder, {gates} gates; <Z_i> of every qubit",
+           "circuits": count, "gates": gates, "plain_loop_s": t_plain,
+           "launch": "direct" if direct else "graph"}
+    out.update(first)
+    if 3 in legs and precisions[0] != 3:
+        out["bf16x3"] = legs[3]
+    return out
 
 
 def planner_leg(n, record_qubits, record_count, depth, k):

@@ -142,6 +142,10 @@ int qcs_ext_batch_add_circuit(qcs_ext_batch *batch, bit_num_t num_qubits, int32_
  * on LDS tiles of 2^stage_bits amplitudes when they have at least stage_bits + 4 qubits
  * (stage_bits 11, 12 or 13; 0: one launch per block). */
 int qcs_ext_batch_set_tuning(qcs_ext_batch *batch, int lds_max_qubits, int block_size, int block_qubits, int streams, int stage_bits);
+/* Block products of the circuits in HBM: mode 1 = FP32 MFMA (the default), 3 = BF16x3
+ * split-precision MFMA (the modes of qcs_ext_set_precision; states complex64 and <Z> FP64
+ * either way).  The oracle computes in FP64 and only checks the mode. */
+int qcs_ext_batch_set_precision(qcs_ext_batch *batch, int mode);
 /* Searches the tuning parameters on this batch; writes the chosen 5 values to tuning. */
 int qcs_ext_batch_autotune(qcs_ext_batch *batch, int *tuning);
 /* expz receives sum(num_qubits) doubles, circuit by circuit; seconds = device time. */

@@ -1469,6 +1469,15 @@ extern "C"
         });
     }
     int qcs_ext_batch_set_tuning(qcs_ext_batch *b, int, int, int, int, int) { return guard([&] { if (!b) throw std::runtime_error("batch is NULL"); }); }
+    int qcs_ext_batch_set_precision(qcs_ext_batch *b, int mode)
+    {
+        return guard([&] {
+            if (!b)
+                throw std::runtime_error("batch is NULL");
+            if (mode != 1 && mode != 3)
+                throw std::runtime_error("batch: precision must be 1 or 3");
+        });
+    }
     int qcs_ext_batch_autotune(qcs_ext_batch *b, int *tuning)
     {
         return guard([&] {

@@ -133,6 +133,14 @@ namespace qcore
         tuning_ = t;
     }
 
+    void Batch::set_precision(int mode)
+    {
+        if (mode != 1 && mode != 3)
+            throw std::runtime_error("batch: precision must be 1 (FP32 MFMA) or 3 (BF16x3 MFMA)");
+        release();
+        x3_ = mode == 3;
+    }
+
     void Batch::prepare()
     {
         if (prepared_)
@@ -240,12 +248,12 @@ namespace qcore
                 {
                     Staged sc{ci, {}};
                     for (const auto &st : plan)
-                        sc.stages.push_back(prepare_stage(rg, st, kq, T.stage_bits, C.n, mats, false));
+                        sc.stages.push_back(prepare_stage(rg, st, kq, T.stage_bits, C.n, mats, x3_));
                     staged.push_back(std::move(sc));
                     continue;
                 }
             }
-            blocked.push_back(Blocked{ci, prepare_blocks(rg, plan_greedy(rg, kq), kq, C.n, mats)});
+            blocked.push_back(Blocked{ci, prepare_blocks(rg, plan_greedy(rg, kq), kq, C.n, mats, x3_)});
         }
         // largest first: round launches hand out the biggest states' tiles first, and with
         // sizes descending every state starts at a multiple of its own size
@@ -389,7 +397,7 @@ namespace qcore
         // or issued directly by run_direct
         work_ = [this, blocked = std::move(blocked), blocked_state, blocked_partial, rounds, nstaged, kq_staged,
                  stage_bits = T.stage_bits, zsum_offset, zsum_blocks_total, zsum_qubits_total, grid, nsmall,
-                 small_max_n, ns, block_size = T.block_size]() mutable {
+                 small_max_n, ns, block_size = T.block_size, x3 = x3_]() mutable {
             hipStream_t s0 = streams_[0];
             qsv::check(hipEventRecord(fork_join_[0], s0), "hipEventRecord");
             for (int i = 1; i < ns; ++i)
@@ -404,7 +412,7 @@ namespace qcore
             {
                 hipStream_t s = stream();
                 for (const auto &r : rounds)
-                    launch_stage_tasks(s, kq_staged, stage_bits, false, d_mats_, r.task_offset, r.ntasks, r.tiles);
+                    launch_stage_tasks(s, kq_staged, stage_bits, x3, d_mats_, r.task_offset, r.ntasks, r.tiles);
                 qsv::launch_zsum_tasks(s, reinterpret_cast<const qsv::ZsumTask *>(d_mats_ + zsum_offset), nstaged, zsum_blocks_total, zsum_qubits_total);
             }
             for (size_t i = 0; i < blocked.size(); ++i)
@@ -531,9 +539,11 @@ namespace qcore
         };
         try_set([](BatchTuning &t, int v) { t.lds_max_qubits = v; }, {12, 13, 14});
         try_set([](BatchTuning &t, int v) { t.block_size = v; }, {256, 512, 1024});
-        try_set([](BatchTuning &t, int v) { t.block_qubits = v; }, {4, 5});
+        // block size and stage tile size together: the better block size depends on the
+        // tile (FP32 products favour 4-qubit blocks, BF16x3 ones 5-qubit blocks on some tiles)
+        try_set([](BatchTuning &t, int v) { t.block_qubits = v / 16; t.stage_bits = v % 16; },
+                {4 * 16 + 0, 4 * 16 + 11, 4 * 16 + 12, 4 * 16 + 13, 5 * 16 + 0, 5 * 16 + 11, 5 * 16 + 12, 5 * 16 + 13});
         try_set([](BatchTuning &t, int v) { t.streams = v; }, {1, 2, 4});
-        try_set([](BatchTuning &t, int v) { t.stage_bits = v; }, {0, 11, 12, 13});
         set_tuning(best);
         return best;
     }

@@ -43,6 +43,11 @@ namespace qcore
         int total_qubits() const { return total_qubits_; }
         void set_tuning(const BatchTuning &t);
         const BatchTuning &tuning() const { return tuning_; }
+        // Block products of the circuits in HBM: 1 = FP32 MFMA (default), 3 = BF16x3
+        // split-precision MFMA (three bf16 products, FP32 accumulation).  States stay
+        // complex64 and <Z> FP64 either way; the LDS-resident circuits are unaffected.
+        void set_precision(int mode);
+        int precision() const { return x3_ ? 3 : 1; }
         // Runs every circuit from |0...0>, writes <Z_q> of circuit c at offset
         // sum_{c' < c} n_c' + q; returns seconds between graph launch and completion.
         double run(double *expz);
@@ -81,6 +86,7 @@ namespace qcore
         std::vector<Circuit> circuits_;
         int total_qubits_ = 0;
         BatchTuning tuning_;
+        bool x3_ = false;
         bool prepared_ = false;
         int device_ = 0;
         std::vector<hipStream_t> streams_;

@@ -590,6 +590,14 @@ extern "C"
             b->core->set_tuning(t);
         });
     }
+    int qcs_ext_batch_set_precision(qcs_ext_batch *b, int mode)
+    {
+        return guard([&] {
+            if (!b || !b->core)
+                throw std::runtime_error("batch is NULL");
+            b->core->set_precision(mode);
+        });
+    }
     int qcs_ext_batch_autotune(qcs_ext_batch *b, int *tuning)
     {
         return guard([&] {

@@ -55,6 +55,9 @@ namespace qsv
 #ifndef QSV_STAGE_WAVES_11
 #define QSV_STAGE_WAVES_11 1
 #endif
+#ifndef QSV_STAGE_WAVES_11_X3
+#define QSV_STAGE_WAVES_11_X3 1
+#endif
 // BF16x3 stage blocks issue the next column groups' LDS gathers before the current groups'
 // MFMA work, and take two column groups per pass with 5-qubit blocks (measured on 32-qubit
 // brick stages of ~5 blocks: 27.3 -> 25.9 ms; FP32 stages gain nothing from either)
@@ -1040,7 +1043,7 @@ namespace qsv
         auto go = [&](auto kern, int waves) { hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), 0, st, base, tasks, ntasks, total_tiles); };
         constexpr int W13 = QSV_STAGE_WAVES_13, W12 = QSV_STAGE_WAVES_12, W11 = QSV_STAGE_WAVES_11;
         constexpr int X13 = QSV_STAGE_WAVES_13_X3, X12 = QSV_STAGE_WAVES_12_X3;
-        constexpr int W14 = QSV_STAGE_WAVES_14, X14 = QSV_STAGE_WAVES_14_X3;
+        constexpr int W14 = QSV_STAGE_WAVES_14, X14 = QSV_STAGE_WAVES_14_X3, X11 = QSV_STAGE_WAVES_11_X3;
         if (kq == 5 && m == 14)
             x3 ? go(k_stage<5, 14, true, X14>, X14) : go(k_stage<5, 14, false, W14>, W14);
         else if (m == 14)
@@ -1050,13 +1053,13 @@ namespace qsv
         else if (kq == 5 && m == 12)
             x3 ? go(k_stage<5, 12, true, X12>, X12) : go(k_stage<5, 12, false, W12>, W12);
         else if (kq == 5)
-            x3 ? go(k_stage<5, 11, true, W11>, W11) : go(k_stage<5, 11, false, W11>, W11);
+            x3 ? go(k_stage<5, 11, true, X11>, X11) : go(k_stage<5, 11, false, W11>, W11);
         else if (m == 13)
             x3 ? go(k_stage<4, 13, true, X13>, X13) : go(k_stage<4, 13, false, W13>, W13);
         else if (m == 12)
             x3 ? go(k_stage<4, 12, true, X12>, X12) : go(k_stage<4, 12, false, W12>, W12);
         else
-            x3 ? go(k_stage<4, 11, true, W11>, W11) : go(k_stage<4, 11, false, W11>, W11);
+            x3 ? go(k_stage<4, 11, true, X11>, X11) : go(k_stage<4, 11, false, W11>, W11);
         check(hipGetLastError(), "stage kernel");
     }
 

@@ -148,6 +148,7 @@ class Library:
         lib.qcs_ext_batch_destroy.argtypes = [vp]
         lib.qcs_ext_batch_add_circuit.argtypes = [vp, bit_num_t, ctypes.c_int32, P(ctypes.c_char_p), P(bit_num_t), P(bit_num_t), P(ctypes.c_double)]
         lib.qcs_ext_batch_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.qcs_ext_batch_set_precision.argtypes = [vp, ctypes.c_int]
         lib.qcs_ext_batch_autotune.argtypes = [vp, P(ctypes.c_int)]
         lib.qcs_ext_batch_run.argtypes = [vp, vp, u64, P(ctypes.c_double)]
         lib.qcs_ext_batch_run_plain.argtypes = [vp, vp, u64, P(ctypes.c_double)]
@@ -494,6 +495,10 @@ class Batch:
 
     def set_tuning(self, lds_max_qubits=14, block_size=512, block_qubits=5, streams=4, stage_bits=12):
         self.L.check(self.lib.qcs_ext_batch_set_tuning(self.h, lds_max_qubits, block_size, block_qubits, streams, stage_bits), "batch_set_tuning")
+
+    def set_precision(self, mode):
+        """1: FP32 MFMA block products (default); 3: BF16x3 split-precision MFMA."""
+        self.L.check(self.lib.qcs_ext_batch_set_precision(self.h, mode), "batch_set_precision")
 
     def autotune(self):
         t = (ctypes.c_int * 5)()

@@ -38,6 +38,11 @@ def test_batch_rejects_bad_input(oracle):
         b.add(3, [("h", 3, (), -1)])  # target out of range
     with pytest.raises(pyqcs.QcsError):
         b.add(3, [("x", 1, (), 1)])  # control == target
+    b.set_precision(3)
+    b.set_precision(1)
+    for mode in (0, 2, 4):
+        with pytest.raises(pyqcs.QcsError):
+            b.set_precision(mode)
 
 
 @pytest.mark.gpu
@@ -93,6 +98,37 @@ def test_gpu_batch_stage_rounds(gpu, oracle):
     # rounds and per-block launches agree to complex64 rounding
     a, c = results[(14, 512, 5, 2, 11)], results[(14, 512, 5, 1, 0)]
     assert max(np.max(np.abs(x - y)) for x, y in zip(a, c)) < 2e-5
+
+
+@pytest.mark.gpu
+def test_gpu_batch_bf16x3(gpu, oracle):
+    # BF16x3 split-precision MFMA for the circuits in HBM: stage rounds (2^11 and 2^12 tiles)
+    # and per-block launches (stage_bits 0), against the oracle's FP64 plain method; the
+    # LDS-resident circuits (<= 12 or 14 qubits) take the same path in both modes
+    work = circuits.variational_batch(12, seed=31, nmin=10, nmax=18, dmin=1, dmax=6)
+    ref = pyqcs.Batch(oracle)
+    for n, g in work:
+        ref.add(n, g)
+    want, _ = ref.run()
+    for tuning in [(14, 512, 4, 2, 11), (14, 512, 5, 1, 12), (12, 256, 4, 2, 0), (12, 256, 5, 1, 0)]:
+        got = {}
+        for mode in (1, 3):
+            b = pyqcs.Batch(gpu)
+            for n, g in work:
+                b.add(n, g)
+            b.set_tuning(*tuning)
+            b.set_precision(mode)
+            got[mode], _ = b.run()
+            again, _ = b.run()
+            direct, _ = b.run_direct()
+            assert all(np.array_equal(x, y) for x, y in zip(got[mode], again))
+            assert all(np.array_equal(x, y) for x, y in zip(got[mode], direct))
+            b.close()
+        err = {m: max(np.max(np.abs(x - y)) for x, y in zip(got[m], want)) for m in (1, 3)}
+        print(f"tuning {tuning}: max |d<Z>| FP32 {err[1]:.2e} BF16x3 {err[3]:.2e}", flush=True)
+        assert err[1] < 2e-5 and err[3] < 5e-5, tuning
+        # the modes run different products on the circuits in HBM
+        assert any(not np.array_equal(x, y) for x, y in zip(got[1], got[3])), tuning
 
 
 @pytest.mark.gpu
