@@ -361,7 +361,8 @@ namespace qcore
                     continue;
                 }
                 float f[kPlanFeatures];
-                stage_features(n, math, o.k, st.tile, (int)plan_greedy(sub, o.k).size(), (int)st.gates.size(), f);
+                const int nb = f64_ ? stage128_lds_blocks(sub, o.k) : (int)plan_greedy(sub, o.k).size();
+                stage_features(n, math, o.k, st.tile, nb, (int)st.gates.size(), f);
                 X.insert(X.end(), f, f + kPlanFeatures);
             }
         }
@@ -505,7 +506,7 @@ namespace qcore
                         if (f64_)
                         {
                             L.st128 = prepare_stage128(gs, st, n, mats64, kb);
-                            nblocks = L.st128.nblocks;
+                            nblocks = stage128_lds_blocks(gs, kb);
                         }
                         else
                         {
@@ -1071,6 +1072,7 @@ namespace qcore
             sub.push_back(gates[gi]);
         const auto plan = plan_greedy(sub, kq);
         std::vector<qsv::StageBlock> desc;
+        std::vector<std::vector<std::complex<double>>> diagonals; // of the diagonal blocks
         for (const auto &pb : plan)
         {
             if (pb.wide)
@@ -1101,17 +1103,66 @@ namespace qcore
             for (int b = 0; b < m; ++b)
                 if (!is_block[b])
                     sb.w[j++] = 1 << b;
+            // products of diagonal gates have exactly zero off-diagonal entries
+            bool diagonal = true;
+            for (int r = 0; r < K && diagonal; ++r)
+                for (int c = 0; c < K && diagonal; ++c)
+                    diagonal = r == c || M[r * K + c] == std::complex<double>(0.0, 0.0);
+            if (diagonal)
+            {
+                sb.pad[0] = qsv::kStageBlockDiagonal;
+                std::vector<std::complex<double>> d(K);
+                for (int e = 0; e < K; ++e)
+                    d[e] = M[e * K + e];
+                diagonals.push_back(std::move(d));
+            }
             desc.push_back(sb);
         }
+        // Diagonal blocks after the last other block (all of them in an all-diagonal stage)
+        // become one table over the tile indices: their diagonals multiplied in block order
+        // (tile index bit b = tile qubit b = LDS weight 1 << b).  An all-diagonal stage runs
+        // as that table alone; otherwise the table is applied as the tile is stored.
+        size_t first_trail = desc.size();
+        while (first_trail > 0 && (desc[first_trail - 1].pad[0] & qsv::kStageBlockDiagonal))
+            --first_trail;
+        const size_t ntrail = desc.size() - first_trail;
         PreparedStage128 p;
         p.gates = (int)st.gates.size();
-        p.nblocks = (int)desc.size();
         p.bytes = 2.0 * 16.0 * double(uint64_t(1) << n);
+        size_t table = SIZE_MAX;
+        if (ntrail > 0)
+        {
+            while (mats.size() % 2)
+                mats.push_back(0.0);
+            table = mats.size();
+            const size_t d0 = diagonals.size() - ntrail; // the trailing blocks' diagonals
+            for (int idx = 0; idx < (1 << m); ++idx)
+            {
+                std::complex<double> v(1.0, 0.0);
+                for (size_t bi = first_trail; bi < desc.size(); ++bi)
+                {
+                    int e = 0;
+                    for (int jj = 0; jj < kq; ++jj)
+                        if (idx & desc[bi].w[jj])
+                            e |= 1 << jj;
+                    v *= diagonals[d0 + (bi - first_trail)][e];
+                }
+                mats.push_back(v.real());
+                mats.push_back(v.imag());
+            }
+            desc.resize(first_trail);
+        }
+        if (desc.empty())
+            p.table_offset = table;
+        else
+            p.trail_offset = table;
+        p.nblocks = (int)desc.size();
         while (mats.size() % 2)
             mats.push_back(0.0);
         p.desc_offset = mats.size();
         mats.resize(mats.size() + desc.size() * sizeof(qsv::StageBlock) / sizeof(double));
-        std::memcpy(mats.data() + p.desc_offset, desc.data(), desc.size() * sizeof(qsv::StageBlock));
+        if (!desc.empty())
+            std::memcpy(mats.data() + p.desc_offset, desc.data(), desc.size() * sizeof(qsv::StageBlock));
         p.tsub.set_bits = 0;
         p.tsub.nops = m;
         p.op.m = m;
@@ -1125,10 +1176,26 @@ namespace qcore
         return p;
     }
 
+    int stage128_lds_blocks(const std::vector<ResolvedGate> &gates, int kq)
+    {
+        const auto plan = plan_greedy(gates, kq);
+        int last = 0;
+        for (int i = 0; i < (int)plan.size(); ++i)
+            for (int gi : plan[i].gates)
+            {
+                const Kind kd = gates[gi].spec.kind;
+                if (gates[gi].mmask != 0 || !(kd == Kind::GlobalPhase || kd == Kind::PhaseOne || kd == Kind::Diag1 || kd == Kind::Diag2 || kd == Kind::Identity))
+                    last = i + 1;
+            }
+        return last;
+    }
+
     void launch_prepared_stage128(hipStream_t s, qsv::c128 *state, int n, PreparedStage128 &p, const double *mats_dev)
     {
         p.op.M = mats_dev;
         p.op.blocks = reinterpret_cast<const qsv::StageBlock *>(mats_dev + p.desc_offset);
+        p.op.table = p.table_offset == SIZE_MAX ? nullptr : mats_dev + p.table_offset;
+        p.op.trail = p.trail_offset == SIZE_MAX ? nullptr : mats_dev + p.trail_offset;
         qsv::launch_stage128(s, state, n, p.tsub, p.op);
     }
 

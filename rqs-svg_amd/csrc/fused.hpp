@@ -131,10 +131,17 @@ namespace qcore
         int nblocks = 0;
         double bytes = 0;
         size_t desc_offset = 0; // doubles: StageBlock array
+        // doubles: the product table of an all-diagonal stage (2^12 complex), or none
+        size_t table_offset = SIZE_MAX;
+        // doubles: the product table of the diagonal blocks after the last other block, or none
+        size_t trail_offset = SIZE_MAX;
         qsv::Subspace tsub{};
         qsv::Stage128Op op{};
     };
     PreparedStage128 prepare_stage128(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int n, std::vector<double> &mats, int kq = 4);
+    // Blocks a complex128 stage over these gates runs through LDS: its greedy blocks up to
+    // the last one with a non-diagonal gate (the diagonal blocks after it fold into a table).
+    int stage128_lds_blocks(const std::vector<ResolvedGate> &gates, int kq);
     void launch_prepared_stage128(hipStream_t s, qsv::c128 *state, int n, PreparedStage128 &p, const double *mats_dev);
 
     class CostModel;

@@ -915,6 +915,33 @@ namespace qsv
                 {
                     const __attribute__((address_space(4))) StageBlock &B = as_constant(op.blocks)[bi];
                     const __attribute__((address_space(4))) double *Mx = as_constant(op.M) + B.matrix;
+                    if (B.pad[0] & kStageBlockDiagonal) // uniform: K products per column, not K^2
+                    {
+#pragma unroll
+                        for (int cc = 0; cc < PER_THREAD; ++cc)
+                        {
+                            const int col = t + cc * kThreads;
+                            int cbase = 0;
+#pragma unroll
+                            for (int j = 0; j < kStage128Bits - KQ; ++j)
+                                if ((col >> j) & 1)
+                                    cbase += B.w[KQ + j];
+#pragma unroll
+                            for (int e = 0; e < K; ++e)
+                            {
+                                int off = cbase;
+#pragma unroll
+                                for (int j = 0; j < KQ; ++j)
+                                    if ((e >> j) & 1)
+                                        off += B.w[j];
+                                const v2d x = tile[swz128(off)];
+                                const double mr = Mx[2 * (K * e + e)], mi = Mx[2 * (K * e + e) + 1];
+                                tile[swz128(off)] = v2d{__builtin_fma(mr, x.x, -mi * x.y), __builtin_fma(mr, x.y, mi * x.x)};
+                            }
+                        }
+                        lds_barrier();
+                        continue;
+                    }
 #pragma unroll
                     for (int cc = 0; cc < PER_THREAD; ++cc)
                     {
@@ -956,10 +983,68 @@ namespace qsv
                     }
                     lds_barrier();
                 }
+                if (op.trail != nullptr) // uniform: the trailing diagonal blocks' product
+                {
+                    v2d ph[LOADS];
+#pragma unroll
+                    for (int i = 0; i < LOADS; ++i)
+                        ph[i] = *reinterpret_cast<const v2d *>(op.trail + 2 * (t + i * kThreads));
+#pragma unroll
+                    for (int i = 0; i < LOADS; ++i)
+                    {
+                        const v2d x = tile[swz128(t + i * kThreads)];
+                        const v2d y{__builtin_fma(ph[i].x, x.x, -ph[i].y * x.y), __builtin_fma(ph[i].x, x.y, ph[i].y * x.x)};
+                        __builtin_nontemporal_store(y, reinterpret_cast<v2d *>(s + (base + tile_offset(i))));
+                    }
+                }
+                else
+                {
+#pragma unroll
+                    for (int i = 0; i < LOADS; ++i)
+                        __builtin_nontemporal_store(tile[swz128(t + i * kThreads)], reinterpret_cast<v2d *>(s + (base + tile_offset(i))));
+                }
+                lds_barrier();
+            }
+        }
+        // complex128 stage whose blocks are all diagonal: their product is one table over the
+        // 2^12 tile indices, so each amplitude is multiplied by its entry in registers.  Tiles
+        // are addressed as in k_stage128 (thread t takes tile indices t + i * kThreads), so a
+        // thread's table entries are the same for every tile and stay in registers.
+        __global__ __launch_bounds__(kThreads) void k_stage128_table(c128 *__restrict__ s, uint64_t ntiles, Subspace tsub, Stage128Op op)
+        {
+            typedef double v2d __attribute__((ext_vector_type(2)));
+            constexpr int LOADS = (1 << kStage128Bits) / kThreads;
+            const int t = threadIdx.x;
+            uint64_t g_t = 0;
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+                if ((t >> b) & 1)
+                    g_t += op.gw[b];
+            uint64_t g_i[LOADS];
+            v2d ph[LOADS];
+#pragma unroll
+            for (int i = 0; i < LOADS; ++i)
+            {
+                g_i[i] = 0;
+#pragma unroll
+                for (int b = 8; b < kStage128Bits; ++b)
+                    if (((i * kThreads) >> b) & 1)
+                        g_i[i] += op.gw[b];
+                ph[i] = *reinterpret_cast<const v2d *>(op.table + 2 * (t + i * kThreads));
+            }
+            for (uint64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x)
+            {
+                const uint64_t base = expand(tl, tsub) + g_t;
+                v2d a[LOADS];
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
-                    __builtin_nontemporal_store(tile[swz128(t + i * kThreads)], reinterpret_cast<v2d *>(s + (base + tile_offset(i))));
-                lds_barrier();
+                    a[i] = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(s + (base + g_i[i])));
+#pragma unroll
+                for (int i = 0; i < LOADS; ++i)
+                {
+                    const v2d y{__builtin_fma(ph[i].x, a[i].x, -ph[i].y * a[i].y), __builtin_fma(ph[i].x, a[i].y, ph[i].y * a[i].x)};
+                    __builtin_nontemporal_store(y, reinterpret_cast<v2d *>(s + (base + g_i[i])));
+                }
             }
         }
     } // namespace
@@ -1083,9 +1168,17 @@ namespace qsv
                 throw std::runtime_error("stage128: tile weights do not match the tile qubits");
         if (tsub.set_bits != 0)
             throw std::runtime_error("stage128: tiles take no controls");
+        const uint64_t ntiles = uint64_t(1) << (n - op.m);
+        if (op.table != nullptr)
+        {
+            // a streaming pass: enough workgroups to fill the chip several times over
+            const uint64_t grid = ntiles < 16384 ? ntiles : 16384;
+            hipLaunchKernelGGL(k_stage128_table, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+            check(hipGetLastError(), "stage128 table kernel");
+            return;
+        }
         if (op.nblocks < 1 || op.blocks == nullptr || op.M == nullptr)
             throw std::runtime_error("stage128: no blocks");
-        const uint64_t ntiles = uint64_t(1) << (n - op.m);
         const uint64_t grid = ntiles < 8192 ? ntiles : 8192;
         if (op.kq == 4)
             hipLaunchKernelGGL(k_stage128<4>, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);

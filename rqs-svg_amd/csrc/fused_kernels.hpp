@@ -72,7 +72,13 @@ namespace qsv
 
     // complex128 stage: tile of 2^12 amplitudes, blocks of kq = 3 or 4 qubits in FP64;
     // blocks[i].matrix is the offset in doubles of the 2^kq x 2^kq complex matrix (row-major,
-    // interleaved) from M; w[0..kq-1] block bits, w[kq..11] column bits.
+    // interleaved) from M; w[0..kq-1] block bits, w[kq..11] column bits; blocks[i].pad[0] = 1
+    // marks a diagonal matrix (only its diagonal is applied).
+    // table != nullptr: every block of the stage is diagonal and table holds their product,
+    // 2^12 complex values indexed by the tile index (bit b = tile qubit b); the kernel then
+    // only multiplies each amplitude by its entry (no blocks, no LDS).
+    // trail != nullptr: a table of the same form for diagonal blocks that follow the
+    // nblocks blocks, applied to each amplitude as the tile is stored.
     struct Stage128Op
     {
         int m;
@@ -81,7 +87,10 @@ namespace qsv
         uint64_t gw[16];
         const StageBlock *blocks;
         const double *M;
+        const double *table;
+        const double *trail;
     };
+    constexpr uint32_t kStageBlockDiagonal = 1;
     void launch_stage128(hipStream_t s, c128 *state, int n, const Subspace &tsub, const Stage128Op &op);
 
     // complex128 block on 4 qubits in FP64.  M: 16x16 complex, row-major, interleaved

@@ -267,6 +267,32 @@ def test_stages_complex128_match_oracle(gpu, oracle, k):
         assert gates == len(circ)
 
 
+@pytest.mark.parametrize("k", [3, 4])
+def test_stages_complex128_diagonal(gpu, oracle, k):
+    # diagonal gates only (controlled phases, rz, rzz, z, s, t, ... with positive and negative
+    # controls, negative controls included): every stage is diagonal and runs as one product
+    # table over the tile; then a QFT on a random state, whose controlled-phase runs give
+    # all-diagonal stages between stages with dense blocks.  FP64 bars against the oracle.
+    n = 16
+    names = [nm for nm in sorted(pyqcs.GATE_TARGETS) if nm in ("z", "s", "sdg", "t", "tdg", "p", "u1", "rz", "rzz", "global_phase")]
+    circ = [g for g in circuits.random_all_gates(n, 4000, seed=40 + k, max_ctrls=2) if g[0] in names][:300]
+    assert len(circ) == 300
+    psi = random_state(n, 9)
+    for c in (circ, circuits.qft(n)):
+        with pyqcs.Simulator(gpu, n, precision=2, block_qubits=k) as f:
+            f.set_statevector(psi)
+            f.run(c)
+            got = f.statevector()
+            gates, launches = f.fusion_stats()
+        with pyqcs.Simulator(oracle, n) as o:
+            o.set_statevector(psi)
+            o.run(c)
+            want = o.statevector()
+        say(f"c128 diagonal stages n={n} k={k}: max|d| {np.max(np.abs(got - want)):.2e} gates {gates} launches {launches}")
+        assert np.max(np.abs(got - want)) <= 1e-13
+        assert gates == len(c)
+
+
 @pytest.mark.parametrize("precision", [1, 3])
 def test_stage_launch_chunks(gpu, oracle, precision):
     # deep circuits on few qubits give stages with more blocks than one launch's LDS holds
