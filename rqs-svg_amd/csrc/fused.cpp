@@ -422,12 +422,30 @@ namespace qcore
             if (n >= m + 4)
                 tiles.push_back(m);
         // 1..24 random gates (U3, or CX/CZ with probability 0.3) on the qubit set qs
-        auto random_gates = [&](const std::vector<int> &qs, std::vector<ResolvedGate> &out) {
+        // diagonal: controlled phases and rz only (complex128 stages fold such blocks into
+        // phase tables, much cheaper than dense blocks)
+        auto random_gates = [&](const std::vector<int> &qs, std::vector<ResolvedGate> &out, bool diagonal = false) {
             const int ng = 1 + int(rnd() % 24);
             for (int g = 0; g < ng; ++g)
             {
                 ResolvedGate r;
-                if (qs.size() >= 2 && unif() < 0.3)
+                if (diagonal)
+                {
+                    double prm[1] = {6.283 * unif()};
+                    const int a = qs[rnd() % qs.size()];
+                    r.tphys = {a};
+                    if (qs.size() >= 2 && unif() < 0.7)
+                    {
+                        int b = a;
+                        while (b == a)
+                            b = qs[rnd() % qs.size()];
+                        r.spec = make_gate("p", prm);
+                        r.pphys = {b};
+                    }
+                    else
+                        r.spec = make_gate("rz", prm);
+                }
+                else if (qs.size() >= 2 && unif() < 0.3)
                 {
                     double none[1] = {0.0};
                     r.spec = make_gate(unif() < 0.5 ? "z" : "x", none);
@@ -497,8 +515,11 @@ namespace qcore
                         std::sort(st.tile.begin(), st.tile.end());
                         std::vector<ResolvedGate> gs;
                         const int nb = 1 + int(rnd() % 12);
+                        // complex128: a quarter of the stages all diagonal, a quarter ending
+                        // in a run of diagonal blocks
+                        const int shape = f64_ ? int(rnd() % 4) : 3;
                         for (int b = 0; b < nb; ++b)
-                            random_gates(random_subset(st.tile, 1 + int(rnd() % kb)), gs);
+                            random_gates(random_subset(st.tile, 1 + int(rnd() % kb)), gs, shape == 0 || (shape == 1 && 2 * b >= nb));
                         st.gates.resize(gs.size());
                         std::iota(st.gates.begin(), st.gates.end(), 0);
                         L.tile_bits = m;
