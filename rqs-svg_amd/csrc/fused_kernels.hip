@@ -1007,43 +1007,47 @@ namespace qsv
             }
         }
         // complex128 stage whose blocks are all diagonal: their product is one table over the
-        // 2^12 tile indices, so each amplitude is multiplied by its entry in registers.  Tiles
-        // are addressed as in k_stage128 (thread t takes tile indices t + i * kThreads), so a
-        // thread's table entries are the same for every tile and stay in registers.
-        __global__ __launch_bounds__(kThreads) void k_stage128_table(c128 *__restrict__ s, uint64_t ntiles, Subspace tsub, Stage128Op op)
+        // 2^12 tile indices, and each amplitude is multiplied by its entry.  The state is
+        // streamed in order (no tiles): amplitude a's tile index is a's bits 0..3 (tile qubits
+        // 0..3) and its bits hi[0..7] (the other tile qubits) as bits 4..11.  Each workgroup
+        // copies the 64 KiB table into LDS once; the 8 XCDs each stream one contiguous eighth.
+        // 1024-thread workgroups, two per CU (LDS), keep as many loads in flight as the pair
+        // kernels' 256-thread ones.
+        constexpr int kTablePer = 4, kTableThreads = 1024;
+        __global__ __launch_bounds__(kTableThreads) void k_stage128_table(c128 *__restrict__ s, uint64_t count, Subspace tsub, const double *__restrict__ table)
         {
             typedef double v2d __attribute__((ext_vector_type(2)));
-            constexpr int LOADS = (1 << kStage128Bits) / kThreads;
-            const int t = threadIdx.x;
-            uint64_t g_t = 0;
+            constexpr int kThreads = kTableThreads;
+            __shared__ __attribute__((aligned(16))) v2d tab[1 << kStage128Bits];
+            for (int i = threadIdx.x; i < (1 << kStage128Bits); i += kThreads)
+                tab[i] = *reinterpret_cast<const v2d *>(table + 2 * i);
+            __syncthreads();
+            int hi[kStage128Bits - 4];
 #pragma unroll
-            for (int b = 0; b < 8; ++b)
-                if ((t >> b) & 1)
-                    g_t += op.gw[b];
-            uint64_t g_i[LOADS];
-            v2d ph[LOADS];
-#pragma unroll
-            for (int i = 0; i < LOADS; ++i)
+            for (int b = 0; b < kStage128Bits - 4; ++b)
+                hi[b] = tsub.ops[4 + b];
+            const uint64_t per_block = uint64_t(kThreads) * kTablePer;
+            const uint64_t nchunks = count / per_block;
+            uint64_t nb = gridDim.x, b0 = blockIdx.x;
+            if ((nb & 7) == 0)
+                b0 = (b0 & 7) * (nb >> 3) + (b0 >> 3);
+            for (uint64_t c = b0; c < nchunks; c += nb)
             {
-                g_i[i] = 0;
+                v2d a[kTablePer];
 #pragma unroll
-                for (int b = 8; b < kStage128Bits; ++b)
-                    if (((i * kThreads) >> b) & 1)
-                        g_i[i] += op.gw[b];
-                ph[i] = *reinterpret_cast<const v2d *>(op.table + 2 * (t + i * kThreads));
-            }
-            for (uint64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x)
-            {
-                const uint64_t base = expand(tl, tsub) + g_t;
-                v2d a[LOADS];
+                for (int j = 0; j < kTablePer; ++j)
+                    a[j] = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(s + (c * per_block + uint64_t(j) * kThreads + threadIdx.x)));
 #pragma unroll
-                for (int i = 0; i < LOADS; ++i)
-                    a[i] = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(s + (base + g_i[i])));
-#pragma unroll
-                for (int i = 0; i < LOADS; ++i)
+                for (int j = 0; j < kTablePer; ++j)
                 {
-                    const v2d y{__builtin_fma(ph[i].x, a[i].x, -ph[i].y * a[i].y), __builtin_fma(ph[i].x, a[i].y, ph[i].y * a[i].x)};
-                    __builtin_nontemporal_store(y, reinterpret_cast<v2d *>(s + (base + g_i[i])));
+                    const uint64_t i = c * per_block + uint64_t(j) * kThreads + threadIdx.x;
+                    uint32_t e = uint32_t(i & 15);
+#pragma unroll
+                    for (int b = 0; b < kStage128Bits - 4; ++b)
+                        e |= uint32_t((i >> hi[b]) & 1) << (4 + b);
+                    const v2d ph = tab[e];
+                    const v2d y{__builtin_fma(ph.x, a[j].x, -ph.y * a[j].y), __builtin_fma(ph.x, a[j].y, ph.y * a[j].x)};
+                    __builtin_nontemporal_store(y, reinterpret_cast<v2d *>(s + i));
                 }
             }
         }
@@ -1171,9 +1175,11 @@ namespace qsv
         const uint64_t ntiles = uint64_t(1) << (n - op.m);
         if (op.table != nullptr)
         {
-            // a streaming pass: enough workgroups to fill the chip several times over
-            const uint64_t grid = ntiles < 16384 ? ntiles : 16384;
-            hipLaunchKernelGGL(k_stage128_table, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+            // a streaming pass in amplitude order; 2^n is a multiple of the chunk (n > 12)
+            const uint64_t count = uint64_t(1) << n;
+            const uint64_t chunks = count / (uint64_t(kTableThreads) * kTablePer);
+            const uint64_t grid = chunks < 4096 ? chunks : 4096;
+            hipLaunchKernelGGL(k_stage128_table, dim3(grid), dim3(kTableThreads), 0, st, state, count, tsub, op.table);
             check(hipGetLastError(), "stage128 table kernel");
             return;
         }
