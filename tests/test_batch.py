@@ -168,6 +168,13 @@ def test_gpu_batch_autotune(gpu):
     assert t["stage_bits"] in (0, 11, 12, 13)
     after, _ = b.run()
     assert max(np.max(np.abs(x - y)) for x, y in zip(before, after)) < 2e-5
+    # the search within BF16x3 products
+    b.set_precision(3)
+    t3 = b.autotune()
+    assert t3["block_qubits"] in (4, 5) and t3["stage_bits"] in (0, 11, 12, 13)
+    fast, _ = b.run()
+    assert max(np.max(np.abs(x - y)) for x, y in zip(before, fast)) < 5e-5
+    b.close()
 
 
 @pytest.mark.gpu

@@ -273,12 +273,12 @@ def test_stages_complex128_diagonal(gpu, oracle, k):
     # controls, negative controls included): every stage is diagonal and runs as one product
     # table over the tile; then a QFT on a random state, whose controlled-phase runs give
     # all-diagonal stages between stages with dense blocks.  FP64 bars against the oracle.
-    n = 16
     names = [nm for nm in sorted(pyqcs.GATE_TARGETS) if nm in ("z", "s", "sdg", "t", "tdg", "p", "u1", "rz", "rzz", "global_phase")]
-    circ = [g for g in circuits.random_all_gates(n, 4000, seed=40 + k, max_ctrls=2) if g[0] in names][:300]
-    assert len(circ) == 300
-    psi = random_state(n, 9)
-    for c in (circ, circuits.qft(n)):
+    for n, c in ((16, None), (17, None), (16, circuits.qft(16))):
+        if c is None:
+            c = [g for g in circuits.random_all_gates(n, 4000, seed=40 + k + n, max_ctrls=2) if g[0] in names][:300]
+            assert len(c) == 300
+        psi = random_state(n, 9)
         with pyqcs.Simulator(gpu, n, precision=2, block_qubits=k) as f:
             f.set_statevector(psi)
             f.run(c)
@@ -289,6 +289,7 @@ def test_stages_complex128_diagonal(gpu, oracle, k):
             o.run(c)
             want = o.statevector()
         say(f"c128 diagonal stages n={n} k={k}: max|d| {np.max(np.abs(got - want)):.2e} gates {gates} launches {launches}")
+        assert launches < len(c) // 4  # fused into stages, not run gate by gate
         assert np.max(np.abs(got - want)) <= 1e-13
         assert gates == len(c)
 
