@@ -89,6 +89,41 @@ def cpu_baseline(n, circ, budget_s=20.0):
     }
 
 
+def small_leg(n=20, layers=20, shots=1024):
+    """The reference's CPU-runnable case on one GPU against the oracle on the host cores:
+    20 qubits complex128, H then a CNOT chain (GHZ), 20 layers of U3 on every qubit and CZ
+    on a random perfect matching, 1024 samples.  Both run the same API calls (one per gate);
+    the timed region is the circuit from |0...0> plus the sampling, states allocated."""
+    circ = circuits.ghz_u3_cz(n, layers, seed=CIRCUIT_SEED + 11)
+    res = {}
+    for name, lib in (("gpu", pyqcs.gpu_library()), ("cpu", pyqcs.oracle_library())):
+        with pyqcs.Simulator(lib, n, seed=1) as s:
+            s.run(circ[:n])  # first launches / first touch
+            best = None
+            for _ in range(3 if name == "gpu" else 1):
+                s.call("set_zero_state")
+                s.norm()
+                t0 = time.perf_counter()
+                s.run(circ)
+                smp = s.sample(shots, seed=SAMPLE_SEED)
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            res[name] = (best, s.statevector(), smp)
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    (tg, psi_g, s_g), (tc, psi_c, s_c) = res["gpu"], res["cpu"]
+    diff = float(np.max(np.abs(psi_g - psi_c)))
+    same = int(np.sum(s_g == s_c))
+    log(f"small leg: {n} q c128, {len(circ)} gates + {shots} samples: GPU {tg * 1e3:.1f} ms, "
+        f"CPU oracle {tc * 1e3:.1f} ms on {cores} cores ({tc / tg:.1f}x); max|d amp| {diff:.1e}, "
+        f"{same}/{shots} samples identical")
+    return {"config": f"{n} qubits complex128: H + CNOT chain, {layers} layers U3 on all + CZ on a "
+                      f"random matching, {shots} samples (plain method, one kernel per gate)",
+            "gates": len(circ), "shots": shots, "gpu_s": tg, "gpu_gates_per_s": len(circ) / tg,
+            "cpu_oracle_s": tc, "cpu_cores": cores, "gpu_vs_cpu": tc / tg,
+            "max_abs_diff_amp": diff, "samples_identical": same}
+
+
 FUSED_VARIANTS = (
     # (result key, precision mode, stage tile bits)
     ("fused_c64", 1, 13),
@@ -332,6 +367,7 @@ def main():
                     help="issue the batch's launches directly instead of one graph launch "
                          "(same kernels; for profilers that do not follow graph launches)")
     ap.add_argument("--no-planner", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the 20-qubit GPU-vs-CPU leg")
     ap.add_argument("--planner-qubits", type=int, default=33)
     ap.add_argument("--planner-depth", type=int, default=40)
     ap.add_argument("--planner-k", type=int, default=4)
@@ -431,6 +467,8 @@ def main():
         "sweeps_per_s": alg_bytes / (2 * 16 * (1 << n)) * args.steps / wall,
         "hbm_frac_whole_step": alg_bytes * args.steps / wall / HBM_PEAK,
     }
+    if not args.no_small:
+        result["small_20q"] = small_leg()
     if not args.no_fused:
         result.update(fused_leg(args.fused_qubits, args.fused_depth, args.fused_k, args.shots))
     if not args.no_batch:
