@@ -105,9 +105,10 @@ namespace qsv
             v2f f = *reinterpret_cast<const v2f *>(p);
             return v2d{double(f.x), double(f.y)};
         }
-        // Gate-kernel amplitude access: non-temporal on large states, cached on states that
-        // fit the 256 MB memory-side cache, which then stay there from gate to gate (measured
-        // on 24 qubits, 256 MiB: pair kernel 95 -> 77 us; on 30 qubits non-temporal is faster)
+        // Gate-kernel amplitude access: cached on states of 64-256 MiB, which then stay in the
+        // 256 MB memory-side cache from gate to gate (measured on 24 qubits, 256 MiB: pair
+        // kernel 95 -> 77 us), non-temporal on larger states (faster on 30 qubits) and on
+        // launch-bound small ones (see use_nt)
         template <bool NT>
         __device__ __forceinline__ v2d ldx(const c128 *p)
         {
