@@ -126,6 +126,21 @@ def test_random_circuit_all_gates(gpu, oracle):
     assert fidelity(g, o) >= 1 - FID_TOL
 
 
+@pytest.mark.parametrize("n", [22, 24])
+def test_cached_size_all_gates(gpu, oracle, n):
+    # 2^22 and 2^24 amplitudes (64 and 256 MiB): the gate kernels' cached-access
+    # instantiations (pair, low-target pair, phase, low-operand phase, diagonal); low and
+    # high targets and controls, on a random state
+    circ = circuits.random_all_gates(n, 120, seed=60 + n, max_ctrls=3)
+    circ += [("h", (t,), (), (), ()) for t in (0, 3, 5, 12, n - 1)]
+    circ += [("z", (1,), (), (0,), ()), ("s", (2,), (), (n - 1,), ()), ("rz", (n - 2,), (0.4,), (), ())]
+    psi = random_state(n, 3)
+    say(f"cached-size circuit n={n}")
+    g, o = run_both(gpu, oracle, n, circ, psi=psi)
+    assert np.max(np.abs(g - o)) <= AMP_TOL
+    assert fidelity(g, o) >= 1 - FID_TOL
+
+
 def test_config_ghz_u3_cz_20q(gpu, oracle):
     # 20 qubits: H, CNOT chain, 20 layers of U3 + CZ matching; statevector and 1024 samples
     n = 20
