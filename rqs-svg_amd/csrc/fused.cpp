@@ -361,7 +361,7 @@ namespace qcore
                     continue;
                 }
                 float f[kPlanFeatures];
-                const int nb = f64_ ? stage128_lds_blocks(sub, o.k) : (int)plan_greedy(sub, o.k).size();
+                const int nb = f64_ ? stage128_lds_blocks(sub, o.k) : stage64_lds_blocks(sub, o.k);
                 stage_features(n, math, o.k, st.tile, nb, (int)st.gates.size(), f);
                 X.insert(X.end(), f, f + kPlanFeatures);
             }
@@ -422,8 +422,8 @@ namespace qcore
             if (n >= m + 4)
                 tiles.push_back(m);
         // 1..24 random gates (U3, or CX/CZ with probability 0.3) on the qubit set qs
-        // diagonal: controlled phases and rz only (complex128 stages fold such blocks into
-        // phase tables, much cheaper than dense blocks)
+        // diagonal: controlled phases and rz only (stages fold such blocks into phase
+        // tables, much cheaper than dense blocks)
         auto random_gates = [&](const std::vector<int> &qs, std::vector<ResolvedGate> &out, bool diagonal = false) {
             const int ng = 1 + int(rnd() % 24);
             for (int g = 0; g < ng; ++g)
@@ -515,11 +515,11 @@ namespace qcore
                         std::sort(st.tile.begin(), st.tile.end());
                         std::vector<ResolvedGate> gs;
                         const int nb = 1 + int(rnd() % 12);
-                        // complex128: a quarter of the stages all diagonal, a quarter ending
-                        // in a run of diagonal blocks
-                        const int shape = f64_ ? int(rnd() % 4) : 3;
+                        // a quarter of the stages all diagonal (phase tables); complex128: a
+                        // quarter ending in a run of diagonal blocks (a table at the store)
+                        const int shape = int(rnd() % 4);
                         for (int b = 0; b < nb; ++b)
-                            random_gates(random_subset(st.tile, 1 + int(rnd() % kb)), gs, shape == 0 || (shape == 1 && 2 * b >= nb));
+                            random_gates(random_subset(st.tile, 1 + int(rnd() % kb)), gs, shape == 0 || (f64_ && shape == 1 && 2 * b >= nb));
                         st.gates.resize(gs.size());
                         std::iota(st.gates.begin(), st.gates.end(), 0);
                         L.tile_bits = m;
@@ -531,9 +531,9 @@ namespace qcore
                         }
                         else
                         {
-                            L.st = prepare_stage(gs, st, kb, m, n, mats, x3_);
+                            L.st = prepare_stage(gs, st, kb, m, n, mats, x3_, true);
                             attach_stage(L.st, state_, mats);
-                            nblocks = L.st.nblocks;
+                            nblocks = stage64_lds_blocks(gs, kb);
                         }
                         stage_features(n, math, kb, st.tile, nblocks, (int)gs.size(), L.f);
                         continue;
@@ -947,7 +947,7 @@ namespace qcore
         return window.size() < grow.size() ? window : grow;
     }
 
-    PreparedStage prepare_stage(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int kq, int m, int n, std::vector<float> &mats, bool x3)
+    PreparedStage prepare_stage(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int kq, int m, int n, std::vector<float> &mats, bool x3, bool table)
     {
         if ((int)st.tile.size() != m)
             throw std::runtime_error("stage: tile size mismatch");
@@ -960,6 +960,8 @@ namespace qcore
         const size_t per = size_t(D) * D;
         const int K = 1 << kq;
         std::vector<qsv::StageBlock> desc;
+        std::vector<std::vector<cd>> diagonals; // while every block so far is diagonal
+        bool all_diagonal = table;
         for (const auto &pb : plan)
         {
             if (pb.wide)
@@ -971,6 +973,17 @@ namespace qcore
                 if (local_pos(lay.qubits, st.tile[i]) < 0)
                     lay.qubits.push_back(st.tile[i]);
             const auto M = block_matrix(sub, pb, lay);
+            if (all_diagonal)
+            {
+                // products of diagonal gates have exactly zero off-diagonal entries
+                for (int r = 0; r < K && all_diagonal; ++r)
+                    for (int c = 0; c < K && all_diagonal; ++c)
+                        all_diagonal = r == c || M[size_t(r) * K + c] == cd(0.0, 0.0);
+                std::vector<cd> d(K);
+                for (int e = 0; e < K; ++e)
+                    d[e] = M[size_t(e) * K + e];
+                diagonals.push_back(std::move(d));
+            }
             std::vector<double> R(per);
             for (int r = 0; r < K; ++r)
                 for (int c = 0; c < K; ++c)
@@ -1028,7 +1041,39 @@ namespace qcore
         p.tsub.nops = m;
         for (int b = 0; b < m; ++b)
             p.tsub.ops[b] = st.tile[b];
+        if (all_diagonal && !desc.empty())
+        {
+            // tile index bit b = tile qubit b = the blocks' weight 1 << b; block order
+            while (mats.size() % 4)
+                mats.push_back(0.f);
+            p.table_offset = mats.size();
+            for (int idx = 0; idx < (1 << m); ++idx)
+            {
+                cd v(1.0, 0.0);
+                for (size_t bi = 0; bi < desc.size(); ++bi)
+                {
+                    int e = 0;
+                    for (int jj = 0; jj < kq; ++jj)
+                        if (idx & desc[bi].w[jj])
+                            e |= 1 << jj;
+                    v *= diagonals[bi][e];
+                }
+                mats.push_back(float(v.real()));
+                mats.push_back(float(v.imag()));
+            }
+        }
         return p;
+    }
+
+    int stage64_lds_blocks(const std::vector<ResolvedGate> &gates, int kq)
+    {
+        for (const auto &g : gates)
+        {
+            const Kind kd = g.spec.kind;
+            if (g.mmask != 0 || !(kd == Kind::GlobalPhase || kd == Kind::PhaseOne || kd == Kind::Diag1 || kd == Kind::Diag2 || kd == Kind::Identity))
+                return (int)plan_greedy(gates, kq).size();
+        }
+        return 0;
     }
 
     qsv::StageTask stage_task(const PreparedStage &p, qsv::c64 *state, uint64_t tile_begin)
@@ -1063,13 +1108,17 @@ namespace qcore
     void attach_stage(PreparedStage &p, qsv::c64 *state, std::vector<float> &mats)
     {
         p.task_offset = append_stage_tasks(mats, {stage_task(p, state, 0)});
+        p.state = state;
     }
 
     void launch_prepared_stage(hipStream_t s, const PreparedStage &p, const float *mats_dev)
     {
         if (p.task_offset == 0)
             throw std::runtime_error("stage: launched without a task");
-        launch_stage_tasks(s, p.kq, p.m, p.x3, mats_dev, p.task_offset, 1, uint64_t(1) << (p.n - p.m));
+        if (p.table_offset != SIZE_MAX)
+            qsv::launch_stage_table64(s, p.state, p.n, p.tsub, mats_dev + p.table_offset);
+        else
+            launch_stage_tasks(s, p.kq, p.m, p.x3, mats_dev, p.task_offset, 1, uint64_t(1) << (p.n - p.m));
     }
 
     void launch_stage_tasks(hipStream_t s, int kq, int m, bool x3, const float *mats_dev, size_t task_offset, int ntasks, uint64_t total_tiles)
@@ -1254,7 +1303,7 @@ namespace qcore
                     it.st128 = prepare_stage128(gates, s, n, mats64, k);
                 else
                 {
-                    it.st = prepare_stage(gates, s, k, m, n, mats, x3_);
+                    it.st = prepare_stage(gates, s, k, m, n, mats, x3_, true);
                     attach_stage(it.st, state_, mats);
                 }
             }

@@ -107,11 +107,20 @@ namespace qcore
         double bytes = 0;       // one sweep
         size_t desc_offset = 0; // floats: StageBlock array
         size_t task_offset = 0; // floats: its one-state task (attach_stage), 0: none yet
+        // floats: the product table of an all-diagonal stage (2^m complex), or none
+        size_t table_offset = SIZE_MAX;
+        qsv::c64 *state = nullptr; // the state of attach_stage
         qsv::Subspace tsub{};
     };
     // Plans the stage's gates into blocks of at most kq qubits inside the tile and appends
-    // their matrices (FP32, or bfloat16 halves when x3) and descriptors to `mats`.
-    PreparedStage prepare_stage(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int kq, int m, int n, std::vector<float> &mats, bool x3);
+    // their matrices (FP32, or bfloat16 halves when x3) and descriptors to `mats`.  With
+    // table, a stage whose blocks are all diagonal also gets the product of their diagonals
+    // over the tile indices (FP64 products rounded to FP32), and an attached stage then runs
+    // as that table alone.
+    PreparedStage prepare_stage(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int kq, int m, int n, std::vector<float> &mats, bool x3, bool table = false);
+    // Blocks a complex64 stage over these gates runs through LDS: none when all of its gates
+    // are diagonal (a table stage), else its greedy blocks.
+    int stage64_lds_blocks(const std::vector<ResolvedGate> &gates, int kq);
     // The stage's task on `state` as the launch's tiles from tile_begin on (checked).
     qsv::StageTask stage_task(const PreparedStage &p, qsv::c64 *state, uint64_t tile_begin);
     // Appends tasks to `mats` 16-B aligned; returns their offset in floats (never 0).

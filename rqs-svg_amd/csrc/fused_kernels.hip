@@ -1051,7 +1051,78 @@ namespace qsv
                 }
             }
         }
+        // complex64 stage whose blocks are all diagonal: as k_stage128_table, with the table
+        // of 2^M complex floats in LDS and two adjacent amplitudes (tile indices e, e | 1) per
+        // 16-byte access.
+        constexpr int kTable64Per = 4, kTable64Threads = 1024;
+        template <int M>
+        __global__ __launch_bounds__(kTable64Threads) void k_stage_table64(c64 *__restrict__ s, uint64_t count, Subspace tsub, const float *__restrict__ table)
+        {
+            __shared__ __attribute__((aligned(16))) v2f tab[1 << M];
+            for (int i = threadIdx.x; i < (1 << M); i += kTable64Threads)
+                tab[i] = *reinterpret_cast<const v2f *>(table + 2 * i);
+            __syncthreads();
+            int hi[M - 4];
+#pragma unroll
+            for (int b = 0; b < M - 4; ++b)
+                hi[b] = tsub.ops[4 + b];
+            const uint64_t per_block = uint64_t(kTable64Threads) * kTable64Per * 2;
+            const uint64_t nchunks = count / per_block;
+            uint64_t nb = gridDim.x, b0 = blockIdx.x;
+            if ((nb & 7) == 0)
+                b0 = (b0 & 7) * (nb >> 3) + (b0 >> 3);
+            for (uint64_t c = b0; c < nchunks; c += nb)
+            {
+                v4f a[kTable64Per];
+#pragma unroll
+                for (int j = 0; j < kTable64Per; ++j)
+                    a[j] = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (c * per_block + 2 * (uint64_t(j) * kTable64Threads + threadIdx.x))));
+#pragma unroll
+                for (int j = 0; j < kTable64Per; ++j)
+                {
+                    const uint64_t i = c * per_block + 2 * (uint64_t(j) * kTable64Threads + threadIdx.x);
+                    uint32_t e = uint32_t(i & 15);
+#pragma unroll
+                    for (int b = 0; b < M - 4; ++b)
+                        e |= uint32_t((i >> hi[b]) & 1) << (4 + b);
+                    const v2f p0 = tab[e], p1 = tab[e | 1];
+                    const v4f y{__builtin_fmaf(p0.x, a[j].x, -p0.y * a[j].y), __builtin_fmaf(p0.x, a[j].y, p0.y * a[j].x),
+                                __builtin_fmaf(p1.x, a[j].z, -p1.y * a[j].w), __builtin_fmaf(p1.x, a[j].w, p1.y * a[j].z)};
+                    __builtin_nontemporal_store(y, reinterpret_cast<v4f *>(s + i));
+                }
+            }
+        }
     } // namespace
+
+    void launch_stage_table64(hipStream_t st, c64 *state, int n, const Subspace &tsub, const float *table)
+    {
+        const int m = tsub.nops;
+        if (m < 11 || m > 14 || n <= m)
+            throw std::runtime_error("stage table: tile bits must be 11..14 and below the state size");
+        for (int k = 0; k < m; ++k)
+            if (tsub.ops[k] < 0 || tsub.ops[k] >= n || (k > 0 && tsub.ops[k] <= tsub.ops[k - 1]))
+                throw std::runtime_error("stage table: tile qubits must be ascending and < n");
+        for (int b = 0; b < 4; ++b)
+            if (tsub.ops[b] != b)
+                throw std::runtime_error("stage table: tile qubits must include qubits 0..3");
+        if (state == nullptr || table == nullptr)
+            throw std::runtime_error("stage table: null pointer");
+        const uint64_t count = uint64_t(1) << n;
+        const uint64_t per_block = uint64_t(kTable64Threads) * kTable64Per * 2;
+        if (count % per_block)
+            throw std::runtime_error("stage table: state smaller than one chunk");
+        const uint64_t chunks = count / per_block;
+        const uint64_t grid = chunks < 4096 ? chunks : 4096;
+        if (m == 11)
+            hipLaunchKernelGGL(k_stage_table64<11>, dim3(grid), dim3(kTable64Threads), 0, st, state, count, tsub, table);
+        else if (m == 12)
+            hipLaunchKernelGGL(k_stage_table64<12>, dim3(grid), dim3(kTable64Threads), 0, st, state, count, tsub, table);
+        else if (m == 13)
+            hipLaunchKernelGGL(k_stage_table64<13>, dim3(grid), dim3(kTable64Threads), 0, st, state, count, tsub, table);
+        else
+            hipLaunchKernelGGL(k_stage_table64<14>, dim3(grid), dim3(kTable64Threads), 0, st, state, count, tsub, table);
+        check(hipGetLastError(), "stage table kernel");
+    }
 
     void launch_block128(hipStream_t st, c128 *state, int n, const Subspace &sub, const Block128Op &op, bool staged)
     {

@@ -69,6 +69,10 @@ namespace qsv
     // tasks: device array of ntasks tasks with ascending tile_begin, the first 0, all checked
     // with check_stage_task; total_tiles = the tiles of all tasks.
     void launch_stage(hipStream_t s, int kq, int m, bool x3, const float *base, const StageTask *tasks, int ntasks, uint64_t total_tiles);
+    // complex64 stage whose blocks are all diagonal: every amplitude times table[tile index],
+    // the tile index made of the amplitude's bits at tsub.ops[0..m-1] (m = tsub.nops, 11..14,
+    // ops[0..3] = 0..3); table holds 2^m complex floats in device memory; 2^n >= 2^13.
+    void launch_stage_table64(hipStream_t s, c64 *state, int n, const Subspace &tsub, const float *table);
 
     // complex128 stage: tile of 2^12 amplitudes, blocks of kq = 3 or 4 qubits in FP64;
     // blocks[i].matrix is the offset in doubles of the 2^kq x 2^kq complex matrix (row-major,

@@ -295,6 +295,33 @@ def test_stages_complex128_diagonal(gpu, oracle, k):
 
 
 @pytest.mark.parametrize("precision", [1, 3])
+@pytest.mark.parametrize("m", [11, 13, 14])
+def test_stages_complex64_diagonal(gpu, oracle, precision, m):
+    # complex64 stages whose blocks are all diagonal run as one phase table over the tile
+    # (tile bits 11, 13, 14: tables of 16-128 KiB); diagonal gates only, then a QFT, whose
+    # controlled-phase runs give such stages between dense ones; random states
+    n = m + 4
+    names = [nm for nm in sorted(pyqcs.GATE_TARGETS) if nm in ("z", "s", "sdg", "t", "tdg", "p", "u1", "rz", "rzz", "global_phase")]
+    diag = [g for g in circuits.random_all_gates(n, 4000, seed=70 + m, max_ctrls=2) if g[0] in names][:300]
+    assert len(diag) == 300
+    psi = random_state(n, 11)
+    for c in (diag, circuits.qft(n)):
+        with pyqcs.Simulator(gpu, n, precision=precision, block_qubits=5, stage_bits=m) as f:
+            f.set_statevector(psi)
+            f.run(c)
+            got = f.statevector()
+            gates, launches = f.fusion_stats()
+        with pyqcs.Simulator(oracle, n) as o:
+            o.set_statevector(psi)
+            o.run(c)
+            want = o.statevector()
+        say(f"c64 diagonal stages n={n} m={m} precision={precision}: fidelity {fidelity(got, want):.12f} max|d| {np.max(np.abs(got - want)):.2e} launches {launches}")
+        assert fidelity(got, want) >= 1 - FID_TOL and np.max(np.abs(got - want)) <= AMP_TOL
+        assert gates == len(c)
+    assert launches < len(c) // 4
+
+
+@pytest.mark.parametrize("precision", [1, 3])
 def test_stage_launch_chunks(gpu, oracle, precision):
     # deep circuits on few qubits give stages with more blocks than one launch's LDS holds
     # (5 blocks of 5 qubits with 2^12 tiles): those run as several launches over the tile
