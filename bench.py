@@ -197,6 +197,52 @@ def fused_leg(n, depth, k, shots, variants=FUSED_VARIANTS):
     return out
 
 
+def qft_c64_leg(n=32, k=5, stage_bits=13):
+    """A phase-heavy circuit in the precision form: the n-qubit QFT (H, controlled-phase
+    ladders, closing swaps) as complex64 stages (FP32 and BF16x3 MFMA; stages whose blocks
+    are all diagonal run as one phase-table pass) against the plain complex128 method.  The
+    QFT acts on the state of one random brick layer (on |0...0> it would only make the
+    uniform superposition); that preparation is not timed."""
+    lib = pyqcs.gpu_library()
+    circ = circuits.qft(n)
+    warm = circuits.brick(n, 1, seed=1)[: n + 2]
+    prep = circuits.brick(n, 1, seed=CIRCUIT_SEED + 9)
+    plain = pyqcs.Simulator(lib, n, seed=1)
+    plain.run(warm)
+    plain.call("set_zero_state")
+    plain.run(prep)
+    plain.norm()
+    t0 = time.perf_counter()
+    plain.run(circ)
+    plain.probability_total()
+    t_plain = time.perf_counter() - t0
+    out = {"config": f"qft{n}: {n}-qubit QFT ({len(circ)} gates), complex64 stages of k={k} blocks "
+                     f"on 2^{stage_bits}-amplitude tiles vs plain complex128, on the state of one "
+                     f"random brick layer",
+           "gates": len(circ), "plain_c128_s": t_plain}
+    for mode, key in ((3, "bf16x3"), (1, "fp32")):
+        fused = pyqcs.Simulator(lib, n, seed=1, precision=mode, block_qubits=k, stage_bits=stage_bits)
+        fused.run(warm)
+        fused.call("set_zero_state")
+        fused.run(prep)
+        fused.norm()
+        g0, b0 = fused.fusion_stats()
+        t0 = time.perf_counter()
+        fused.run(circ)
+        fused.probability_total()
+        dt = time.perf_counter() - t0
+        g1, b1 = fused.fusion_stats()
+        ov, maxdiff = fused.overlap(plain)
+        fid = abs(ov) ** 2 / (fused.norm() * plain.norm())
+        fused.close()
+        log(f"qft leg: {key} {dt:.3f}s ({b1 - b0} sweeps) vs plain c128 {t_plain:.3f}s = {t_plain / dt:.1f}x, "
+            f"fidelity {fid:.12f}, max|d| {maxdiff:.2e}")
+        out[key] = {"fused_c64_s": dt, "sweeps": b1 - b0, "speedup": t_plain / dt,
+                    "fidelity": fid, "infidelity": 1 - fid, "max_abs_diff": maxdiff}
+    plain.close()
+    return out
+
+
 def batch_leg(count, seed=20250404, direct=False, tuning=None, precisions=(1, 3)):
     """512 variational circuits (12-22 qubits, 4-16 layers of RY/RZ + CNOT ladder): one
     autotuned hipGraph launch (complex64 states, FP64 <Z> reductions) against the per-circuit
@@ -469,6 +515,8 @@ def main():
     }
     if not args.no_small:
         result["small_20q"] = small_leg()
+    if not args.no_fused:
+        result["fused_qft"] = qft_c64_leg(args.fused_qubits)
     if not args.no_fused:
         result.update(fused_leg(args.fused_qubits, args.fused_depth, args.fused_k, args.shots))
     if not args.no_batch:
