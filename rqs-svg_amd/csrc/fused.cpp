@@ -515,11 +515,11 @@ namespace qcore
                         std::sort(st.tile.begin(), st.tile.end());
                         std::vector<ResolvedGate> gs;
                         const int nb = 1 + int(rnd() % 12);
-                        // a quarter of the stages all diagonal (phase tables); complex128: a
-                        // quarter ending in a run of diagonal blocks (a table at the store)
+                        // a quarter of the stages all diagonal (phase tables), a quarter
+                        // ending in a run of diagonal blocks (a table at the store)
                         const int shape = int(rnd() % 4);
                         for (int b = 0; b < nb; ++b)
-                            random_gates(random_subset(st.tile, 1 + int(rnd() % kb)), gs, shape == 0 || (f64_ && shape == 1 && 2 * b >= nb));
+                            random_gates(random_subset(st.tile, 1 + int(rnd() % kb)), gs, shape == 0 || (shape == 1 && 2 * b >= nb));
                         st.gates.resize(gs.size());
                         std::iota(st.gates.begin(), st.gates.end(), 0);
                         L.tile_bits = m;
@@ -960,8 +960,7 @@ namespace qcore
         const size_t per = size_t(D) * D;
         const int K = 1 << kq;
         std::vector<qsv::StageBlock> desc;
-        std::vector<std::vector<cd>> diagonals; // while every block so far is diagonal
-        bool all_diagonal = table;
+        std::vector<std::vector<cd>> diagonals; // per block: its diagonal, empty if not diagonal
         for (const auto &pb : plan)
         {
             if (pb.wide)
@@ -973,15 +972,15 @@ namespace qcore
                 if (local_pos(lay.qubits, st.tile[i]) < 0)
                     lay.qubits.push_back(st.tile[i]);
             const auto M = block_matrix(sub, pb, lay);
-            if (all_diagonal)
             {
                 // products of diagonal gates have exactly zero off-diagonal entries
-                for (int r = 0; r < K && all_diagonal; ++r)
-                    for (int c = 0; c < K && all_diagonal; ++c)
-                        all_diagonal = r == c || M[size_t(r) * K + c] == cd(0.0, 0.0);
-                std::vector<cd> d(K);
-                for (int e = 0; e < K; ++e)
-                    d[e] = M[size_t(e) * K + e];
+                bool diagonal = table;
+                for (int r = 0; r < K && diagonal; ++r)
+                    for (int c = 0; c < K && diagonal; ++c)
+                        diagonal = r == c || M[size_t(r) * K + c] == cd(0.0, 0.0);
+                std::vector<cd> d;
+                for (int e = 0; diagonal && e < K; ++e)
+                    d.push_back(M[size_t(e) * K + e]);
                 diagonals.push_back(std::move(d));
             }
             std::vector<double> R(per);
@@ -1026,31 +1025,32 @@ namespace qcore
         static_assert(sizeof(qsv::StageBlock) % 16 == 0, "StageBlock keeps 16-B alignment");
         PreparedStage p;
         p.gates = (int)st.gates.size();
-        p.nblocks = (int)desc.size();
         p.n = n;
         p.kq = kq;
         p.m = m;
         p.x3 = x3;
         p.bytes = 2.0 * 8.0 * double(uint64_t(1) << n);
-        while (mats.size() % 4)
-            mats.push_back(0.f);
-        p.desc_offset = mats.size();
-        mats.resize(mats.size() + desc.size() * sizeof(qsv::StageBlock) / sizeof(float));
-        std::memcpy(mats.data() + p.desc_offset, desc.data(), desc.size() * sizeof(qsv::StageBlock));
         p.tsub.set_bits = 0;
         p.tsub.nops = m;
         for (int b = 0; b < m; ++b)
             p.tsub.ops[b] = st.tile[b];
-        if (all_diagonal && !desc.empty())
+        // With table: the diagonal blocks after the last other block become one table over
+        // the tile indices (tile index bit b = tile qubit b = the blocks' weight 1 << b),
+        // their diagonals multiplied in block order (FP64, rounded to FP32).  A stage of
+        // diagonal blocks only runs as that table alone; otherwise the stage kernel applies
+        // it as the tile is stored, and the blocks before it run as usual.
+        size_t first_trail = desc.size();
+        while (first_trail > 0 && !diagonals[first_trail - 1].empty())
+            --first_trail;
+        if (table && first_trail < desc.size())
         {
-            // tile index bit b = tile qubit b = the blocks' weight 1 << b; block order
             while (mats.size() % 4)
                 mats.push_back(0.f);
-            p.table_offset = mats.size();
+            (first_trail == 0 ? p.table_offset : p.trail_offset) = mats.size();
             for (int idx = 0; idx < (1 << m); ++idx)
             {
                 cd v(1.0, 0.0);
-                for (size_t bi = 0; bi < desc.size(); ++bi)
+                for (size_t bi = first_trail; bi < desc.size(); ++bi)
                 {
                     int e = 0;
                     for (int jj = 0; jj < kq; ++jj)
@@ -1061,20 +1061,19 @@ namespace qcore
                 mats.push_back(float(v.real()));
                 mats.push_back(float(v.imag()));
             }
+            if (first_trail > 0)
+                desc.resize(first_trail);
         }
+        p.nblocks = (int)desc.size();
+        while (mats.size() % 4)
+            mats.push_back(0.f);
+        p.desc_offset = mats.size();
+        mats.resize(mats.size() + desc.size() * sizeof(qsv::StageBlock) / sizeof(float));
+        std::memcpy(mats.data() + p.desc_offset, desc.data(), desc.size() * sizeof(qsv::StageBlock));
         return p;
     }
 
-    int stage64_lds_blocks(const std::vector<ResolvedGate> &gates, int kq)
-    {
-        for (const auto &g : gates)
-        {
-            const Kind kd = g.spec.kind;
-            if (g.mmask != 0 || !(kd == Kind::GlobalPhase || kd == Kind::PhaseOne || kd == Kind::Diag1 || kd == Kind::Diag2 || kd == Kind::Identity))
-                return (int)plan_greedy(gates, kq).size();
-        }
-        return 0;
-    }
+    int stage64_lds_blocks(const std::vector<ResolvedGate> &gates, int kq) { return stage128_lds_blocks(gates, kq); }
 
     qsv::StageTask stage_task(const PreparedStage &p, qsv::c64 *state, uint64_t tile_begin)
     {
@@ -1088,6 +1087,13 @@ namespace qcore
         t.tsub = p.tsub;
         t.blocks = uint32_t(p.desc_offset);
         t.nblocks = p.nblocks;
+        if (p.trail_offset != SIZE_MAX)
+        {
+            if (p.trail_offset > 0xffffffffu)
+                throw std::runtime_error("stage: matrix buffer beyond 2^32 floats");
+            t.flags |= qsv::kStageTrailTable;
+            t.trail = uint32_t(p.trail_offset);
+        }
         qsv::check_stage_task(t, p.n, p.kq, p.m);
         return t;
     }

@@ -109,17 +109,18 @@ namespace qcore
         size_t task_offset = 0; // floats: its one-state task (attach_stage), 0: none yet
         // floats: the product table of an all-diagonal stage (2^m complex), or none
         size_t table_offset = SIZE_MAX;
+        // floats: the product table of the diagonal blocks after the last other block, or none
+        size_t trail_offset = SIZE_MAX;
         qsv::c64 *state = nullptr; // the state of attach_stage
         qsv::Subspace tsub{};
     };
     // Plans the stage's gates into blocks of at most kq qubits inside the tile and appends
     // their matrices (FP32, or bfloat16 halves when x3) and descriptors to `mats`.  With
-    // table, a stage whose blocks are all diagonal also gets the product of their diagonals
-    // over the tile indices (FP64 products rounded to FP32), and an attached stage then runs
-    // as that table alone.
+    // table, the diagonal blocks after the last other block become the product of their
+    // diagonals over the tile indices (FP64 products rounded to FP32): a stage of diagonal
+    // blocks only then runs as that table alone, another applies it as its tiles are stored.
     PreparedStage prepare_stage(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int kq, int m, int n, std::vector<float> &mats, bool x3, bool table = false);
-    // Blocks a complex64 stage over these gates runs through LDS: none when all of its gates
-    // are diagonal (a table stage), else its greedy blocks.
+    // Blocks a complex64 stage with tables runs through LDS (as stage128_lds_blocks).
     int stage64_lds_blocks(const std::vector<ResolvedGate> &gates, int kq);
     // The stage's task on `state` as the launch's tiles from tile_begin on (checked).
     qsv::StageTask stage_task(const PreparedStage &p, qsv::c64 *state, uint64_t tile_begin);

@@ -805,9 +805,24 @@ namespace qsv
                 for (int i = 0; i < LOADS; ++i)
                     *reinterpret_cast<v4f *>(&tile[swz64(2 * tm + i * 2 * NTHM)]) = v[i];
             }
-            __device__ __forceinline__ void store(const v2f *tile, TaskPtr T, uint64_t base) const
+            __device__ __forceinline__ void store(const v2f *tile, TaskPtr T, uint64_t base, const float *__restrict__ mats) const
             {
                 c64 *__restrict__ s = T->state;
+                if (T->flags & kStageTrailTable) // uniform: the diagonal blocks after the others
+                {
+                    const v4f *tab = reinterpret_cast<const v4f *>(mats + T->trail);
+#pragma unroll
+                    for (int i = 0; i < LOADS; ++i)
+                    {
+                        // tile indices e, e + 1 with e = 2 * tm + i * 2 * NTHM
+                        const v4f p = tab[tm + i * NTHM];
+                        const v4f v = *reinterpret_cast<const v4f *>(&tile[swz64(2 * tm + i * 2 * NTHM)]);
+                        const v4f y{__builtin_fmaf(p.x, v.x, -p.y * v.y), __builtin_fmaf(p.x, v.y, p.y * v.x),
+                                    __builtin_fmaf(p.z, v.z, -p.w * v.w), __builtin_fmaf(p.z, v.w, p.w * v.z)};
+                        __builtin_nontemporal_store(y, reinterpret_cast<v4f *>(s + (base + offset(T, i))));
+                    }
+                    return;
+                }
 #pragma unroll
                 for (int i = 0; i < LOADS; ++i)
                 {
@@ -848,7 +863,7 @@ namespace qsv
                     stage_block<KQ, M, X3, W>(tile, mats, blocks[bi], wave, c, h);
                     stage_barrier<W>();
                 }
-                io.store(tile, T, base);
+                io.store(tile, T, base, mats);
                 // the LDS tile is reused only when this workgroup takes another tile
                 if (g + gridDim.x < total_tiles)
                     stage_barrier<W>();
@@ -1183,6 +1198,8 @@ namespace qsv
             throw std::runtime_error("stage: tiles take no controls");
         if (t.nblocks < 1 || t.state == nullptr || t.blocks % 4 != 0)
             throw std::runtime_error("stage: task without blocks or state, or misaligned descriptors");
+        if ((t.flags & kStageTrailTable) && t.trail % 4 != 0)
+            throw std::runtime_error("stage: misaligned trailing table");
     }
 
     void launch_stage(hipStream_t st, int kq, int m, bool x3, const float *base, const StageTask *tasks, int ntasks, uint64_t total_tiles)

@@ -61,9 +61,10 @@ namespace qsv
         uint32_t blocks;
         int nblocks;
         uint32_t flags; // kStageZeroState: the state is |0...0>; the tiles are made, not loaded
-        uint32_t pad;
+        uint32_t trail; // kStageTrailTable: floats from the matrix base to a table of 2^m
+                        // complex floats (tile index bit b = tile qubit b) applied at the store
     };
-    constexpr uint32_t kStageZeroState = 1;
+    constexpr uint32_t kStageZeroState = 1, kStageTrailTable = 2;
     // Host-side check of one task before upload (the kernel trusts the tasks).
     void check_stage_task(const StageTask &t, int n, int kq, int m);
     // tasks: device array of ntasks tasks with ascending tile_begin, the first 0, all checked
