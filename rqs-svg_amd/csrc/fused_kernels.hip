@@ -810,8 +810,11 @@ namespace qsv
                 c64 *__restrict__ s = T->state;
                 if (T->flags & kStageTrailTable) // uniform: the diagonal blocks after the others
                 {
+                    // unrolled by 4 only: fully unrolled, this rarely taken path raised the
+                    // kernel's register count (k_stage<4, 11, true, 1>: 155 -> 178) and cost
+                    // the dense stages a wave per SIMD
                     const v4f *tab = reinterpret_cast<const v4f *>(mats + T->trail);
-#pragma unroll
+#pragma unroll 4
                     for (int i = 0; i < LOADS; ++i)
                     {
                         // tile indices e, e + 1 with e = 2 * tm + i * 2 * NTHM
