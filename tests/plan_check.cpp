@@ -2,12 +2,18 @@
 // every gate lands in exactly one stage, stages keep the gates' order, a stage's gates lie
 // inside its tile (m qubits, ascending, 0..3 included), a stage without a tile holds one
 // gate wider than the block size, and any two gates whose order the plan swaps commute
-// (disjoint qubits, or both diagonal without measured targets).  Prints one line per case
-// and "ok" at the end; exits 1 at the first violation.
+// (disjoint qubits, or both diagonal without measured targets).  Then the phase tables of
+// stages of diagonal gates (qcore::prepare_stage128 and prepare_stage with tables): entry
+// idx against the product of the gates' own phases on the tile qubits' bits of idx, from
+// the gate definitions (P(t) = diag(1, e^{it}), RZ(t) = diag(e^{-it/2}, e^{it/2}), Z, S, T;
+// controls: the phase only where every control bit is 1).  Prints one line per case and
+// "ok" at the end; exits 1 at the first violation.
 // Usage: plan_check <cases> <seed>
 #include "fused.hpp"
 
 #include <algorithm>
+#include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -140,6 +146,85 @@ int main(int argc, char **argv)
                         fail("two gates that do not commute were reordered", c);
                 }
             std::printf("case %d n %d m %d k %d gates %d tiles %d: %zu stages\n", c, n, m, k, ngates, tiles, plan.size());
+        }
+    }
+    // phase tables of all-diagonal stages
+    for (int c = 0; c < cases; ++c)
+    {
+        const int n = 16 + int(mix(seed) % 6);
+        const int m = 12;
+        std::vector<int> tile = {0, 1, 2, 3};
+        const int w0 = 4 + int(mix(seed) % (n - 11)); // a window of 8 qubits above qubit 3
+        for (int q = w0; q < w0 + 8; ++q)
+            tile.push_back(q);
+        const char *names[] = {"p", "rz", "z", "s", "t"};
+        std::vector<ResolvedGate> gates;
+        std::vector<double> angle;
+        const int ng = 1 + int(mix(seed) % 40);
+        for (int i = 0; i < ng; ++i)
+        {
+            ResolvedGate r;
+            const int kind = int(mix(seed) % 5);
+            double prm[4] = {6.283 * double(mix(seed) >> 11) * 0x1.0p-53, 0, 0, 0};
+            r.spec = make_gate(names[kind], prm);
+            std::vector<int> pool = tile;
+            for (int q = m - 1; q > 0; --q)
+                std::swap(pool[q], pool[mix(seed) % (q + 1)]);
+            r.tphys = {pool[0]};
+            const int nc = kind == 1 ? 0 : int(mix(seed) % 3); // rz without controls
+            for (int j = 0; j < nc; ++j)
+                r.pphys.push_back(pool[1 + j]);
+            gates.push_back(r);
+            angle.push_back(prm[0]);
+        }
+        PlannedStage st;
+        st.tile = tile;
+        for (int i = 0; i < ng; ++i)
+            st.gates.push_back(i);
+        // the phase of gate i on the basis state whose tile bits are those of idx
+        auto expect = [&](int idx) {
+            std::complex<double> v(1.0, 0.0);
+            auto bit = [&](int q) { return (idx >> (std::find(tile.begin(), tile.end(), q) - tile.begin())) & 1; };
+            for (int i = 0; i < ng; ++i)
+            {
+                const ResolvedGate &g = gates[i];
+                bool on = true;
+                for (int q : g.pphys)
+                    on = on && bit(q);
+                if (!on)
+                    continue;
+                const int t = bit(g.tphys[0]);
+                const double a = angle[i];
+                const std::string nm = g.spec.name;
+                if (nm == "rz")
+                    v *= std::polar(1.0, t ? a / 2 : -a / 2);
+                else if (t)
+                    v *= std::polar(1.0, nm == "p" ? a : nm == "z" ? M_PI : nm == "s" ? M_PI / 2 : M_PI / 4);
+            }
+            return v;
+        };
+        for (int kq : {3, 4})
+        {
+            std::vector<double> mats64;
+            const auto p128 = prepare_stage128(gates, st, n, mats64, kq);
+            if (p128.table_offset == SIZE_MAX || p128.nblocks != 0)
+                fail("complex128 stage of diagonal gates without a table", c);
+            std::vector<float> mats;
+            const auto p64 = prepare_stage(gates, st, kq + 1, m, n, mats, false, true);
+            if (p64.table_offset == SIZE_MAX)
+                fail("complex64 stage of diagonal gates without a table", c);
+            double e128 = 0, e64 = 0;
+            for (int idx = 0; idx < (1 << m); ++idx)
+            {
+                const std::complex<double> want = expect(idx);
+                const std::complex<double> got(mats64[p128.table_offset + 2 * idx], mats64[p128.table_offset + 2 * idx + 1]);
+                const std::complex<double> got64(mats[p64.table_offset + 2 * idx], mats[p64.table_offset + 2 * idx + 1]);
+                e128 = std::max(e128, std::abs(got - want));
+                e64 = std::max(e64, std::abs(got64 - want));
+            }
+            if (e128 > 1e-12 || e64 > 1e-6)
+                fail("phase table differs from the gates' phases", c);
+            std::printf("table case %d n %d window %d gates %d kq %d: max|d| %.1e (c128) %.1e (c64)\n", c, n, w0, ng, kq, e128, e64);
         }
     }
     std::printf("ok\n");

@@ -3,8 +3,10 @@ library's planner and checks seeded random gate lists under the grown, window an
 tile choices -- every gate in exactly one stage, order kept inside a stage, gates inside
 their stage's tile, and every pair of gates the plan reorders commuting (disjoint qubits,
 or both diagonal).  Checked against a planted mistake: a planner that lets any gate pass a
-deferred diagonal gate fails the last check on the first case.  The GPU parity tests
-cover the resulting states."""
+deferred diagonal gate fails the last check on the first case.  It then checks the phase
+tables of stages of diagonal gates (complex128 and complex64) entry by entry against the
+gates' own phases computed from their definitions; a flipped RZ sign in that computation
+fails the first case.  The GPU parity tests cover the resulting states."""
 import os
 import shutil
 import subprocess
