@@ -311,6 +311,129 @@ def batch_leg(count, seed=20250404, direct=False, tuning=None, precisions=(1, 3)
     return out
 
 
+def _class_rate(sim, cls):
+    """Launches, average ms and algorithmic GB/s of one kernel class since the last reset."""
+    v = sim.profile_get().get(cls)
+    if not v:
+        return None
+    avg = v["seconds"] / v["launches"]
+    gbs = v["bytes"] / v["seconds"] / 1e9
+    return {"launches": v["launches"], "avg_ms": avg * 1e3, "GBps": gbs, "frac": gbs / (HBM_PEAK / 1e9),
+            "bytes_per_launch": v["bytes"] / v["launches"]}
+
+
+SUBSYSTEM_GATES = (
+    # (gate, parameter count, kernel class, operand tuples as functions of n)
+    ("rz", 1, "diag", lambda n: [(0,), (5,), (17,), (n - 1,)]),
+    ("rzz", 1, "diag", lambda n: [(0, 1), (14, 15), (n - 2, n - 1)]),
+    ("rxx", 1, "dense4", lambda n: [(0, 1), (3, 4), (14, 15), (n - 3, n - 1)]),
+    ("ecr", 0, "dense4", lambda n: [(0, 1), (14, 15), (n - 2, n - 1)]),
+    ("dcx", 0, "mono", lambda n: [(0, 1), (14, 15), (n - 2, n - 1)]),
+    ("rccx", 0, "mono", lambda n: [(0, 1, 2), (14, 15, 16), (n - 3, n - 2, n - 1)]),
+    ("swap", 0, "pair", lambda n: [(0, 1), (1, 2), (0, n - 1), (14, 15), (n - 2, n - 1)]),
+    ("u3", 3, "pair", lambda n: [(0,), (2,), (6,), (12,), (20,), (n - 1,)]),
+)
+
+
+def subsystem_leg(n=30, reps=4):
+    """Every kernel family of the plain method on an n-qubit complex128 state (one random
+    brick layer, so every amplitude is non-zero), timed with HIP events around each launch:
+    algorithmic bytes per launch (2 * 16 B * amplitudes the gate may change; a permutation
+    pass one sweep; a measurement's {p0,p1} reduction one read of the state, its collapse
+    the zeroed half, plus a read and write of the kept half when renormalising; the
+    sampling prefix one read) over the average launch time, against 8 TB/s.  Per gate and
+    operand position, since low operands (qubits 0..2, 16-48 B apart) are the hard case.
+    The measurement cases also give the reference's per-qubit sampling path (measure every
+    qubit in turn, src/qcs_main.cpp:153-193, src/qcs.cu:2545-2677) as wall time."""
+    lib = pyqcs.gpu_library()
+    sim = pyqcs.Simulator(lib, n, seed=7)
+    prep = circuits.brick(n, 1, seed=CIRCUIT_SEED + 21)
+
+    def fresh():
+        sim.profile(False)
+        sim.call("set_zero_state")
+        sim.run(prep)
+        sim.flush()
+        sim.norm()
+
+    out = {"config": f"{n} qubits complex128, state of one random brick layer; {reps} launches per case"}
+    fresh()
+    gates = {}
+    for name, npar, cls, ops in SUBSYSTEM_GATES:
+        rows = []
+        for opnd in ops(n):
+            params = tuple(0.3 + 0.1 * i for i in range(npar))
+            sim.profile_reset()
+            sim.profile(True)
+            for _ in range(reps):
+                sim.gate(name, opnd, params)
+            r = _class_rate(sim, cls)
+            sim.profile(False)
+            r["operands"] = list(opnd)
+            rows.append(r)
+            log(f"subsystems: {name}{tuple(opnd)} {cls}: {r['avg_ms']:.3f} ms, {r['GBps']:.0f} GB/s ({r['frac']:.2f})")
+        gates[name] = rows
+    out["gates"] = gates
+    ident = list(range(n))
+    rng = np.random.default_rng(CIRCUIT_SEED)
+    low_high = ident.copy()
+    for a, b in ((0, n - 3), (1, n - 1), (2, n - 5), (9, 14)):
+        low_high[a], low_high[b] = low_high[b], low_high[a]
+    cases = {"low_high_4_pairs": low_high, "reversal": ident[::-1],
+             "random": [int(q) for q in rng.permutation(n)]}
+    perm = {}
+    for name, want in cases.items():
+        sim.profile_reset()
+        sim.profile(True)
+        t0 = time.perf_counter()
+        sim.permute(want)
+        sim.flush()
+        sim.norm()
+        dt = time.perf_counter() - t0
+        r = _class_rate(sim, "permute")
+        sim.profile(False)
+        sim.permute(ident)
+        r["passes"] = r["launches"]
+        perm[name] = r
+        log(f"subsystems: permute {name}: {r['launches']} passes, {r['avg_ms']:.3f} ms per pass, "
+            f"{r['GBps']:.0f} GB/s ({r['frac']:.2f})")
+    out["permute"] = perm
+    meas = {}
+    for renorm in (False, True):
+        sim.set_renormalize(renorm)
+        key = "renormalised" if renorm else "plain"
+        rows = []
+        for q in (0, n // 2, n - 1):
+            fresh()
+            sim.profile_reset()
+            sim.profile(True)
+            sim.measure(q)
+            red, col = _class_rate(sim, "reduce"), _class_rate(sim, "collapse")
+            sim.profile(False)
+            rows.append({"qubit": q, "reduce": red, "collapse": col})
+            log(f"subsystems: measure q{q} ({key}): reduce {red['avg_ms']:.3f} ms {red['GBps']:.0f} GB/s, "
+                f"collapse {col['avg_ms']:.3f} ms {col['GBps']:.0f} GB/s")
+        fresh()
+        t0 = time.perf_counter()
+        bits = [sim.measure(q) for q in range(n)]
+        dt = time.perf_counter() - t0
+        meas[key] = {"first_measure_by_qubit": rows, "all_qubits_s": dt, "outcome": "".join(map(str, bits))}
+        log(f"subsystems: measure all {n} qubits in turn ({key}): {dt * 1e3:.1f} ms")
+    sim.set_renormalize(False)
+    out["measure"] = meas
+    fresh()
+    sim.profile_reset()
+    sim.profile(True)
+    sim.probability_total()
+    fold = _class_rate(sim, "reduce")
+    sim.sample(4096, seed=SAMPLE_SEED)
+    sim.profile(False)
+    out["prefix_fold"] = fold
+    log(f"subsystems: sampling prefix {fold['avg_ms']:.3f} ms, {fold['GBps']:.0f} GB/s ({fold['frac']:.2f})")
+    sim.close()
+    return out
+
+
 def planner_leg(n, record_qubits, record_count, depth, k):
     """Learned fusion planner on complex128 fused blocks: record launch timings (single
     blocks and multi-block stages) on states of `record_qubits` qubits, train the cost-model
@@ -413,6 +536,9 @@ def main():
                     help="issue the batch's launches directly instead of one graph launch "
                          "(same kernels; for profilers that do not follow graph launches)")
     ap.add_argument("--no-planner", action="store_true")
+    ap.add_argument("--no-subsystems", action="store_true",
+                    help="skip the per-kernel-family leg (diagonal, 4x4, monomial, swap, "
+                         "permutation, measurement, prefix) at the headline size")
     ap.add_argument("--no-small", action="store_true", help="skip the 20-qubit GPU-vs-CPU leg")
     ap.add_argument("--planner-qubits", type=int, default=33)
     ap.add_argument("--planner-depth", type=int, default=40)
@@ -513,6 +639,8 @@ def main():
         "sweeps_per_s": alg_bytes / (2 * 16 * (1 << n)) * args.steps / wall,
         "hbm_frac_whole_step": alg_bytes * args.steps / wall / HBM_PEAK,
     }
+    if not args.no_subsystems:
+        result["subsystems"] = subsystem_leg(n)
     if not args.no_small:
         result["small_20q"] = small_leg()
     if not args.no_fused:

@@ -61,7 +61,8 @@ enum qcs_ext_kernel_class
  * std::random_device as in the reference (src/qcs.cu:1845-1852). */
 int qcs_ext_set_seed(qcs_simulator *sim, uint64_t seed);
 
-/* Copy the whole state out / in, logical order, 2 * num_amplitudes doubles. */
+/* Copy the whole state out / in, logical order, 2 * num_amplitudes doubles.  Setting the
+ * state clears the measured lists, as loading does. */
 int qcs_ext_get_statevector(qcs_simulator *sim, double *out, uint64_t num_amplitudes);
 int qcs_ext_set_statevector(qcs_simulator *sim, const double *in, uint64_t num_amplitudes);
 

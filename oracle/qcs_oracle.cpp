@@ -1261,6 +1261,9 @@ extern "C"
                 uint64_t l = c.logical_index(i);
                 c.state[i] = oracle::cplx(in[2 * l], in[2 * l + 1]);
             }
+            // a new state: earlier measurement results no longer describe it
+            c.m0.clear();
+            c.m1.clear();
         });
     }
     int qcs_ext_load_statevector(qcs_simulator *s, const char *path)

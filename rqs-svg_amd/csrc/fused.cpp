@@ -597,17 +597,6 @@ namespace qcore
             (void)hipEventDestroy(e);
     }
 
-    void FusedEngine::swap_physical(int a, int b)
-    {
-        flush();
-        ResolvedGate g;
-        double none[1] = {0.0};
-        g.spec = make_gate("swap", none);
-        g.tphys = {a, b};
-        std::vector<ResolvedGate> gates{g};
-        apply_blocks(gates, plan_greedy(gates, k_));
-    }
-
     namespace
     {
         // float -> bfloat16 bits, round to nearest even (NaN stays NaN)

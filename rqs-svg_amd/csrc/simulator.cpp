@@ -138,6 +138,7 @@ namespace qcore
 
     void Simulator::reinitialize_mapping()
     { // src/qcs.cu:1860-1885
+        state_changed();
         p2l_.assign(n_, 0);
         l2p_.assign(n_, 0);
         if (initial_p2l.empty())
@@ -223,11 +224,13 @@ namespace qcore
         stage_len_ = 0;
         allocated_ = false;
         n_ = 0;
+        state_changed();
     }
 
     void Simulator::set_zero_state()
     { // src/qcs.cu:2065-2079
         need_alloc();
+        state_changed();
         clear_measurement_state();
         if (fused_)
             fused_->drop(); // queued gates act on a state that is being overwritten
@@ -239,6 +242,7 @@ namespace qcore
     void Simulator::set_sequential_state()
     { // src/qcs.cu:71-76, 2025-2043
         need_alloc();
+        state_changed();
         clear_measurement_state();
         if (fused_)
             fused_->drop();
@@ -248,6 +252,7 @@ namespace qcore
     void Simulator::set_flat_state()
     { // src/qcs.cu:78-82, 2045-2063
         need_alloc();
+        state_changed();
         clear_measurement_state();
         if (fused_)
             fused_->drop();
@@ -257,6 +262,7 @@ namespace qcore
     void Simulator::set_entangled_state()
     { // src/qcs.cu:2081-2095 with one rank: amp[0] = 1, nothing else written
         need_alloc();
+        state_changed();
         clear_measurement_state();
         flush(); // amp[0] = 1 on top of the current state
         with_state([&](auto *st) { qsv::launch_set_one(stream_, st, 0); });
@@ -266,6 +272,7 @@ namespace qcore
     { // src/qcs.cu:2097-2119: curand normals there; this build's own generator here.
       // The measured lists are kept, as in the reference.
         need_alloc();
+        state_changed();
         if (fused_)
             fused_->drop();
         with_state([&](auto *st) { qsv::launch_fill_random(stream_, st, uint64_t(1) << n_, seed_); });
@@ -291,17 +298,13 @@ namespace qcore
             throw std::runtime_error("a qubit appears more than once among targets and controls");
     }
 
-    // The in-HBM replacement of the reference's NCCL global<->local exchange: swap the
-    // data of physical qubits a and b (one half-sweep, the SWAP pair kernel).
+    // The in-HBM replacement of the reference's NCCL global<->local exchange for one pair:
+    // swap the data of physical qubits a and b with the SWAP pair kernel (it reads and writes
+    // only the half of the state the swap changes).  complex128 per-gate state only.
     void Simulator::swap_physical(int a, int b)
     {
         if (a == b)
             return;
-        if (fused_)
-        {
-            fused_->swap_physical(a, b);
-            return;
-        }
         qsv::Subspace sub = make_sub({a, b}, 0);
         qsv::PairOp op{};
         op.offA = uint64_t(1) << a;
@@ -315,8 +318,45 @@ namespace qcore
         prof_end();
     }
 
+    // The mapping after the data of physical qubit q moved to physical qubit dst[q].
+    void Simulator::remap_physical(const std::vector<int> &dst)
+    {
+        std::vector<int> p2l(n_);
+        for (int q = 0; q < n_; ++q)
+            p2l[dst[q]] = p2l_[q];
+        p2l_ = p2l;
+        for (int q = 0; q < n_; ++q)
+            l2p_[p2l_[q]] = q;
+    }
+
+    // Moves the data of every physical qubit q to physical qubit dst[q]: one sweep of the
+    // LDS-tiled permutation kernel per pass (a pass moves up to 6 qubits above the low run
+    // that every tile keeps contiguous: 64 complex128 or 128 complex64 amplitudes, 1 KiB),
+    // then the mapping follows the data.
+    void Simulator::move_physical(const std::vector<int> &dst)
+    {
+        bool any = false;
+        for (int q = 0; q < n_; ++q)
+            any = any || dst[q] != q;
+        if (!any)
+            return;
+        flush();
+        state_changed();
+        const int tb = c64_state() ? qsv::permute_tile_bits<qsv::c64>() : qsv::permute_tile_bits<qsv::c128>();
+        const int low = c64_state() ? 7 : 6;
+        const auto passes = n_ <= tb ? std::vector<std::vector<int>>{dst} : qsv::plan_permute_passes(dst, tb - low, low);
+        const double S = c64_state() ? 8.0 : 16.0;
+        for (const auto &p : passes)
+        {
+            prof_begin(QCS_EXT_K_PERMUTE, 2.0 * S * (1ull << n_));
+            with_state([&](auto *st) { qsv::launch_permute(stream_, st, n_, p); });
+            prof_end();
+        }
+        remap_physical(dst);
+    }
+
     void Simulator::ensure_local(std::vector<int> &tphys)
-    { // src/qcs.cu:2206-2332 (bookkeeping) with the data move done by swap_physical
+    { // src/qcs.cu:2206-2332 (bookkeeping) with the data move done in HBM
         // Slot for the i-th global target: n_local-1-i as in the reference, except that
         // slots holding a local target of the same gate are skipped (the reference would move
         // that target away and apply the gate to the wrong qubit).
@@ -335,26 +375,32 @@ namespace qcore
         }
         if (glob.empty())
             return;
-        for (size_t i = 0; i < glob.size(); ++i)
-            swap_physical(glob[i], loc[i]);
-        std::vector<int> loc_log(glob.size()), glob_log(glob.size());
+        std::vector<int> dst(n_);
+        for (int q = 0; q < n_; ++q)
+            dst[q] = q;
         for (size_t i = 0; i < glob.size(); ++i)
         {
-            loc_log[i] = p2l_[loc[i]];
-            glob_log[i] = p2l_[glob[i]];
+            dst[glob[i]] = loc[i];
+            dst[loc[i]] = glob[i];
         }
-        for (size_t i = 0; i < glob.size(); ++i)
+        // one swap of two qubits at 3 or above: the SWAP kernel moves half the state,
+        // coalesced; anything else takes one permutation sweep for all the swaps
+        if (!fused_ && glob.size() == 1 && std::min(glob[0], loc[0]) >= 3)
         {
-            p2l_[glob[i]] = loc_log[i];
-            p2l_[loc[i]] = glob_log[i];
-            l2p_[glob_log[i]] = loc[i];
-            l2p_[loc_log[i]] = glob[i];
+            state_changed();
+            swap_physical(glob[0], loc[0]);
+            remap_physical(dst);
+        }
+        else
+        {
+            move_physical(dst);
         }
     }
 
     void Simulator::operate(const GateSpec &g, std::vector<int> targets, std::vector<int> neg, std::vector<int> pos)
     {
         need_alloc();
+        state_changed();
         if ((int)targets.size() != g.ntargets)
             throw std::runtime_error(fmt("gate needs %d target qubit(s), %d given", g.ntargets, (int)targets.size()));
         check_operands(targets, neg, pos);
@@ -567,6 +613,7 @@ namespace qcore
         if (contains(m1_, q))
             return 1;
         flush();
+        state_changed();
         std::vector<int> tphys = {l2p_[q]};
         ensure_local(tphys);
         const int tq = tphys[0];
@@ -674,6 +721,7 @@ namespace qcore
         flush();
         if (in == nullptr || count != (uint64_t(1) << n_))
             throw std::runtime_error("statevector buffer size mismatch");
+        state_changed();
         if (fused_)
             fused_->drop();
         ensure_stage();
@@ -686,6 +734,8 @@ namespace qcore
             with_state([&](auto *st) { qsv::launch_scatter_logical(stream_, st, stage_dev_, first, chunk, l2p_.data(), n_); });
         }
         qsv::check(hipStreamSynchronize(stream_), "sync");
+        // a new state: earlier measurement results no longer describe it
+        clear_measurement_state();
     }
 
     void Simulator::load_statevector(const char *path)
@@ -707,6 +757,7 @@ namespace qcore
             if (size < 0 || uint64_t(size) != count * sizeof(qsv::c128))
                 throw std::runtime_error("statevector file holds " + std::to_string(size) + " bytes, expected " + std::to_string(count * sizeof(qsv::c128)));
             std::rewind(fp);
+            state_changed();
             if (fused_)
                 fused_->drop();
             ensure_stage();
@@ -750,13 +801,19 @@ namespace qcore
     {
         need_alloc();
         flush();
+        // the levels are built once per state: sample() after probability_total() (or a
+        // second probability_total()) on an unchanged state reuses them
+        if (levels_valid_)
+            return levels_total_;
         ensure_levels();
         prof_begin(QCS_EXT_K_REDUCE, (c64_state() ? 8.0 : 16.0) * (1ull << n_));
         with_state([&](auto *st) { qsv::build_fold_levels(stream_, st, uint64_t(1) << n_, levels_); });
         prof_end();
         qsv::check(hipMemcpyAsync(pinned_, levels_.back(), sizeof(double), hipMemcpyDeviceToHost, stream_), "copy");
         qsv::check(hipStreamSynchronize(stream_), "sync");
-        return pinned_[0];
+        levels_total_ = pinned_[0];
+        levels_valid_ = true;
+        return levels_total_;
     }
 
     void Simulator::sample(uint64_t seed, uint64_t shots, uint64_t *out)
@@ -832,6 +889,7 @@ namespace qcore
     {
         need_alloc();
         flush();
+        state_changed();
         if ((int)want.size() != n_)
             throw std::runtime_error("mapping size mismatch");
         std::vector<bool> used(n_, false);
@@ -841,19 +899,11 @@ namespace qcore
                 throw std::runtime_error("not a permutation");
             used[l] = true;
         }
-        // move logical qubit want[q] into physical slot q, one qubit swap at a time
+        // logical qubit want[q] goes to physical slot q
+        std::vector<int> dst(n_);
         for (int q = 0; q < n_; ++q)
-        {
-            int cur = l2p_[want[q]];
-            if (cur == q)
-                continue;
-            swap_physical(q, cur);
-            int lq = p2l_[q], lc = p2l_[cur];
-            p2l_[q] = lc;
-            p2l_[cur] = lq;
-            l2p_[lc] = q;
-            l2p_[lq] = cur;
-        }
+            dst[l2p_[want[q]]] = q;
+        move_physical(dst);
     }
 
     void Simulator::overlap(Simulator &other, double out[3])
@@ -904,6 +954,7 @@ namespace qcore
             throw std::runtime_error("recording block timings needs a fused precision mode (1, 2 or 3, allocated)");
         if (count < 0 || (count > 0 && (!features || !log2_us)))
             throw std::runtime_error("planner_record: bad output buffers");
+        state_changed();
         fused_->record(count, seed, features, log2_us);
     }
 

@@ -92,6 +92,8 @@ namespace qcore
         void check_operands(const std::vector<int> &t, const std::vector<int> &ng, const std::vector<int> &ps) const;
         void ensure_local(std::vector<int> &tphys);
         void swap_physical(int a, int b);
+        void move_physical(const std::vector<int> &dst);
+        void remap_physical(const std::vector<int> &dst);
         void apply_resolved(const ResolvedGate &r);
         void ensure_levels();
         uint64_t logical_chunk() const;
@@ -127,6 +129,11 @@ namespace qcore
         double *work_ = nullptr;
         uint64_t work_len_ = 0;
         std::vector<double *> levels_;
+        // the sampling prefix levels (and their total) describe the current state; cleared
+        // by every call that may change the state
+        bool levels_valid_ = false;
+        double levels_total_ = 0.0;
+        void state_changed() { levels_valid_ = false; }
         double *pinned_ = nullptr; // 64 doubles of pinned host memory
         qsv::c128 *stage_dev_ = nullptr;
         void *stage_host_ = nullptr;

@@ -515,8 +515,10 @@ namespace qsv
         // entry 64*g + L = (((x0 + x1) + x2) + ... + x63) over entries 4096*g + 64*L + 0..63.
         // The 4096 entries go through LDS in 4 chunks of 16 columns, so that every load
         // instruction covers runs of 16 consecutive entries (256 B of complex128 state) and
-        // lane L then continues its own row's sum over the chunk in order.  4 waves per
-        // workgroup, 8.5 KiB of LDS per wave.
+        // lane L then continues its own row's sum over the chunk in order.  The loads of the
+        // next chunk are issued before the current chunk is summed, so each wave keeps one
+        // chunk (16 KiB) in flight while it works.  4 waves per workgroup, 8.5 KiB of LDS
+        // per wave.
         constexpr int kFoldWaves = 4;
         constexpr int kFoldRow = 17; // doubles per LDS row (16 + 1 of padding)
 
@@ -530,59 +532,73 @@ namespace qsv
             if (group >= ngroups) // whole wave leaves together; no workgroup barrier below
                 return;
             double *tl = tiles[wave];
-            double acc = 0.0;
             // 8-byte entries (complex64 amplitudes, level sums) go two per lane so that each
-            // load moves 16 B: lane = (run, column pair); 16-byte amplitudes one per lane
+            // load moves 16 B: lane = (row, column pair); 16-byte amplitudes one per lane
             constexpr bool PAIRS = !(FROM_STATE && sizeof(A) == 16);
+            constexpr int NL = PAIRS ? 8 : 16;   // 16-byte loads per lane and chunk
+            constexpr int RUNS = PAIRS ? 8 : 4;  // rows covered by one load instruction
+            constexpr int ES = FROM_STATE ? int(sizeof(A)) : 8; // bytes per entry
+            const int run = PAIRS ? (lane >> 3) : (lane >> 4);
+            const int col = PAIRS ? 2 * (lane & 7) : (lane & 15); // the lane's first column
+            const char *bytes = static_cast<const char *>(src);
+            // raw 16 bytes per load: one complex128, two complex64 or two level sums; entries
+            // at or past `count` read as zero
+            auto load = [&](int chunk, v2d(&r)[NL]) {
+#pragma unroll
+                for (int j = 0; j < NL; ++j)
+                {
+                    const uint64_t e = group * 4096 + uint64_t(RUNS * j + run) * 64 + 16 * chunk + col;
+                    const v2d *p = reinterpret_cast<const v2d *>(bytes + e * ES);
+                    if (e + (PAIRS ? 1 : 0) < count)
+                        r[j] = FROM_STATE ? __builtin_nontemporal_load(p) : *p;
+                    else if (PAIRS && e < count)
+                    {
+                        if (FROM_STATE)
+                        {
+                            const v2f f = *reinterpret_cast<const v2f *>(p);
+                            r[j] = __builtin_bit_cast(v2d, v4f{f.x, f.y, 0.0f, 0.0f});
+                        }
+                        else
+                            r[j] = v2d{*reinterpret_cast<const double *>(p), 0.0};
+                    }
+                    else
+                        r[j] = v2d{0.0, 0.0};
+                }
+            };
+            auto put = [&](const v2d(&r)[NL]) {
+#pragma unroll
+                for (int j = 0; j < NL; ++j)
+                {
+                    const int row = RUNS * j + run;
+                    if (PAIRS)
+                    {
+                        double v0 = r[j].x, v1 = r[j].y;
+                        if (FROM_STATE)
+                        {
+                            const v4f z = __builtin_bit_cast(v4f, r[j]);
+                            v0 = prob(v2d{double(z.x), double(z.y)});
+                            v1 = prob(v2d{double(z.z), double(z.w)});
+                        }
+                        tl[row * kFoldRow + col] = v0;
+                        tl[row * kFoldRow + col + 1] = v1;
+                    }
+                    else
+                    {
+                        tl[row * kFoldRow + col] = prob(r[j]);
+                    }
+                }
+            };
+            v2d ra[NL], rb[NL];
+            load(0, ra);
+            double acc = 0.0;
 #pragma unroll
             for (int chunk = 0; chunk < 4; ++chunk)
             {
-                if (PAIRS)
-                {
-                    const int run = lane >> 3, pc = lane & 7;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j)
-                    {
-                        const int r = 8 * j + run;
-                        const uint64_t e = group * 4096 + uint64_t(r) * 64 + 16 * chunk + 2 * pc;
-                        double v0 = 0.0, v1 = 0.0;
-                        if (e + 1 < count)
-                        {
-                            if (FROM_STATE)
-                            {
-                                const v4f z = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(static_cast<const A *>(src) + e));
-                                v0 = prob(v2d{double(z.x), double(z.y)});
-                                v1 = prob(v2d{double(z.z), double(z.w)});
-                            }
-                            else
-                            {
-                                const v2d z = *reinterpret_cast<const v2d *>(static_cast<const double *>(src) + e);
-                                v0 = z.x;
-                                v1 = z.y;
-                            }
-                        }
-                        else if (e < count)
-                        {
-                            v0 = FROM_STATE ? prob(lda(static_cast<const A *>(src) + e)) : static_cast<const double *>(src)[e];
-                        }
-                        tl[r * kFoldRow + 2 * pc] = v0;
-                        tl[r * kFoldRow + 2 * pc + 1] = v1;
-                    }
-                }
-                else
-                {
-                    const int run = lane >> 4, col = lane & 15;
-#pragma unroll
-                    for (int j = 0; j < 16; ++j)
-                    {
-                        const int r = 4 * j + run;
-                        const uint64_t e = group * 4096 + uint64_t(r) * 64 + 16 * chunk + col;
-                        double v = 0.0;
-                        if (e < count)
-                            v = prob(lda(static_cast<const A *>(src) + e));
-                        tl[r * kFoldRow + col] = v;
-                    }
-                }
+                v2d(&cur)[NL] = (chunk & 1) ? rb : ra;
+                v2d(&nxt)[NL] = (chunk & 1) ? ra : rb;
+                if (chunk < 3)
+                    load(chunk + 1, nxt);
+                put(cur);
                 wave_sync();
 #pragma unroll
                 for (int c = 0; c < 16; ++c)
@@ -658,6 +674,116 @@ namespace qsv
             for (int q = 0; q < map.n; ++q)
                 logical |= ((block >> q) & 1) << map.p2l[q];
             out[shot] = logical;
+        }
+
+        // ---------------- qubit permutation ----------------
+        // A tile is 2^m amplitudes whose index bits at the tile qubits T[0..m-1] vary (the
+        // moved qubits plus the lowest unmoved ones, so T holds a run of low qubits and every
+        // load and store instruction covers contiguous runs).  The workgroup loads its tile
+        // into LDS in tile order, then writes output tile position y from tile position
+        // x(y) = OR over b of bit b of y shifted to src[b], the input tile bit that lands at
+        // output bit b.  Input and output use the same address map, so the tile is updated in
+        // place and workgroups own disjoint addresses.
+        constexpr int kPermThreads = 512;
+        constexpr int kPermLds = 65536; // bytes
+
+        struct PermuteOp
+        {
+            int m;           // tile bits
+            int src[16];     // for output tile bit b: the input tile bit that moves there
+            uint64_t gw[16]; // global stride of tile bit b
+        };
+
+        // LDS slot of tile position x: the entries of one 128-B LDS row (8 complex128 or 16
+        // complex64) XOR-ed with every higher field of the same width, so that entries 2^h
+        // apart (a moved low qubit) fall in different banks.
+        template <class A>
+        __device__ __forceinline__ int perm_slot(int x)
+        {
+            if constexpr (sizeof(A) == 16)
+                return x ^ (((x >> 3) ^ (x >> 6) ^ (x >> 9)) & 7);
+            else
+                return x ^ (((x >> 4) ^ (x >> 8) ^ (x >> 12)) & 15);
+        }
+
+        template <class A>
+        __device__ __forceinline__ A ld_amp(const A *p)
+        {
+            if constexpr (sizeof(A) == 16)
+                return __builtin_bit_cast(A, __builtin_nontemporal_load(reinterpret_cast<const v2d *>(p)));
+            else
+                return __builtin_bit_cast(A, __builtin_nontemporal_load(reinterpret_cast<const v2f *>(p)));
+        }
+        template <class A>
+        __device__ __forceinline__ void st_amp(A *p, A v)
+        {
+            if constexpr (sizeof(A) == 16)
+                __builtin_nontemporal_store(__builtin_bit_cast(v2d, v), reinterpret_cast<v2d *>(p));
+            else
+                __builtin_nontemporal_store(__builtin_bit_cast(v2f, v), reinterpret_cast<v2f *>(p));
+        }
+
+        template <class A>
+        __global__ __launch_bounds__(kPermThreads) void k_permute(A *__restrict__ s, uint64_t ntiles, Subspace tsub, PermuteOp op)
+        {
+            constexpr int CAP = kPermLds / int(sizeof(A));
+            constexpr int PER = CAP / kPermThreads; // tile positions per thread at most
+            __shared__ __attribute__((aligned(16))) A tile[CAP];
+            const int nt = 1 << op.m;
+            const int t = threadIdx.x;
+            // the thread's low 9 position bits: global offset, and source position on output
+            uint64_t g_t = 0;
+            int x_t = 0;
+            for (int b = 0; b < 9 && b < op.m; ++b)
+                if ((t >> b) & 1)
+                {
+                    g_t += op.gw[b];
+                    x_t |= 1 << op.src[b];
+                }
+            for (uint64_t tl = xcd_block(); tl < ntiles; tl += gridDim.x)
+            {
+                const uint64_t base = expand(tl, tsub);
+                A a[PER];
+#pragma unroll
+                for (int i = 0; i < PER; ++i)
+                {
+                    const int x = i * kPermThreads + t;
+                    if (x < nt)
+                    {
+                        uint64_t g = base + g_t;
+                        for (int b = 9; b < op.m; ++b) // uniform
+                            if (((i * kPermThreads) >> b) & 1)
+                                g += op.gw[b];
+                        a[i] = ld_amp(s + g);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < PER; ++i)
+                {
+                    const int x = i * kPermThreads + t;
+                    if (x < nt)
+                        tile[perm_slot<A>(x)] = a[i];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < PER; ++i)
+                {
+                    const int y = i * kPermThreads + t;
+                    if (y < nt)
+                    {
+                        uint64_t g = base + g_t;
+                        int x = x_t;
+                        for (int b = 9; b < op.m; ++b) // uniform
+                            if (((i * kPermThreads) >> b) & 1)
+                            {
+                                g += op.gw[b];
+                                x |= 1 << op.src[b];
+                            }
+                        st_amp(s + g, tile[perm_slot<A>(x)]);
+                    }
+                }
+                __syncthreads(); // the next tile's positions are written to LDS
+            }
         }
 
         // ---------------- layout ----------------
@@ -1104,6 +1230,129 @@ namespace qsv
     }
 
     template <class A>
+    int permute_tile_bits() { return kPermLds / int(sizeof(A)) == 4096 ? 12 : 13; }
+
+    template <class A>
+    void launch_permute(hipStream_t s, A *state, int n, const std::vector<int> &dst)
+    {
+        if (n < 1 || n > kMaxQubits || (int)dst.size() != n)
+            throw std::runtime_error("permute: permutation size does not match the state");
+        std::vector<int> seen(n, 0), moved;
+        for (int q = 0; q < n; ++q)
+        {
+            if (dst[q] < 0 || dst[q] >= n || seen[dst[q]]++)
+                throw std::runtime_error("permute: not a permutation of the qubits");
+            if (dst[q] != q)
+                moved.push_back(q);
+        }
+        if (moved.empty())
+            return;
+        const int m = std::min(n, permute_tile_bits<A>());
+        if ((int)moved.size() > m)
+            throw std::runtime_error("permute: more moved qubits than tile bits");
+        // tile bits: the moved qubits and the lowest unmoved ones
+        std::vector<int> tb = moved;
+        for (int q = 0; (int)tb.size() < m; ++q)
+            if (dst[q] == q)
+                tb.push_back(q);
+        std::sort(tb.begin(), tb.end());
+        std::vector<int> pos(n, -1);
+        for (int b = 0; b < m; ++b)
+            pos[tb[b]] = b;
+        PermuteOp op{};
+        op.m = m;
+        Subspace tsub{};
+        tsub.nops = m;
+        for (int b = 0; b < m; ++b)
+        {
+            op.gw[b] = uint64_t(1) << tb[b];
+            tsub.ops[b] = tb[b];
+            const int out = pos[dst[tb[b]]];
+            if (out < 0)
+                throw std::runtime_error("permute: tile bits not closed under the permutation");
+            op.src[out] = b;
+        }
+        const uint64_t ntiles = uint64_t(1) << (n - m);
+        hipLaunchKernelGGL(k_permute<A>, dim3(grid_for(ntiles, 1)), dim3(kPermThreads), 0, s, state, ntiles, tsub, op);
+        after_launch("permute kernel");
+    }
+
+    std::vector<std::vector<int>> plan_permute_passes(const std::vector<int> &dst, int budget, int free_below)
+    {
+        const int n = (int)dst.size();
+        if (budget < 2)
+            throw std::runtime_error("permute: pass budget below 2");
+        std::vector<int> seen(n, 0);
+        for (int q = 0; q < n; ++q)
+            if (dst[q] < 0 || dst[q] >= n || seen[dst[q]]++)
+                throw std::runtime_error("permute: not a permutation of the qubits");
+        auto cost = [&](int q) { return q >= free_below ? 1 : 0; };
+        std::vector<int> rest = dst; // still to do: data now at q goes to rest[q]
+        std::vector<std::vector<int>> passes;
+        for (;;)
+        {
+            std::vector<int> pass(n);
+            for (int q = 0; q < n; ++q)
+                pass[q] = q;
+            int used = 0;
+            bool any = false;
+            std::vector<char> done(n, 0);
+            for (int q0 = 0; q0 < n; ++q0)
+            {
+                if (done[q0] || rest[q0] == q0)
+                    continue;
+                std::vector<int> cyc; // q0 -> rest[q0] -> ...
+                for (int q = q0; !done[q]; q = rest[q])
+                {
+                    done[q] = 1;
+                    cyc.push_back(q);
+                }
+                int c = 0;
+                for (int q : cyc)
+                    c += cost(q);
+                if (used + c <= budget)
+                {
+                    for (int q : cyc)
+                        pass[q] = rest[q];
+                    used += c;
+                    any = true;
+                    continue;
+                }
+                // the longest prefix c1..cr that fits: c_i -> c_{i+1}, c_r -> c1; what
+                // remains of the cycle is (c1 c_{r+1} ... c_L)
+                int r = 0, cr = 0;
+                while (r < (int)cyc.size() && used + cr + cost(cyc[r]) <= budget)
+                    cr += cost(cyc[r++]);
+                if (r < 2)
+                    continue;
+                for (int i = 0; i + 1 < r; ++i)
+                    pass[cyc[i]] = cyc[i + 1];
+                pass[cyc[r - 1]] = cyc[0];
+                used += cr;
+                any = true;
+            }
+            if (!any)
+                break;
+            std::vector<int> hit(n, 0);
+            for (int q = 0; q < n; ++q)
+                if (hit[pass[q]]++)
+                    throw std::runtime_error("permute: a planned pass is not a permutation");
+            if ((int)passes.size() >= n)
+                throw std::runtime_error("permute: pass planning does not converge");
+            // rest' o pass = rest: the data now at pass[q] still goes to rest[q]
+            std::vector<int> next(n);
+            for (int q = 0; q < n; ++q)
+                next[pass[q]] = rest[q];
+            rest = next;
+            passes.push_back(pass);
+        }
+        for (int q = 0; q < n; ++q)
+            if (rest[q] != q)
+                throw std::runtime_error("permute: pass planning did not finish");
+        return passes;
+    }
+
+    template <class A>
     void launch_gather_logical(hipStream_t s, const A *state, c128 *out, uint64_t first, uint64_t len, const int *l2p, int n)
     {
         L2P m{};
@@ -1137,7 +1386,9 @@ namespace qsv
     template void build_fold_levels<A>(hipStream_t, const A *, uint64_t, const std::vector<double *> &);                                                \
     template void launch_sample_search<A>(hipStream_t, const A *, uint64_t, const std::vector<double *> &, double, uint64_t, uint64_t, const int *, int, uint64_t *); \
     template void launch_gather_logical<A>(hipStream_t, const A *, c128 *, uint64_t, uint64_t, const int *, int);                                      \
-    template void launch_scatter_logical<A>(hipStream_t, A *, const c128 *, uint64_t, uint64_t, const int *, int);
+    template void launch_scatter_logical<A>(hipStream_t, A *, const c128 *, uint64_t, uint64_t, const int *, int); \
+    template int permute_tile_bits<A>();                                                                                                                \
+    template void launch_permute<A>(hipStream_t, A *, int, const std::vector<int> &);
 
     QSV_INSTANTIATE(c128)
     QSV_INSTANTIATE(c64)

@@ -121,6 +121,19 @@ namespace qsv
     template <class A>
     void launch_sample_search(hipStream_t s, const A *state, uint64_t count, const std::vector<double *> &levels, double total, uint64_t seed, uint64_t nshots, const int *p2l, int n, uint64_t *out_dev);
 
+    // ---- qubit permutation (the in-HBM replacement of the reference's rank exchange) ----
+    // Moves the data of every physical qubit q to physical qubit dst[q] in one sweep: the
+    // amplitude at index i goes to the index whose bit dst[q] is bit q of i.  The moved
+    // qubits (dst[q] != q) and the lowest unmoved ones form the tile bits (12 for
+    // complex128, 13 for complex64: 64 KiB of LDS); a workgroup loads a tile, then writes
+    // it back permuted.  At most permute_tile_bits(A) qubits may move (checked).
+    template <class A> int permute_tile_bits();
+    template <class A> void launch_permute(hipStream_t s, A *state, int n, const std::vector<int> &dst);
+    // Splits a permutation (qubit q -> dst[q]) into passes for launch_permute, applied in
+    // order, each moving at most `budget` qubits at or above `free_below` (qubits below it
+    // are in every tile anyway).  budget >= 2.
+    std::vector<std::vector<int>> plan_permute_passes(const std::vector<int> &dst, int budget, int free_below);
+
     // ---- layout: logical-order staging is always complex128 ----
     template <class A>
     void launch_gather_logical(hipStream_t s, const A *state, c128 *out, uint64_t first, uint64_t len, const int *l2p, int n);
