@@ -75,6 +75,18 @@ int qcs_ext_load_statevector(qcs_simulator *sim, const char *path);
 /* Seeded sampling of num_shots full-register outcomes (logical indices). */
 int qcs_ext_sample(qcs_simulator *sim, uint64_t seed, uint64_t num_shots, uint64_t *outcomes);
 
+/* qcs_ext_sample without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The prefix levels and the outcomes live in one
+ * file-scope device array of libqcs.so (130.5 MiB, held by every GPU the library runs on
+ * until the process ends), so the limits are num_qubits <= 30 and num_shots <= 65536;
+ * beyond them the call fails.  Gates queued by a fused precision mode must be flushed
+ * first (qcs_ext_flush), since a flush may allocate.  Outcomes equal qcs_ext_sample's.
+ * The array keeps the last call's contents; calls that overlap on one GPU (from several
+ * threads) race on it.  The oracle library checks the same limits and samples as
+ * qcs_ext_sample. */
+int qcs_ext_sample_noalloc(qcs_simulator *sim, uint64_t seed, uint64_t num_shots, uint64_t *outcomes);
+
 /* The probability total of the prefix defined above, and the canonical sum of p(i). */
 int qcs_ext_probability_total(qcs_simulator *sim, double *total);
 int qcs_ext_norm(qcs_simulator *sim, double *norm);

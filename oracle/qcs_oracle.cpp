@@ -1297,6 +1297,21 @@ extern "C"
             c.sample(sd, shots, out);
         });
     }
+    int qcs_ext_sample_noalloc(qcs_simulator *s, uint64_t sd, uint64_t shots, uint64_t *out)
+    {
+        // the limits of libqcs.so's scratch array, then qcs_ext_sample
+        return guard([&] {
+            auto &c = core(s);
+            c.need_alloc();
+            if (c.n > 30)
+                throw std::runtime_error(oracle::fmt("sample_noalloc: %d qubits; the scratch array holds the prefix of at most 30", c.n));
+            if (shots > 65536)
+                throw std::runtime_error("sample_noalloc: more than 65536 shots");
+            if (shots > 0 && out == nullptr)
+                throw std::runtime_error("outcome buffer must not be NULL");
+            c.sample(sd, shots, out);
+        });
+    }
     int qcs_ext_probability_total(qcs_simulator *s, double *total)
     {
         return guard([&] {

@@ -413,6 +413,7 @@ extern "C"
     int qcs_ext_set_statevector(qcs_simulator *s, const double *in, uint64_t n) { return guard([&] { core(s).set_statevector(in, n); }); }
     int qcs_ext_load_statevector(qcs_simulator *s, const char *path) { return guard([&] { core(s).load_statevector(path); }); }
     int qcs_ext_sample(qcs_simulator *s, uint64_t sd, uint64_t shots, uint64_t *out) { return guard([&] { core(s).sample(sd, shots, out); }); }
+    int qcs_ext_sample_noalloc(qcs_simulator *s, uint64_t sd, uint64_t shots, uint64_t *out) { return guard([&] { core(s).sample_noalloc(sd, shots, out); }); }
     int qcs_ext_probability_total(qcs_simulator *s, double *total) { return guard([&] { put(total, core(s).probability_total()); }); }
     int qcs_ext_norm(qcs_simulator *s, double *norm) { return guard([&] { put(norm, core(s).norm()); }); }
     int qcs_ext_expect_z(qcs_simulator *s, double *out, int count) { return guard([&] { core(s).expect_z(out, count); }); }

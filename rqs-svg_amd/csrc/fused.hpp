@@ -175,6 +175,7 @@ namespace qcore
         void enqueue(const ResolvedGate &g);
         void flush();
         void drop() { queue_.clear(); }
+        bool has_queued() const { return !queue_.empty(); }
         // LDS tile bits of the stage kernel (11, 12 or 13), or 0 for one launch per block.
         void set_stage_bits(int m);
         int stage_bits() const { return stage_bits_; }

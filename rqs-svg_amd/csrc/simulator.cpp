@@ -843,6 +843,14 @@ namespace qcore
         (void)hipFree(dev);
     }
 
+    void Simulator::sample_noalloc(uint64_t seed, uint64_t shots, uint64_t *out)
+    {
+        need_alloc();
+        if (fused_ && fused_->has_queued())
+            throw std::runtime_error("sample_noalloc: fused gates are queued and their flush may allocate; call qcs_ext_flush first");
+        with_state([&](auto *st) { qsv::sample_noalloc(stream_, device_, st, n_, seed, shots, p2l_.data(), pinned_, out); });
+    }
+
     double Simulator::norm()
     {
         need_alloc();

@@ -57,6 +57,9 @@ namespace qcore
         void set_statevector(const double *in, uint64_t count);
         void load_statevector(const char *path);
         void sample(uint64_t seed, uint64_t shots, uint64_t *out);
+        // sample() with the prefix levels and outcomes in the library's scratch array: no
+        // allocation; refuses queued fused gates (their flush may allocate)
+        void sample_noalloc(uint64_t seed, uint64_t shots, uint64_t *out);
         double probability_total();
         double norm();
         void expect_z(double *out, int count);

@@ -18,6 +18,7 @@
 #include "sv_kernels.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -616,8 +617,8 @@ namespace qsv
         struct Levels
         {
             int nlev;               // number of levels including level 0 (the state)
-            uint64_t size[12];
-            const double *ptr[12];  // ptr[0] unused (state)
+            uint64_t size[kMaxFoldLevels];
+            const double *ptr[kMaxFoldLevels]; // ptr[0] unused (state)
         };
 
         struct P2L
@@ -639,7 +640,7 @@ namespace qsv
             if (shot >= nshots)
                 return;
             const double r = uniform(seed, shot) * total;
-            double offs[12];
+            double offs[kMaxFoldLevels];
             int depth = 0;
             uint64_t block = 0;
             for (int lev = L.nlev - 2; lev >= 0; --lev)
@@ -1187,46 +1188,151 @@ namespace qsv
         return sz;
     }
 
+    namespace
+    {
+        // sz[0] = count, sz[l] = ceil(sz[l-1] / 64) down to 1; returns the number of sizes
+        int fold_sizes(uint64_t count, uint64_t (&sz)[kMaxFoldLevels])
+        {
+            int ns = 0;
+            sz[ns++] = count;
+            while (sz[ns - 1] > 1)
+            {
+                if (ns == kMaxFoldLevels)
+                    throw std::runtime_error("fold: too many levels");
+                sz[ns] = (sz[ns - 1] + 63) / 64;
+                ++ns;
+            }
+            return ns;
+        }
+
+        // levels[l - 1] receives level l of the fold (sizes sz[1 .. ns-1])
+        template <class A>
+        void fold_into(hipStream_t s, const A *state, const uint64_t *sz, int ns, double *const *levels)
+        {
+            for (int lev = 1; lev < ns; ++lev)
+            {
+                const uint64_t nout = sz[lev];
+                const uint64_t groups = (nout + 63) / 64;
+                const uint64_t blocks = (groups + kFoldWaves - 1) / kFoldWaves;
+                if (lev == 1)
+                    hipLaunchKernelGGL((k_fold64<true, A>), dim3(blocks), dim3(64 * kFoldWaves), 0, s, static_cast<const void *>(state), sz[0], levels[0], nout, groups);
+                else
+                    hipLaunchKernelGGL((k_fold64<false, A>), dim3(blocks), dim3(64 * kFoldWaves), 0, s, static_cast<const void *>(levels[lev - 2]), sz[lev - 1], levels[lev - 1], nout, groups);
+                after_launch("fold kernel");
+            }
+        }
+
+        template <class A>
+        void search_into(hipStream_t s, const A *state, const uint64_t *sz, int ns, const double *const *levels, double total, uint64_t seed, uint64_t nshots, const int *p2l, int n, uint64_t *out_dev)
+        {
+            if (ns > kMaxFoldLevels || n > kMaxQubits)
+                throw std::runtime_error("sample: bad level layout");
+            Levels L{};
+            L.nlev = ns;
+            for (int i = 0; i < ns; ++i)
+            {
+                L.size[i] = sz[i];
+                L.ptr[i] = i == 0 ? nullptr : levels[i - 1];
+            }
+            P2L m{};
+            m.n = n;
+            for (int q = 0; q < n; ++q)
+                m.p2l[q] = p2l[q];
+            if (nshots == 0)
+                return;
+            hipLaunchKernelGGL(k_sample_search<A>, dim3((nshots + 63) / 64), dim3(64), 0, s, state, L, total, seed, nshots, m, out_dev);
+            after_launch("sample kernel");
+        }
+    } // namespace
+
     template <class A>
     void build_fold_levels(hipStream_t s, const A *state, uint64_t count, const std::vector<double *> &levels)
     {
-        auto sz = fold_level_sizes(count);
-        if (levels.size() + 1 != sz.size())
+        uint64_t sz[kMaxFoldLevels];
+        const int ns = fold_sizes(count, sz);
+        if ((int)levels.size() + 1 != ns)
             throw std::runtime_error("fold: level buffer count mismatch");
-        for (size_t lev = 1; lev < sz.size(); ++lev)
-        {
-            const uint64_t nout = sz[lev];
-            const uint64_t groups = (nout + 63) / 64;
-            const uint64_t blocks = (groups + kFoldWaves - 1) / kFoldWaves;
-            if (lev == 1)
-                hipLaunchKernelGGL((k_fold64<true, A>), dim3(blocks), dim3(64 * kFoldWaves), 0, s, static_cast<const void *>(state), sz[0], levels[0], nout, groups);
-            else
-                hipLaunchKernelGGL((k_fold64<false, A>), dim3(blocks), dim3(64 * kFoldWaves), 0, s, static_cast<const void *>(levels[lev - 2]), sz[lev - 1], levels[lev - 1], nout, groups);
-            after_launch("fold kernel");
-        }
+        fold_into(s, state, sz, ns, levels.data());
     }
 
     template <class A>
     void launch_sample_search(hipStream_t s, const A *state, uint64_t count, const std::vector<double *> &levels, double total, uint64_t seed, uint64_t nshots, const int *p2l, int n, uint64_t *out_dev)
     {
-        auto sz = fold_level_sizes(count);
-        if (sz.size() > 12 || levels.size() + 1 != sz.size() || n > kMaxQubits)
+        uint64_t sz[kMaxFoldLevels];
+        const int ns = fold_sizes(count, sz);
+        if ((int)levels.size() + 1 != ns)
             throw std::runtime_error("sample: bad level layout");
-        Levels L{};
-        L.nlev = (int)sz.size();
-        for (size_t i = 0; i < sz.size(); ++i)
+        search_into(s, state, sz, ns, levels.data(), total, seed, nshots, p2l, n, out_dev);
+    }
+
+    // ---- sampling without allocation ----
+    // The prefix levels (each level's start 256-B aligned) and then the outcomes.  Every
+    // byte that a call reads it has written first in the same call (the fold writes each
+    // level entry, the search each outcome), so nothing is cleared between calls.
+    static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+    namespace
+    {
+        std::mutex scratch_mu;
+        void *scratch_ptr[kNoAllocMaxDevices] = {};
+
+        // this device's address of `scratch`, looked up once per device with it current
+        unsigned char *scratch_on(int device)
         {
-            L.size[i] = sz[i];
-            L.ptr[i] = i == 0 ? nullptr : levels[i - 1];
+            if (device < 0 || device >= kNoAllocMaxDevices)
+                throw std::runtime_error("sample_noalloc: device index out of range");
+            std::lock_guard<std::mutex> lock(scratch_mu);
+            if (!scratch_ptr[device])
+            {
+                int prev = -1;
+                check(hipGetDevice(&prev), "hipGetDevice");
+                if (prev != device)
+                    check(hipSetDevice(device), "hipSetDevice");
+                void *p = nullptr;
+                const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+                if (prev != device)
+                    check(hipSetDevice(prev), "hipSetDevice");
+                check(e, "hipGetSymbolAddress(scratch)");
+                if (!p)
+                    throw std::runtime_error("hipGetSymbolAddress(scratch) returned NULL");
+                scratch_ptr[device] = p;
+            }
+            return static_cast<unsigned char *>(scratch_ptr[device]);
         }
-        P2L m{};
-        m.n = n;
-        for (int q = 0; q < n; ++q)
-            m.p2l[q] = p2l[q];
+    } // namespace
+
+    template <class A>
+    double sample_noalloc(hipStream_t s, int device, const A *state, int n, uint64_t seed, uint64_t nshots, const int *p2l, double *pinned, uint64_t *out)
+    {
+        if (n < 1 || n > kNoAllocMaxQubits)
+            throw std::runtime_error("sample_noalloc: " + std::to_string(n) + " qubits; the scratch array holds the prefix of at most " + std::to_string(kNoAllocMaxQubits));
+        if (nshots > kNoAllocMaxShots)
+            throw std::runtime_error("sample_noalloc: " + std::to_string(nshots) + " shots; the scratch array holds at most " + std::to_string(kNoAllocMaxShots));
+        if (nshots > 0 && out == nullptr)
+            throw std::runtime_error("outcome buffer must not be NULL");
+        unsigned char *base = scratch_on(device);
+        uint64_t sz[kMaxFoldLevels];
+        const int ns = fold_sizes(uint64_t(1) << n, sz);
+        double *levels[kMaxFoldLevels] = {};
+        uint64_t off = 0;
+        for (int lev = 1; lev < ns; ++lev)
+        {
+            levels[lev - 1] = reinterpret_cast<double *>(base + off);
+            off += (sz[lev] * sizeof(double) + 255) / 256 * 256;
+        }
+        if (off > kNoAllocLevelBytes)
+            throw std::runtime_error("sample_noalloc: prefix levels exceed the scratch array");
+        uint64_t *outcomes = reinterpret_cast<uint64_t *>(base + kNoAllocLevelBytes);
+        fold_into(s, state, sz, ns, levels);
+        check(hipMemcpyAsync(pinned, levels[ns - 2], sizeof(double), hipMemcpyDeviceToHost, s), "copy");
+        check(hipStreamSynchronize(s), "sync");
+        const double total = pinned[0];
         if (nshots == 0)
-            return;
-        hipLaunchKernelGGL(k_sample_search<A>, dim3((nshots + 63) / 64), dim3(64), 0, s, state, L, total, seed, nshots, m, out_dev);
-        after_launch("sample kernel");
+            return total;
+        search_into(s, state, sz, ns, levels, total, seed, nshots, p2l, n, outcomes);
+        check(hipMemcpyAsync(out, outcomes, nshots * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "copy");
+        check(hipStreamSynchronize(s), "sync");
+        return total;
     }
 
     template <class A>
@@ -1385,6 +1491,7 @@ namespace qsv
     template void launch_collapse<A>(hipStream_t, A *, int, int, int, double);                                                                          \
     template void build_fold_levels<A>(hipStream_t, const A *, uint64_t, const std::vector<double *> &);                                                \
     template void launch_sample_search<A>(hipStream_t, const A *, uint64_t, const std::vector<double *> &, double, uint64_t, uint64_t, const int *, int, uint64_t *); \
+    template double sample_noalloc<A>(hipStream_t, int, const A *, int, uint64_t, uint64_t, const int *, double *, uint64_t *); \
     template void launch_gather_logical<A>(hipStream_t, const A *, c128 *, uint64_t, uint64_t, const int *, int);                                      \
     template void launch_scatter_logical<A>(hipStream_t, A *, const c128 *, uint64_t, uint64_t, const int *, int); \
     template int permute_tile_bits<A>();                                                                                                                \

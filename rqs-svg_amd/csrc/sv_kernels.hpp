@@ -121,6 +121,34 @@ namespace qsv
     template <class A>
     void launch_sample_search(hipStream_t s, const A *state, uint64_t count, const std::vector<double *> &levels, double total, uint64_t seed, uint64_t nshots, const int *p2l, int n, uint64_t *out_dev);
 
+    // ---- sampling without allocation ----
+    // The prefix levels and the outcomes live in a file-scope device array of
+    // kNoAllocScratchBytes, which each GPU holds from the library's load until the process
+    // ends: states of at most kNoAllocMaxQubits qubits, at most kNoAllocMaxShots shots.
+    constexpr int kMaxFoldLevels = 12; // 64-ary levels of up to 2^63 entries, level 0 included
+    constexpr int kNoAllocMaxQubits = 30;
+    constexpr uint64_t kNoAllocMaxShots = uint64_t(1) << 16;
+    constexpr int kNoAllocMaxDevices = 64;
+    // bytes of the fold levels of 2^n entries, each level's start aligned to 256 B
+    constexpr uint64_t fold_level_bytes(int n)
+    {
+        uint64_t c = uint64_t(1) << n, total = 0;
+        while (c > 1)
+        {
+            c = (c + 63) / 64;
+            total += (c * 8 + 255) / 256 * 256;
+        }
+        return total;
+    }
+    constexpr uint64_t kNoAllocLevelBytes = fold_level_bytes(kNoAllocMaxQubits);
+    constexpr uint64_t kNoAllocScratchBytes = kNoAllocLevelBytes + kNoAllocMaxShots * 8;
+    // qcs_ext_sample's result on `state` (2^n amplitudes on `device`, queue `s`), built in
+    // the scratch array: no device or pinned memory is allocated.  pinned: one double of
+    // pinned host memory; out: nshots outcomes (host), logical indices.  Synchronises `s`;
+    // returns the probability total.  Calls that overlap on one device race on the array.
+    template <class A>
+    double sample_noalloc(hipStream_t s, int device, const A *state, int n, uint64_t seed, uint64_t nshots, const int *p2l, double *pinned, uint64_t *out);
+
     // ---- qubit permutation (the in-HBM replacement of the reference's rank exchange) ----
     // Moves the data of every physical qubit q to physical qubit dst[q] in one sweep: the
     // amplitude at index i goes to the index whose bit dst[q] is bit q of i.  The moved

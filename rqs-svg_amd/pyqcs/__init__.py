@@ -116,6 +116,7 @@ class Library:
         lib.qcs_ext_set_statevector.argtypes = [vp, vp, u64]
         lib.qcs_ext_load_statevector.argtypes = [vp, ctypes.c_char_p]
         lib.qcs_ext_sample.argtypes = [vp, u64, u64, vp]
+        lib.qcs_ext_sample_noalloc.argtypes = [vp, u64, u64, vp]
         lib.qcs_ext_probability_total.argtypes = [vp, P(ctypes.c_double)]
         lib.qcs_ext_norm.argtypes = [vp, P(ctypes.c_double)]
         lib.qcs_ext_expect_z.argtypes = [vp, vp, ctypes.c_int]
@@ -300,6 +301,12 @@ class Simulator:
     def sample(self, shots, seed):
         out = np.empty(shots, dtype=np.uint64)
         self._call(self.lib.qcs_ext_sample(self.h, seed, shots, out.ctypes.data), "sample")
+        return out
+
+    def sample_noalloc(self, shots, seed):
+        """qcs_ext_sample_noalloc: the same outcomes, with no allocation by the library."""
+        out = np.empty(shots, dtype=np.uint64)
+        self._call(self.lib.qcs_ext_sample_noalloc(self.h, seed, shots, out.ctypes.data), "sample_noalloc")
         return out
 
     def probability_total(self):
