@@ -346,6 +346,126 @@ namespace qsv
             }
         }
 
+        // ---------------- gates with an operand on qubit 0, 1 or 2 ----------------
+        // Amplitudes 1-4 apart (16-64 B) leave per-amplitude accesses covering parts of
+        // lines.  Here a wave takes the 64 consecutive amplitudes of a chunk, one per lane
+        // and per subset of the NH targets at qubit 6 or above (one register each), so every
+        // load and store covers whole lines; the TL = T - NH other targets are lane bits.
+        // The wave puts its amplitudes in LDS and each lane reads the inputs of its output
+        // rows from there.  Lanes outside the pattern of the controls below qubit 6 store
+        // their amplitudes unchanged (writing a line whole costs less than writing part of
+        // it).  Local index of the gate: the lane-bit targets in bits 0..TL-1, the others
+        // above them.
+        struct LowOp
+        {
+            unsigned ctl_mask, ctl_set; // lane bits of the controls below qubit 6, their values
+            unsigned tmask;             // lane bits of the targets below qubit 6
+            unsigned read_mask;         // local inputs that may be non-zero (measured-target rule)
+            unsigned keep_mask;         // local rows left unchanged
+            int dep[16];                // lane bits of low local index kl < 2^TL
+            uint64_t hoff[4];           // address offset of high subset h < 2^NH
+            c128 m[16];                 // dense (T <= 2): row-major 2^T x 2^T
+            int src[16];                // one nonzero per row: out[r] = coef[r] * in[src[r]]
+            c128 coef[16];
+        };
+
+        constexpr int kLowPer = 2; // chunks per wave and tile
+
+        template <int T, int NH, bool MONO, bool NT>
+        __global__ __launch_bounds__(kBlock) void k_low(c128 *__restrict__ s, uint64_t nchunks, Subspace csub, LowOp op)
+        {
+            constexpr int TL = T - NH;
+            constexpr int K = 1 << T;
+            constexpr int H = 1 << NH;
+            constexpr int W = kBlock / 64;
+            static_assert(TL >= 0 && NH <= 2 && (MONO || T <= 2), "shape");
+            __shared__ v2d buf[W][kLowPer][H][64];
+            const int lane = threadIdx.x & 63;
+            const int wave = threadIdx.x >> 6;
+            const bool active = (unsigned(lane) & op.ctl_mask) == op.ctl_set;
+            const int lbase = lane & ~int(op.tmask);
+            int rl = 0; // this lane's low local index
+#pragma unroll
+            for (int i = 0; i < TL; ++i)
+                rl |= ((lane & op.dep[1 << i]) != 0 ? 1 : 0) << i;
+            // per register h: output row r = rl | h << TL, its coefficients and where its
+            // inputs sit in LDS
+            c128 cf[H][MONO ? 1 : K];
+            int s_reg[H], s_lane[H];
+            bool keep[H], readable[H];
+#pragma unroll
+            for (int h = 0; h < H; ++h)
+            {
+                const int r = rl | (h << TL);
+                keep[h] = !active || ((op.keep_mask >> r) & 1);
+                readable[h] = (op.read_mask >> r) & 1;
+                if constexpr (MONO)
+                {
+                    const int sr = op.src[r];
+                    cf[h][0] = op.coef[r];
+                    s_reg[h] = sr >> TL;
+                    s_lane[h] = lbase | op.dep[sr & ((1 << TL) - 1)];
+                }
+                else
+                {
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+                        cf[h][k] = op.m[r * K + k];
+                    s_reg[h] = 0;
+                    s_lane[h] = 0;
+                }
+            }
+            const uint64_t per_tile = uint64_t(W) * kLowPer;
+            const uint64_t ntiles = (nchunks + per_tile - 1) / per_tile;
+            for (uint64_t tile = xcd_block(); tile < ntiles; tile += gridDim.x)
+            {
+                v2d x[kLowPer][H];
+                uint64_t base[kLowPer];
+                bool ok[kLowPer];
+#pragma unroll
+                for (int j = 0; j < kLowPer; ++j)
+                {
+                    const uint64_t chunk = tile * per_tile + uint64_t(j) * W + wave;
+                    ok[j] = chunk < nchunks; // uniform per wave
+                    base[j] = ok[j] ? (expand(chunk, csub) << 6) + lane : 0;
+#pragma unroll
+                    for (int h = 0; h < H; ++h)
+                        x[j][h] = ok[j] ? ldx<NT>(s + (base[j] + op.hoff[h])) : v2d{0.0, 0.0};
+                }
+#pragma unroll
+                for (int j = 0; j < kLowPer; ++j)
+#pragma unroll
+                    for (int h = 0; h < H; ++h)
+                        buf[wave][j][h][lane] = readable[h] ? x[j][h] : v2d{0.0, 0.0};
+                wave_sync();
+#pragma unroll
+                for (int j = 0; j < kLowPer; ++j)
+                {
+                    if (!ok[j])
+                        continue;
+#pragma unroll
+                    for (int h = 0; h < H; ++h)
+                    {
+                        v2d out;
+                        if constexpr (MONO)
+                        {
+                            out = cmul(cf[h][0], buf[wave][j][s_reg[h]][s_lane[h]]);
+                        }
+                        else
+                        {
+                            out = v2d{0.0, 0.0};
+#pragma unroll
+                            for (int k = 0; k < K; ++k)
+                                out = cfma(cf[h][k], buf[wave][j][k >> TL][lbase | op.dep[k & ((1 << TL) - 1)]], out);
+                        }
+                        stx<NT>(s + (base[j] + op.hoff[h]), keep[h] ? x[j][h] : out);
+                    }
+                }
+                // the next tile overwrites this wave's LDS rows
+                wave_sync();
+            }
+        }
+
         // ---------------- state preparation ----------------
         template <class A>
         __global__ void k_fill_basis(A *s, uint64_t count, uint64_t index)
@@ -508,6 +628,35 @@ namespace qsv
                     v2d a = lda(s + keep);
                     sta(s + keep, v2d{a.x * scale, a.y * scale});
                 }
+            }
+        }
+
+        // Collapse on a qubit whose runs are shorter than a line (2^tq amplitudes of S bytes,
+        // fewer than 128 B): every amplitude is read and written, zeroed or scaled by its bit,
+        // so that all accesses cover whole lines.
+        template <class A>
+        __global__ __launch_bounds__(kBlock) void k_collapse_lines(A *__restrict__ s, uint64_t count, int tq, int result, double scale)
+        {
+            uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+            for (; i < count; i += uint64_t(gridDim.x) * kBlock)
+            {
+                const v2d a = lda(s + i);
+                const bool kept = int((i >> tq) & 1) == result;
+                sta(s + i, kept ? v2d{a.x * scale, a.y * scale} : v2d{0.0, 0.0});
+            }
+        }
+
+        // multiply every amplitude by 1/sqrt(*sum) (the norm computed on the device)
+        template <class A>
+        __global__ __launch_bounds__(kBlock) void k_scale_to_unit(A *__restrict__ s, uint64_t count, const double *__restrict__ sum)
+        {
+            const double t = *sum;
+            const double f = t > 0.0 ? 1.0 / sqrt(t) : 1.0;
+            uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+            for (; i < count; i += uint64_t(gridDim.x) * kBlock)
+            {
+                const v2d a = lda(s + i);
+                sta(s + i, v2d{a.x * f, a.y * f});
             }
         }
 
@@ -907,6 +1056,123 @@ namespace qsv
         }
     } // namespace
 
+    namespace
+    {
+        template <int T, int NH, bool MONO>
+        void low_kernel(hipStream_t s, c128 *state, uint64_t nchunks, const Subspace &csub, const LowOp &op, bool nt)
+        {
+            const dim3 grid(grid_for(nchunks, (kBlock / 64) * kLowPer));
+            if (nt)
+                hipLaunchKernelGGL((k_low<T, NH, MONO, true>), grid, dim3(kBlock), 0, s, state, nchunks, csub, op);
+            else
+                hipLaunchKernelGGL((k_low<T, NH, MONO, false>), grid, dim3(kBlock), 0, s, state, nchunks, csub, op);
+        }
+
+        // The wave-chunk kernel (k_low) for a gate on T targets tq[0..T-1] (local bit j =
+        // target j) whose operands (`sub`: targets and controls) include qubit 0, 1 or 2, on a
+        // state of at least 12 qubits with at most 2 targets at qubit 6 or above.  dense:
+        // 2^T x 2^T row-major (T <= 2), or null with the rows src/coef (T <= 4).  Returns
+        // false when the gate does not have that shape.
+        bool launch_low(hipStream_t s, c128 *state, int n, const Subspace &sub, int T, const int *tq, const c128 *dense, const int *src, const c128 *coef, unsigned read_mask, unsigned keep_mask)
+        {
+            if (n < 12 || sub.nops < 1 || sub.ops[0] >= 3 || T < 1 || T > 4 || (dense && T > 2))
+                return false;
+            int low[4], high[4], tl = 0, nh = 0;
+            for (int j = 0; j < T; ++j)
+                (tq[j] < 6 ? low[tl++] : high[nh++]) = j;
+            if (nh > 2)
+                return false;
+            LowOp op{};
+            int newbit[4];
+            for (int i = 0; i < tl; ++i)
+            {
+                newbit[low[i]] = i;
+                op.tmask |= 1u << tq[low[i]];
+            }
+            for (int i = 0; i < nh; ++i)
+                newbit[high[i]] = tl + i;
+            const int K = 1 << T;
+            auto perm = [&](int k) {
+                int r = 0;
+                for (int j = 0; j < T; ++j)
+                    r |= ((k >> j) & 1) << newbit[j];
+                return r;
+            };
+            for (int kl = 0; kl < (1 << tl); ++kl)
+                for (int i = 0; i < tl; ++i)
+                    if ((kl >> i) & 1)
+                        op.dep[kl] |= 1 << tq[low[i]];
+            for (int h = 0; h < (1 << nh); ++h)
+                for (int i = 0; i < nh; ++i)
+                    if ((h >> i) & 1)
+                        op.hoff[h] |= uint64_t(1) << tq[high[i]];
+            // chunks: every operand at 6 or above is a chunk-index operand; controls below 6
+            // are lane bits
+            Subspace csub{};
+            for (int k = 0; k < sub.nops; ++k)
+            {
+                const int q = sub.ops[k];
+                bool is_target = false;
+                for (int j = 0; j < T; ++j)
+                    is_target |= tq[j] == q;
+                const unsigned set = unsigned((sub.set_bits >> q) & 1);
+                if (q < 6 && !is_target)
+                {
+                    op.ctl_mask |= 1u << q;
+                    op.ctl_set |= set << q;
+                }
+                else if (q >= 6)
+                {
+                    csub.ops[csub.nops++] = q - 6;
+                    if (!is_target)
+                        csub.set_bits |= uint64_t(set) << (q - 6);
+                }
+            }
+            if (n - 6 - csub.nops < 0)
+                return false;
+            for (int k = 0; k < K; ++k)
+            {
+                if ((read_mask >> k) & 1)
+                    op.read_mask |= 1u << perm(k);
+                if ((keep_mask >> k) & 1)
+                    op.keep_mask |= 1u << perm(k);
+                if (dense)
+                {
+                    for (int c = 0; c < K; ++c)
+                        op.m[perm(k) * K + perm(c)] = dense[k * K + c];
+                }
+                else
+                {
+                    op.src[perm(k)] = perm(src[k]);
+                    op.coef[perm(k)] = coef[k];
+                }
+            }
+            const uint64_t nchunks = uint64_t(1) << (n - 6 - csub.nops);
+            const bool nt = use_nt(n);
+            const int key = (dense ? 0 : 100) + 10 * T + nh;
+            switch (key)
+            {
+            case 10: low_kernel<1, 0, false>(s, state, nchunks, csub, op, nt); break;
+            case 11: low_kernel<1, 1, false>(s, state, nchunks, csub, op, nt); break;
+            case 20: low_kernel<2, 0, false>(s, state, nchunks, csub, op, nt); break;
+            case 21: low_kernel<2, 1, false>(s, state, nchunks, csub, op, nt); break;
+            case 22: low_kernel<2, 2, false>(s, state, nchunks, csub, op, nt); break;
+            case 120: low_kernel<2, 0, true>(s, state, nchunks, csub, op, nt); break;
+            case 121: low_kernel<2, 1, true>(s, state, nchunks, csub, op, nt); break;
+            case 122: low_kernel<2, 2, true>(s, state, nchunks, csub, op, nt); break;
+            case 130: low_kernel<3, 0, true>(s, state, nchunks, csub, op, nt); break;
+            case 131: low_kernel<3, 1, true>(s, state, nchunks, csub, op, nt); break;
+            case 132: low_kernel<3, 2, true>(s, state, nchunks, csub, op, nt); break;
+            case 140: low_kernel<4, 0, true>(s, state, nchunks, csub, op, nt); break;
+            case 141: low_kernel<4, 1, true>(s, state, nchunks, csub, op, nt); break;
+            case 142: low_kernel<4, 2, true>(s, state, nchunks, csub, op, nt); break;
+            default: return false;
+            }
+            after_launch("gate kernel (operands below qubit 3)");
+            return true;
+        }
+    } // namespace
+
     void launch_pair(hipStream_t s, c128 *state, int n, const Subspace &sub, const PairOp &op)
     {
         uint64_t items = items_of(n, sub, "pair");
@@ -938,6 +1204,40 @@ namespace qsv
                     hipLaunchKernelGGL((k_pair_low<2, false>), dim3(grid_for(nchunks, (kBlock / 64) * 2)), dim3(kBlock), 0, s, state, nchunks, csub, t, op);
                 after_launch("pair kernel (low target)");
                 return;
+            }
+        }
+        {
+            // an operand below qubit 3 (a control, or a target with a control below 6): the
+            // 2x2 on the patterns offA, offB as a gate on the targets, the bits of offA | offB
+            const uint64_t tbits = op.offA | op.offB;
+            const int T = __builtin_popcountll(tbits);
+            if (T >= 1 && T <= 2)
+            {
+                int tq[2], nt_ = 0;
+                for (uint64_t b = tbits; b; b &= b - 1)
+                    tq[nt_++] = __builtin_ctzll(b);
+                auto local = [&](uint64_t off) {
+                    int k = 0;
+                    for (int j = 0; j < T; ++j)
+                        k |= int((off >> tq[j]) & 1) << j;
+                    return k;
+                };
+                const int pa = local(op.offA), pb = local(op.offB), K = 1 << T;
+                c128 m[16] = {};
+                for (int k = 0; k < K; ++k)
+                    m[k * K + k] = c128{1.0, 0.0};
+                m[pa * K + pa] = op.m[0];
+                m[pa * K + pb] = op.m[1];
+                m[pb * K + pa] = op.m[2];
+                m[pb * K + pb] = op.m[3];
+                unsigned rmask = (1u << K) - 1, keep = (1u << K) - 1;
+                keep &= ~((1u << pa) | (1u << pb));
+                if (op.zero_mask & 1)
+                    rmask &= ~(1u << pa);
+                if (op.zero_mask & 2)
+                    rmask &= ~(1u << pb);
+                if (pa != pb && launch_low(s, state, n, sub, T, tq, m, nullptr, nullptr, rmask, keep))
+                    return;
             }
         }
         const bool nt = use_nt(n);
@@ -1014,6 +1314,11 @@ namespace qsv
         uint64_t items = items_of(n, sub, "dense4");
         for (int k = 0; k < 4; ++k)
             check_offset(n, op.off[k], "dense4");
+        {
+            const int tq[2] = {__builtin_ctzll(op.off[1] | (uint64_t(1) << 63)), __builtin_ctzll(op.off[2] | (uint64_t(1) << 63))};
+            if (op.off[1] && op.off[2] && launch_low(s, state, n, sub, 2, tq, op.m, nullptr, nullptr, op.read_mask, ~op.write_mask & 15u))
+                return;
+        }
         hipLaunchKernelGGL(k_dense4, dim3(grid_for(items, kBlock)), dim3(kBlock), 0, s, state, items, sub, op);
         after_launch("dense4 kernel");
     }
@@ -1028,6 +1333,32 @@ namespace qsv
         for (int k = 0; k < op.nrows; ++k)
             if (op.row[k] < 0 || op.row[k] >= (1 << op.ntargets) || op.src[k] < 0 || op.src[k] >= (1 << op.ntargets))
                 throw std::runtime_error("mono: row index out of range");
+        {
+            // rows not listed are left unchanged
+            const int K = 1 << op.ntargets;
+            int tq[4], src[16];
+            c128 coef[16];
+            unsigned keep = (1u << K) - 1;
+            bool ok = true;
+            for (int j = 0; j < op.ntargets; ++j)
+            {
+                ok &= op.off[1 << j] != 0;
+                tq[j] = __builtin_ctzll(op.off[1 << j] | (uint64_t(1) << 63));
+            }
+            for (int k = 0; k < K; ++k)
+            {
+                src[k] = k;
+                coef[k] = c128{1.0, 0.0};
+            }
+            for (int k = 0; k < op.nrows; ++k)
+            {
+                src[op.row[k]] = op.src[k];
+                coef[op.row[k]] = op.coef[k];
+                keep &= ~(1u << op.row[k]);
+            }
+            if (ok && op.ntargets >= 2 && launch_low(s, state, n, sub, op.ntargets, tq, nullptr, src, coef, op.read_mask, keep))
+                return;
+        }
         hipLaunchKernelGGL(k_mono, dim3(grid_for(items, kBlock)), dim3(kBlock), 0, s, state, items, sub, op);
         after_launch("mono kernel");
     }
@@ -1142,6 +1473,27 @@ namespace qsv
         check(hipStreamSynchronize(s), "reduce sync");
     }
 
+    template <class A>
+    void renormalize_async(hipStream_t s, A *state, uint64_t count, double *work)
+    {
+        double *dev_out = work;
+        const uint64_t g = (count + kTile - 1) / kTile;
+        double *lvl = work + 2;
+        if (g == 1)
+        {
+            hipLaunchKernelGGL(k_reduce_probs<A>, dim3(1), dim3(kBlock), 0, s, state, count, uint64_t(0), dev_out);
+            after_launch("reduce kernel");
+        }
+        else
+        {
+            hipLaunchKernelGGL(k_reduce_probs<A>, dim3(g), dim3(kBlock), 0, s, state, count, uint64_t(0), lvl);
+            after_launch("reduce kernel");
+            finish_tiles<1>(s, lvl, g, g, lvl + g, dev_out);
+        }
+        hipLaunchKernelGGL(k_scale_to_unit<A>, dim3(grid_for(count, kBlock * 4)), dim3(kBlock), 0, s, state, count, static_cast<const double *>(dev_out));
+        after_launch("scale kernel");
+    }
+
     template <class A, class B>
     void overlap(hipStream_t s, const A *a, const B *b, uint64_t count, double *work, double out[3])
     {
@@ -1175,6 +1527,14 @@ namespace qsv
     {
         if (tq < 0 || tq >= n || n < 1)
             throw std::runtime_error("collapse: qubit out of range");
+        if ((uint64_t(sizeof(A)) << tq) < 128 && n >= 8)
+        {
+            // runs of 2^tq amplitudes shorter than a line: whole lines read and written
+            const uint64_t count = uint64_t(1) << n;
+            hipLaunchKernelGGL(k_collapse_lines<A>, dim3(grid_for(count, kBlock * 4)), dim3(kBlock), 0, s, state, count, tq, result, scale);
+            after_launch("collapse kernel (whole lines)");
+            return;
+        }
         uint64_t half = uint64_t(1) << (n - 1);
         hipLaunchKernelGGL(k_collapse<A>, dim3(grid_for(half, kBlock)), dim3(kBlock), 0, s, state, half, tq, result, scale, scale != 1.0 ? 1 : 0);
         after_launch("collapse kernel");
@@ -1489,6 +1849,7 @@ namespace qsv
     template void reduce_pair_probs<A>(hipStream_t, const A *, const Subspace &, uint64_t, uint64_t, double *, double *);                               \
     template void reduce_probs<A>(hipStream_t, const A *, uint64_t, uint64_t, double *, double *);                                                      \
     template void launch_collapse<A>(hipStream_t, A *, int, int, int, double);                                                                          \
+    template void renormalize_async<A>(hipStream_t, A *, uint64_t, double *);                                                                           \
     template void build_fold_levels<A>(hipStream_t, const A *, uint64_t, const std::vector<double *> &);                                                \
     template void launch_sample_search<A>(hipStream_t, const A *, uint64_t, const std::vector<double *> &, double, uint64_t, uint64_t, const int *, int, uint64_t *); \
     template double sample_noalloc<A>(hipStream_t, int, const A *, int, uint64_t, uint64_t, const int *, double *, uint64_t *); \

@@ -110,6 +110,12 @@ namespace qsv
     template <class A, class B>
     void overlap(hipStream_t s, const A *a, const B *b, uint64_t count, double *work, double out[3]);
 
+    // Scales the state to unit norm without a host round trip: the canonical FP64 sum of
+    // p(i) lands in work[0] on the device, then every amplitude is multiplied by
+    // 1/sqrt(work[0]) in FP64 (left as is when the sum is zero).
+    template <class A>
+    void renormalize_async(hipStream_t s, A *state, uint64_t count, double *work);
+
     // ---- measurement collapse ----
     template <class A> void launch_collapse(hipStream_t s, A *state, int n, int tq, int result, double scale);
 

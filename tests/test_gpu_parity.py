@@ -117,6 +117,78 @@ def test_low_operand_phase_kernel(gpu, oracle):
     assert fidelity(g, o) >= 1 - FID_TOL
 
 
+def test_low_operand_gate_kernel(gpu, oracle):
+    # every gate with an operand on qubit 0, 1 or 2: 2-4-target gates (swap-like, dense
+    # 4x4, one nonzero per row) and 1-target gates with a control there run in the
+    # 64-amplitude chunk kernel, with up to two targets at qubit 6 or above as registers;
+    # controls low and high, positive and negative; a measured target zeroes a branch
+    n = 14
+    rng = np.random.default_rng(79)
+    psi = random_state(n, 11)
+    circ = []
+    for i, name in enumerate(ALL * 3):
+        t = pyqcs.GATE_TARGETS[name]
+        if t == 0:
+            continue
+        low = [int(q) for q in rng.permutation(3)]
+        mid = [int(q) for q in rng.permutation(np.arange(3, 6))]
+        high = [int(q) for q in rng.permutation(np.arange(6, n))]
+        nhigh = min(t - 1, int(rng.integers(0, 3)))
+        targets = low[:1] + (mid + low[1:])[: t - 1 - nhigh] + high[:nhigh]
+        targets = [int(q) for q in rng.permutation(targets)]
+        rest = [q for q in low + mid + high if q not in targets]
+        ctl = [int(q) for q in rng.choice(rest, size=int(rng.integers(0, 3)), replace=False)]
+        if t == 1 and i % 2:  # target high, control low
+            targets = [high[-1]]
+            ctl = [low[0]]
+        pos = [c for c in ctl if rng.random() < 0.5]
+        neg = [c for c in ctl if c not in pos]
+        k = pyqcs.GATE_PARAMS[name]
+        params = tuple(rng.uniform(-math.pi, math.pi, size=k))
+        circ.append((name, tuple(targets), params, tuple(pos), tuple(neg)))
+    say(f"low-operand circuit n={n} {len(circ)} gates")
+    res = []
+    for lib in (gpu, oracle):
+        with pyqcs.Simulator(lib, n, num_clbits=2, seed=6) as sim:
+            sim.set_statevector(psi)
+            sim.run(circ[: len(circ) // 2])
+            bits = [sim.measure_to_clbit(1, 0), sim.measure_to_clbit(7, 1)]
+            sim.run(circ[len(circ) // 2:])
+            res.append((sim.statevector(), bits))
+    assert res[0][1] == res[1][1]
+    assert np.max(np.abs(res[0][0] - res[1][0])) <= AMP_TOL
+    assert fidelity(res[0][0], res[1][0]) >= 1 - FID_TOL
+    # the textbook model gate by gate on the same circuit without the measurements
+    g, o = run_both(gpu, oracle, n, circ, psi=psi)
+    want = psi
+    for name, targets, params, pos, neg in circ:
+        want = textbook.apply(want, n, name, list(targets), params, list(pos), list(neg))
+    assert np.max(np.abs(g - want)) <= AMP_TOL
+    assert np.max(np.abs(g - o)) <= AMP_TOL
+
+
+@pytest.mark.parametrize("renorm", [False, True])
+def test_collapse_states(gpu, oracle, renorm):
+    # post-measurement states (not only their norm) against the oracle: qubits 0..2 take
+    # the whole-line collapse kernel, the others the pair-wise one; plain and renormalised
+    n = 13
+    psi = random_state(n, 12)
+    res = []
+    for lib in (gpu, oracle):
+        with pyqcs.Simulator(lib, n, seed=21, mapping=[3, 0, 12, 1, 2, 4, 5, 6, 7, 8, 9, 10, 11]) as sim:
+            sim.set_renormalize(renorm)
+            sim.set_statevector(psi)
+            bits = [sim.measure(q) for q in (1, 0, 9, 2, 12)]
+            res.append((sim.statevector(), bits))
+    assert res[0][1] == res[1][1]
+    assert np.max(np.abs(res[0][0] - res[1][0])) <= AMP_TOL
+    norm = float(np.vdot(res[0][0], res[0][0]).real)
+    if renorm:
+        assert abs(norm - 1) <= 1e-12
+    else:
+        assert norm < 1 - 1e-3  # five measurements of a random state discard weight
+
+
 def test_random_circuit_all_gates(gpu, oracle):
     n = 12
     circ = circuits.random_all_gates(n, 400, seed=17, max_ctrls=3)
@@ -239,16 +311,21 @@ def test_save_statevector(gpu, oracle, tmp_path):
     assert np.max(np.abs(a - b)) <= AMP_TOL
 
 
-def test_state_preps(gpu, oracle):
+@pytest.mark.parametrize("mapping", [None, [5, 2, 7, 0, 3, 6, 1, 4]])
+def test_state_preps(gpu, oracle, mapping):
+    # physical-index fills (src/qcs.cu:71-82, 2025-2095): under a mapping the logical
+    # vector differs from the identity mapping's, so both sides must write physical indices
     n = 8
     for what in ("set_sequential_state", "set_flat_state", "set_entangled_state", "set_zero_state"):
         outs = []
         for lib in (gpu, oracle):
-            with pyqcs.Simulator(lib, n) as sim:
+            with pyqcs.Simulator(lib, n, mapping=mapping) as sim:
                 sim.gate("h", [3])
                 sim.call(what)
                 outs.append(sim.statevector())
         assert np.array_equal(outs[0], outs[1]), what
+        if what == "set_sequential_state" and mapping is not None:
+            assert not np.array_equal(outs[0], np.arange(1 << n, dtype=np.complex128))
 
 
 def test_events_and_errors(gpu):

@@ -350,6 +350,53 @@ def test_renormalize(oracle):
         assert abs(np.linalg.norm(sim.statevector()) - 1) < 1e-14
 
 
+@pytest.mark.parametrize("mapping", [None, [5, 2, 7, 0, 3, 6, 1, 4]])
+def test_state_preps_physical_index(oracle, mapping):
+    # the reference fills the device array by physical index (src/qcs.cu:71-82, 2025-2095):
+    # sequential puts value P at physical index P, so logical index L reads phys(L) =
+    # sum_q bit(L, q) << l2p[q]; flat is all ones; entangled (one rank) sets physical index
+    # 0 to 1 and clears nothing
+    n = 8
+    l2p = list(range(n)) if mapping is None else [mapping.index(q) for q in range(n)]
+    L = np.arange(1 << n)
+    phys = np.zeros(1 << n, dtype=np.int64)
+    for q in range(n):
+        phys |= ((L >> q) & 1) << l2p[q]
+    with pyqcs.Simulator(oracle, n, mapping=mapping) as sim:
+        sim.call("set_sequential_state")
+        assert np.array_equal(sim.statevector(), phys.astype(np.complex128))
+        sim.call("set_entangled_state")
+        want = phys.astype(np.complex128)
+        want[0] = 1.0
+        assert np.array_equal(sim.statevector(), want)
+        sim.call("set_flat_state")
+        assert np.array_equal(sim.statevector(), np.ones(1 << n, dtype=np.complex128))
+        sim.call("set_zero_state")
+        sim.gate("h", [3])
+        sim.call("set_entangled_state")
+        want = np.zeros(1 << n, dtype=np.complex128)
+        want[0] = 1.0
+        want[1 << 3] = 1 / math.sqrt(2)
+        assert np.max(np.abs(sim.statevector() - want)) <= 2e-16
+
+
+def test_renormalised_collapse_closed_form(oracle):
+    # after measure(q) = b with renormalisation: psi restricted to bit q == b over the
+    # square root of that branch's probability (the p0/p1 of the measurement)
+    n = 7
+    psi = random_state(n, 31)
+    with pyqcs.Simulator(oracle, n, seed=4, mapping=[6, 0, 5, 1, 4, 2, 3]) as sim:
+        sim.set_renormalize(True)
+        sim.set_statevector(psi)
+        want = psi.copy()
+        for q in (2, 0, 6):
+            b = sim.measure(q)
+            keep = ((np.arange(1 << n) >> q) & 1) == b
+            want = np.where(keep, want, 0)
+            want = want / math.sqrt(float(np.sum(np.abs(want) ** 2)))
+            assert np.max(np.abs(sim.statevector() - want)) <= 1e-15
+
+
 def test_local_window_swap_in(oracle):
     # src/qcs.cu:2216-2226: a target on global physical qubit a_i is swapped with local
     # slot n_local-1-i; maps updated as at src/qcs.cu:2306-2328
