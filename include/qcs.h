@@ -1,6 +1,9 @@
 /*
  * qcs.h -- C ABI of the RQS-SVG state-vector simulator, MI355X-native build.
  *
+ * The typedefs and declarations below are those of RQS-SVG's include/qcs.h (Apache License
+ * 2.0, attribution in the NOTICE file at the repository root).
+ *
  * This header is the drop-in contract: every typedef, every C++ `*_cxx` entry point and
  * every `extern "C"` entry point below has exactly the signature of the reference header
  * (include/qcs.h:10-15 for the typedefs, include/qcs.h:18-80 for the C++ functions,

@@ -127,6 +127,18 @@ int qcs_ext_set_precision(qcs_simulator *sim, int mode, int max_block_qubits);
  * 0).  Any time; queued gates are flushed first. */
 int qcs_ext_set_stage_bits(qcs_simulator *sim, int tile_bits);
 
+/* Fused precision modes: a fixed plan for every flush in place of the default one (stages
+ * on the configured tile size with the default block size and tile choice) when no
+ * planner model is attached.  tile_bits > 0: stages on tiles of 2^tile_bits amplitudes
+ * (12 for complex128, 11..14 for complex64, state of at least tile_bits + 4 qubits) with
+ * blocks of block_qubits qubits (complex128 3 or 4, complex64 4 or 5, at most the block
+ * limit) and tiles 0 (grown from qubits 0..3 in gate order), 1 (the best window of
+ * consecutive qubits) or 2 (whichever gives fewer stages, the default).  tile_bits 0: one
+ * sweep per block, blocks of at most block_qubits qubits chosen among the next `window`
+ * blocks by rule 0 (least growth), 1 (fullest block) or 2 (earliest block that fits).
+ * enable 0 restores the default.  Queued gates are flushed first. */
+int qcs_ext_set_plan(qcs_simulator *sim, int enable, int tile_bits, int block_qubits, int tiles, int window, int rule);
+
 /* Apply all queued gates now (no-op in mode 0). */
 int qcs_ext_flush(qcs_simulator *sim);
 
@@ -208,6 +220,15 @@ int qcs_ext_set_planner(qcs_simulator *sim, qcs_ext_mlp *model);
 /* out[0..4]: flushes planned, flushes that took the learned choice, predicted greedy
  * microseconds, predicted microseconds of the executed plans, blocks applied. */
 int qcs_ext_planner_stats(qcs_simulator *sim, double *out);
+/* Times the launches of every stage plan the planner considers for the gates queued now
+ * (tile sizes x block sizes x grown / window tiles), applying the gates once per plan, so
+ * the state afterwards is not the circuit's result: training rows drawn from the plans the
+ * planner is asked to price.  Writes at most `capacity` rows (features as in
+ * qcs_ext_planner_record, log2 microseconds) and their number to *count. */
+int qcs_ext_planner_record_plans(qcs_simulator *sim, int capacity, float *features, float *log2_us, int *count);
+/* The plan the learned planner ran at the last flush: plan[0..4] = stage tile bits (0: one
+ * launch per block), block qubits, tile choice, window, rule (as in qcs_ext_set_plan). */
+int qcs_ext_planner_last_choice(qcs_simulator *sim, int *plan);
 
 /* Per-kernel-class timing with HIP events around every launch (oracle: host clock). */
 int qcs_ext_profile(qcs_simulator *sim, int enable);

@@ -1410,6 +1410,13 @@ extern "C"
                 throw std::runtime_error("stage tile bits must be 0, 11, 12, 13 or 14");
         });
     }
+    int qcs_ext_set_plan(qcs_simulator *s, int, int, int, int, int, int)
+    { // the oracle has no fused path (set_precision refuses every fused mode)
+        return guard([&] {
+            (void)core(s);
+            throw std::runtime_error("the oracle implements only the complex128 plain method");
+        });
+    }
     int qcs_ext_set_precision(qcs_simulator *s, int mode, int k)
     {
         return guard([&] {
@@ -1682,6 +1689,17 @@ extern "C"
         return guard([&] { throw std::runtime_error("the oracle has no kernels to time"); });
     }
     int qcs_ext_set_planner(qcs_simulator *s, qcs_ext_mlp *) { return guard([&] { (void)core(s); }); }
+    int qcs_ext_planner_record_plans(qcs_simulator *, int, float *, float *, int *)
+    {
+        return guard([&] { throw std::runtime_error("the oracle has no kernels to time"); });
+    }
+    int qcs_ext_planner_last_choice(qcs_simulator *s, int *plan)
+    {
+        return guard([&] {
+            (void)core(s);
+            throw std::runtime_error("the oracle has no fusion planner");
+        });
+    }
     int qcs_ext_planner_stats(qcs_simulator *s, double *out)
     {
         return guard([&] {

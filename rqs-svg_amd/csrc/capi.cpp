@@ -424,6 +424,10 @@ extern "C"
     int qcs_ext_set_stage_bits(qcs_simulator *s, int m) { return guard([&] { core(s).set_stage_bits(m); }); }
     int qcs_ext_set_precision(qcs_simulator *s, int mode, int k) { return guard([&] { core(s).set_precision(mode, k); }); }
     int qcs_ext_flush(qcs_simulator *s) { return guard([&] { core(s).flush(); }); }
+    int qcs_ext_set_plan(qcs_simulator *s, int en, int bits, int k, int tiles, int window, int rule)
+    {
+        return guard([&] { core(s).set_plan(en != 0, bits, k, tiles, window, rule); });
+    }
     int qcs_ext_fusion_stats(qcs_simulator *s, uint64_t *gates, uint64_t *blocks)
     {
         return guard([&] {
@@ -527,6 +531,18 @@ extern "C"
             if (m && m->core && m->core->num_features() != qcore::kPlanFeatures)
                 throw std::runtime_error("the planner model must take 16 features");
             core(s).set_planner_model(m ? m->core : nullptr);
+        });
+    }
+    int qcs_ext_planner_record_plans(qcs_simulator *s, int capacity, float *features, float *log2_us, int *count)
+    {
+        return guard([&] { put(count, core(s).planner_record_plans(capacity, features, log2_us)); });
+    }
+    int qcs_ext_planner_last_choice(qcs_simulator *s, int *plan)
+    {
+        return guard([&] {
+            if (!plan)
+                throw std::runtime_error("result pointer must not be NULL");
+            core(s).planner_last_choice(plan);
         });
     }
     int qcs_ext_planner_stats(qcs_simulator *s, double *out)

@@ -5,6 +5,10 @@
 #include "fused.hpp"
 #include "planner.hpp"
 
+#include <exception>
+#include <mutex>
+#include <thread>
+
 #include <qcs_ext.h>
 
 #include <algorithm>
@@ -249,6 +253,8 @@ namespace qcore
             (void)hipHostFree(rhost_);
         if (copy_done_)
             (void)hipEventDestroy(copy_done_);
+        for (hipEvent_t e : ev_pool_)
+            (void)hipEventDestroy(e);
     }
 
     void FusedEngine::reserve_matrices(size_t bytes)
@@ -307,7 +313,92 @@ namespace qcore
             return;
         std::vector<ResolvedGate> gates;
         gates.swap(queue_);
+        apply_plan(gates);
+        renormalize();
+    }
+
+    void FusedEngine::renormalize()
+    {
+        // complex64 block products are unitary only to their arithmetic (BF16x3 holds each
+        // matrix as two bfloat16 parts, about 2^-17 per entry), which adds up to a norm drift
+        // of up to ~1e-5 over a few hundred blocks: each flush ends with an FP64 norm and
+        // rescale, queued on the stream (no host round trip)
+        if (!f64_)
+            qsv::renormalize_async(sim_.stream_, state_, uint64_t(1) << sim_.n_, sim_.work_);
+    }
+
+    std::vector<PlanOption> FusedEngine::plan_options() const
+    {
+        // candidate 0: the plan the engine runs without a model (stages when enabled, else
+        // greedy blocks)
+        const int n = sim_.n_;
         const int m0 = tile_bits();
+        std::vector<int> tiles;
+        for (int m : f64_ ? std::vector<int>{12} : std::vector<int>{11, 12, 13, 14})
+            if (n >= m + 4)
+                tiles.push_back(m);
+        std::vector<PlanOption> opts;
+        opts.push_back(PlanOption{m0, m0 > 0 ? stage_k() : k_, 64, 0, kTilesAuto});
+        for (const auto &o : candidate_options(kq_, tiles, stage_block_sizes()))
+            if (o.k <= k_ || o.stage_bits > 0)
+                opts.push_back(o);
+        return opts;
+    }
+
+    void FusedEngine::option_features(const std::vector<ResolvedGate> &gates, const PlanOption &o, std::vector<float> &X) const
+    {
+        // one row per launch, in the order apply_option issues them
+        const int n = sim_.n_;
+        const float math = math_feature();
+        auto add_block = [&](const std::vector<ResolvedGate> &gs, const PlannedBlock &b) {
+            const BlockLayout lay = layout_block(gs, b, kq_, n);
+            float f[kPlanFeatures];
+            block_features(n, math, kq_, lay, (int)b.gates.size(), block_uses_lds(lay, n, f64_), f);
+            X.insert(X.end(), f, f + kPlanFeatures);
+        };
+        if (o.stage_bits == 0)
+        {
+            for (const auto &b : plan_with(gates, o))
+                add_block(gates, b);
+            return;
+        }
+        for (const auto &st : plan_stages(gates, n, o.stage_bits, o.k, o.tiles))
+        {
+            std::vector<ResolvedGate> sub;
+            for (int gi : st.gates)
+                sub.push_back(gates[gi]);
+            if (st.tile.empty())
+            {
+                for (const auto &b : plan_greedy(sub, k_))
+                    add_block(sub, b);
+                continue;
+            }
+            float f[kPlanFeatures];
+            const int nb = f64_ ? stage128_lds_blocks(sub, o.k) : stage64_lds_blocks(sub, o.k);
+            stage_features(n, math, o.k, st.tile, nb, (int)st.gates.size(), f);
+            X.insert(X.end(), f, f + kPlanFeatures);
+        }
+    }
+
+    void FusedEngine::apply_option(const std::vector<ResolvedGate> &gates, const PlanOption &o)
+    {
+        if (o.stage_bits > 0)
+            apply_stages(gates, o.stage_bits, o.k, o.tiles);
+        else
+            apply_blocks(gates, plan_with(gates, o));
+    }
+
+    void FusedEngine::apply_plan(const std::vector<ResolvedGate> &gates)
+    {
+        const int m0 = tile_bits();
+        if (!model_ && forced_)
+        {
+            if (forced_bits_ > 0)
+                apply_stages(gates, forced_bits_, forced_k_, forced_tiles_);
+            else
+                apply_blocks(gates, plan_rule(gates, forced_k_, forced_window_, forced_rule_));
+            return;
+        }
         if (!model_)
         {
             if (m0 > 0)
@@ -318,53 +409,39 @@ namespace qcore
         }
         // Learned planner: predicted cost of each candidate plan = sum over its launches of
         // 2^prediction (the model predicts log2 microseconds).  Candidate 0 is the plan the
-        // engine runs without a model (stages when enabled, else greedy blocks), and it
-        // judges the others: it is kept unless a candidate is predicted 3 % faster.
-        const int n = sim_.n_;
-        const float math = math_feature();
-        std::vector<int> tiles;
-        for (int m : f64_ ? std::vector<int>{12} : std::vector<int>{11, 12, 13, 14})
-            if (n >= m + 4)
-                tiles.push_back(m);
-        std::vector<PlanOption> opts;
-        opts.push_back(PlanOption{m0, m0 > 0 ? stage_k() : k_, 64, 0, kTilesAuto});
-        for (const auto &o : candidate_options(kq_, tiles, stage_block_sizes()))
-            if (o.k <= k_ || o.stage_bits > 0)
-                opts.push_back(o);
+        // engine runs without a model, and it judges the others: it is kept unless a
+        // candidate is predicted 3 % faster.  The candidates are planned on host threads.
+        const auto opts = plan_options();
+        std::vector<std::vector<float>> rows(opts.size());
+        {
+            const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> pool;
+            std::exception_ptr err;
+            std::mutex err_mu;
+            for (unsigned t = 0; t < hw; ++t)
+                pool.emplace_back([&, t] {
+                    try
+                    {
+                        for (size_t oi = t; oi < opts.size(); oi += hw)
+                            option_features(gates, opts[oi], rows[oi]);
+                    }
+                    catch (...)
+                    {
+                        std::lock_guard<std::mutex> lock(err_mu);
+                        err = std::current_exception();
+                    }
+                });
+            for (auto &th : pool)
+                th.join();
+            if (err)
+                std::rethrow_exception(err);
+        }
         std::vector<float> X;
         std::vector<size_t> first(opts.size() + 1, 0);
-        auto add_block = [&](const std::vector<ResolvedGate> &gs, const PlannedBlock &b) {
-            const BlockLayout lay = layout_block(gs, b, kq_, n);
-            float f[kPlanFeatures];
-            block_features(n, math, kq_, lay, (int)b.gates.size(), block_uses_lds(lay, n, f64_), f);
-            X.insert(X.end(), f, f + kPlanFeatures);
-        };
         for (size_t oi = 0; oi < opts.size(); ++oi)
         {
             first[oi] = X.size() / kPlanFeatures;
-            const PlanOption &o = opts[oi];
-            if (o.stage_bits == 0)
-            {
-                for (const auto &b : plan_with(gates, o))
-                    add_block(gates, b);
-                continue;
-            }
-            for (const auto &st : plan_stages(gates, n, o.stage_bits, o.k, o.tiles))
-            {
-                std::vector<ResolvedGate> sub;
-                for (int gi : st.gates)
-                    sub.push_back(gates[gi]);
-                if (st.tile.empty())
-                {
-                    for (const auto &b : plan_greedy(sub, k_))
-                        add_block(sub, b);
-                    continue;
-                }
-                float f[kPlanFeatures];
-                const int nb = f64_ ? stage128_lds_blocks(sub, o.k) : stage64_lds_blocks(sub, o.k);
-                stage_features(n, math, o.k, st.tile, nb, (int)st.gates.size(), f);
-                X.insert(X.end(), f, f + kPlanFeatures);
-            }
+            X.insert(X.end(), rows[oi].begin(), rows[oi].end());
         }
         first[opts.size()] = X.size() / kPlanFeatures;
         std::vector<float> Y(first.back());
@@ -389,11 +466,110 @@ namespace qcore
             best = 0;
             stat_pred_chosen_ += cost[0];
         }
-        const PlanOption &o = opts[best];
-        if (o.stage_bits > 0)
-            apply_stages(gates, o.stage_bits, o.k, o.tiles);
-        else
-            apply_blocks(gates, plan_with(gates, o));
+        last_choice_ = opts[best];
+        apply_option(gates, opts[best]);
+    }
+
+    void FusedEngine::launch_mark_begin()
+    {
+        if (!launch_log_)
+            return;
+        if (ev_used_ + 2 > ev_pool_.size())
+            for (int i = 0; i < 2; ++i)
+            {
+                hipEvent_t e;
+                qsv::check(hipEventCreate(&e), "hipEventCreate");
+                ev_pool_.push_back(e);
+            }
+        hipEvent_t a = ev_pool_[ev_used_++];
+        qsv::check(hipEventRecord(a, sim_.stream_), "hipEventRecord");
+        launch_log_->push_back({a, nullptr});
+    }
+
+    void FusedEngine::launch_mark_end()
+    {
+        if (!launch_log_)
+            return;
+        hipEvent_t b = ev_pool_[ev_used_++];
+        qsv::check(hipEventRecord(b, sim_.stream_), "hipEventRecord");
+        launch_log_->back().second = b;
+    }
+
+    int FusedEngine::record_plans(int capacity, float *features, float *log2_us)
+    {
+        if (capacity < 0 || (capacity > 0 && (!features || !log2_us)))
+            throw std::runtime_error("planner record: bad output buffers");
+        if (queue_.empty())
+            return 0;
+        std::vector<ResolvedGate> gates;
+        gates.swap(queue_);
+        int count = 0;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> log;
+        for (const auto &o : plan_options())
+        {
+            // stage plans only (block plans are sampled by record()); the default's "fewer
+            // stages" choice repeats the grown or the window plan
+            if (o.stage_bits == 0 || o.tiles == kTilesAuto || count >= capacity)
+                continue;
+            std::vector<float> X;
+            option_features(gates, o, X);
+            const size_t nrows = X.size() / kPlanFeatures;
+            log.clear();
+            ev_used_ = 0;
+            launch_log_ = &log;
+            try
+            {
+                apply_option(gates, o);
+            }
+            catch (...)
+            {
+                launch_log_ = nullptr;
+                throw;
+            }
+            launch_log_ = nullptr;
+            qsv::check(hipStreamSynchronize(sim_.stream_), "sync");
+            if (log.size() != nrows)
+                throw std::runtime_error("planner record: launches and feature rows differ");
+            for (size_t r = 0; r < nrows && count < capacity; ++r)
+            {
+                float ms = 0.f;
+                qsv::check(hipEventElapsedTime(&ms, log[r].first, log[r].second), "hipEventElapsedTime");
+                std::memcpy(features + size_t(count) * kPlanFeatures, &X[r * kPlanFeatures], kPlanFeatures * sizeof(float));
+                log2_us[count] = float(std::log2(std::max(1e3 * double(ms), 0.01)));
+                ++count;
+            }
+        }
+        renormalize();
+        return count;
+    }
+
+    void FusedEngine::set_plan(bool enable, int bits, int k, int tiles, int window, int rule)
+    {
+        flush();
+        if (!enable)
+        {
+            forced_ = false;
+            return;
+        }
+        if (bits != 0)
+        {
+            const bool ok_bits = f64_ ? bits == 12 : (bits >= 11 && bits <= 14);
+            if (!ok_bits || sim_.n_ < bits + 4)
+                throw std::runtime_error("plan: stage tile bits must be 12 (complex128) or 11..14 (complex64), with at least tile_bits + 4 qubits");
+            const auto ks = stage_block_sizes();
+            if (std::find(ks.begin(), ks.end(), k) == ks.end())
+                throw std::runtime_error("plan: this arithmetic's stage kernels take other block sizes");
+            if (tiles != kTilesGrow && tiles != kTilesWindow && tiles != kTilesAuto)
+                throw std::runtime_error("plan: tile choice must be 0 (grow), 1 (window) or 2 (fewer stages)");
+        }
+        else if (k < 1 || k > k_ || window < 1 || rule < 0 || rule > 2)
+            throw std::runtime_error("plan: block plans take 1 <= k <= the block limit, window >= 1, rule 0..2");
+        forced_ = true;
+        forced_bits_ = bits;
+        forced_k_ = k;
+        forced_tiles_ = tiles;
+        forced_window_ = window;
+        forced_rule_ = rule;
     }
 
     void FusedEngine::planner_stats(double out[5]) const
@@ -1328,10 +1504,12 @@ namespace qcore
             if (it.stage)
             {
                 sim_.prof_begin(QCS_EXT_K_FUSED, f64_ ? it.st128.bytes : it.st.bytes);
+                launch_mark_begin();
                 if (f64_)
                     launch_prepared_stage128(sim_.stream_, state128_, n, it.st128, static_cast<const double *>(rdev_));
                 else
                     launch_prepared_stage(sim_.stream_, it.st, static_cast<const float *>(rdev_));
+                launch_mark_end();
                 sim_.prof_end();
                 blocks_applied_ += 1;
                 gates_applied_ += f64_ ? it.st128.gates : it.st.gates;
@@ -1340,7 +1518,9 @@ namespace qcore
             for (auto &b : it.blocks128)
             {
                 sim_.prof_begin(QCS_EXT_K_FUSED, b.bytes);
+                launch_mark_begin();
                 launch_prepared128(sim_.stream_, state128_, n, b, static_cast<const double *>(rdev_));
+                launch_mark_end();
                 sim_.prof_end();
                 blocks_applied_ += 1;
                 gates_applied_ += b.gates;
@@ -1348,7 +1528,9 @@ namespace qcore
             for (auto &b : it.blocks)
             {
                 sim_.prof_begin(QCS_EXT_K_FUSED, b.bytes);
+                launch_mark_begin();
                 launch_prepared(sim_.stream_, state_, n, b, static_cast<const float *>(rdev_));
+                launch_mark_end();
                 sim_.prof_end();
                 blocks_applied_ += 1;
                 gates_applied_ += b.gates;
@@ -1374,7 +1556,9 @@ namespace qcore
             for (auto &b : blocks)
             {
                 sim_.prof_begin(QCS_EXT_K_FUSED, b.bytes);
+                launch_mark_begin();
                 launch_prepared128(sim_.stream_, state128_, n, b, static_cast<const double *>(rdev_));
+                launch_mark_end();
                 sim_.prof_end();
                 blocks_applied_ += 1;
                 gates_applied_ += b.gates;
@@ -1392,7 +1576,9 @@ namespace qcore
         for (auto &b : blocks)
         {
             sim_.prof_begin(QCS_EXT_K_FUSED, b.bytes);
+            launch_mark_begin();
             launch_prepared(sim_.stream_, state_, n, b, static_cast<const float *>(rdev_));
+            launch_mark_end();
             sim_.prof_end();
             blocks_applied_ += 1;
             gates_applied_ += b.gates;

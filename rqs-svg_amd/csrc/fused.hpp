@@ -156,6 +156,16 @@ namespace qcore
 
     class CostModel;
 
+    // One fusion plan of a flush, as the learned planner enumerates them.
+    struct PlanOption
+    {
+        int stage_bits; // > 0: stages on tiles of 2^stage_bits amplitudes; 0: one launch per block
+        int k;          // qubits per block
+        int window;     // blocks examined after the dependency frontier (block plans)
+        int rule;       // 0: least growth, 1: fullest block, 2: earliest block that fits
+        int tiles = kTilesAuto; // stage plans: tile choice (StageTiles)
+    };
+
     class FusedEngine
     {
     public:
@@ -209,12 +219,35 @@ namespace qcore
         // Learned planner: with a model attached, each flush picks the candidate plan with
         // the lowest predicted time unless it does not beat the greedy plan by 3 %.
         void set_model(CostModel *m) { model_ = m; }
+        // A fixed plan for every flush without a model: stages on tiles of 2^tile_bits
+        // amplitudes with k-qubit blocks and tile choice `tiles` (StageTiles), or, with
+        // tile_bits 0, one launch per block planned with k, window and rule (plan_rule).
+        // enable false restores the default plan.
+        void set_plan(bool enable, int tile_bits, int k, int tiles, int window, int rule);
         // flushes, learned choices, predicted greedy us, predicted chosen us, blocks
         void planner_stats(double out[5]) const;
         // Time `count` random candidate blocks on this state; features + log2(microseconds).
         void record(int count, uint64_t seed, float *features, float *log2_us);
+        // Times the launches of every stage plan the learned planner would consider for the
+        // queued gates, each plan applied in turn (the state afterwards is not the circuit's
+        // result); at most `capacity` rows of features + log2(microseconds); returns the count.
+        int record_plans(int capacity, float *features, float *log2_us);
+        // the plan the learned planner ran at the last flush
+        PlanOption last_choice() const { return last_choice_; }
 
     private:
+        // the plan of one flush: the model's choice, the fixed plan, or the default
+        void apply_plan(const std::vector<ResolvedGate> &gates);
+        // candidate plans of the learned planner (index 0: the default plan)
+        std::vector<PlanOption> plan_options() const;
+        // feature rows of a plan's launches, in the order apply_option issues them
+        void option_features(const std::vector<ResolvedGate> &gates, const PlanOption &o, std::vector<float> &X) const;
+        void apply_option(const std::vector<ResolvedGate> &gates, const PlanOption &o);
+        // FP64 norm and rescale of a complex64 state (no-op for complex128)
+        void renormalize();
+        // events around each launch while launch_log_ is set (plan recording)
+        void launch_mark_begin();
+        void launch_mark_end();
         void apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan);
         void apply_stages(const std::vector<ResolvedGate> &gates, int m, int k, int tiles = kTilesAuto);
         void reserve_matrices(size_t bytes);
@@ -236,6 +269,12 @@ namespace qcore
         uint64_t blocks_applied_ = 0;
         uint64_t gates_applied_ = 0;
         CostModel *model_ = nullptr;
+        PlanOption last_choice_{0, 0, 0, 0, kTilesAuto};
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> *launch_log_ = nullptr;
+        std::vector<hipEvent_t> ev_pool_;
+        size_t ev_used_ = 0;
+        bool forced_ = false;
+        int forced_bits_ = 0, forced_k_ = 0, forced_tiles_ = kTilesAuto, forced_window_ = 64, forced_rule_ = 0;
         double stat_flushes_ = 0, stat_learned_ = 0, stat_pred_greedy_ = 0, stat_pred_chosen_ = 0;
     };
 } // namespace qcore

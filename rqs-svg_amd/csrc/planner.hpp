@@ -51,14 +51,6 @@ namespace qcore
         int step_ = 0;
     };
 
-    struct PlanOption
-    {
-        int stage_bits; // > 0: stages on tiles of 2^stage_bits amplitudes; 0: one launch per block
-        int k;          // qubits per block
-        int window;     // blocks examined after the dependency frontier (block plans)
-        int rule;       // 0: least growth, 1: fullest block, 2: earliest block that fits
-        int tiles = kTilesAuto; // stage plans: tile choice (StageTiles)
-    };
     std::vector<PlannedBlock> plan_with(const std::vector<ResolvedGate> &gates, const PlanOption &opt);
     // Block plans: k from max(2, kq - 2) to kq x windows {4, 64} x rules {0, 1, 2}; stage
     // plans: each tile size in `tile_bits` x each block size in `stage_k` x tile choice

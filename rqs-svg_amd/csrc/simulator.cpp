@@ -940,6 +940,13 @@ namespace qcore
         (void)hipFree(work);
     }
 
+    void Simulator::set_plan(bool enable, int tile_bits, int k, int tiles, int window, int rule)
+    {
+        if (!fused_)
+            throw std::runtime_error("a fixed fusion plan needs a fused precision mode (1, 2 or 3, allocated)");
+        fused_->set_plan(enable, tile_bits, k, tiles, window, rule);
+    }
+
     void Simulator::set_planner_model(CostModel *m)
     {
         if (!fused_)
@@ -954,6 +961,26 @@ namespace qcore
             throw std::runtime_error("the fusion planner needs a fused precision mode (1, 2 or 3, allocated)");
         flush();
         fused_->planner_stats(out);
+    }
+
+    int Simulator::planner_record_plans(int capacity, float *features, float *log2_us)
+    {
+        if (!fused_)
+            throw std::runtime_error("recording plan timings needs a fused precision mode (1, 2 or 3, allocated)");
+        state_changed();
+        return fused_->record_plans(capacity, features, log2_us);
+    }
+
+    void Simulator::planner_last_choice(int out[5])
+    {
+        if (!fused_)
+            throw std::runtime_error("the fusion planner needs a fused precision mode (1, 2 or 3, allocated)");
+        const PlanOption o = fused_->last_choice();
+        out[0] = o.stage_bits;
+        out[1] = o.k;
+        out[2] = o.tiles;
+        out[3] = o.window;
+        out[4] = o.rule;
     }
 
     void Simulator::planner_record(int count, uint64_t seed, float *features, float *log2_us)

@@ -71,8 +71,11 @@ namespace qcore
         void flush();
         void overlap(Simulator &other, double out[3]);
         void set_planner_model(CostModel *m);
+        void set_plan(bool enable, int tile_bits, int k, int tiles, int window, int rule);
         void planner_stats(double out[5]);
         void planner_record(int count, uint64_t seed, float *features, float *log2_us);
+        int planner_record_plans(int capacity, float *features, float *log2_us);
+        void planner_last_choice(int out[5]);
         void fusion_stats(uint64_t *gates, uint64_t *blocks);
 
         int event_create();

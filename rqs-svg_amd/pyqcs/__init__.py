@@ -127,6 +127,7 @@ class Library:
         lib.qcs_ext_set_stage_bits.argtypes = [vp, ctypes.c_int]
         lib.qcs_ext_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.qcs_ext_flush.argtypes = [vp]
+        lib.qcs_ext_set_plan.argtypes = [vp] + [ctypes.c_int] * 6
         lib.qcs_ext_fusion_stats.argtypes = [vp, P(u64), P(u64)]
         lib.qcs_ext_overlap.argtypes = [vp, vp, vp]
         lib.qcs_ext_profile.argtypes = [vp, ctypes.c_int]
@@ -145,6 +146,8 @@ class Library:
         lib.qcs_ext_planner_record.argtypes = [vp, ctypes.c_int, u64, vp, vp]
         lib.qcs_ext_set_planner.argtypes = [vp, vp]
         lib.qcs_ext_planner_stats.argtypes = [vp, vp]
+        lib.qcs_ext_planner_record_plans.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+        lib.qcs_ext_planner_last_choice.argtypes = [vp, vp]
         lib.qcs_ext_batch_create.argtypes = [P(vp)]
         lib.qcs_ext_batch_destroy.argtypes = [vp]
         lib.qcs_ext_batch_add_circuit.argtypes = [vp, bit_num_t, ctypes.c_int32, P(ctypes.c_char_p), P(bit_num_t), P(bit_num_t), P(ctypes.c_double)]
@@ -348,6 +351,11 @@ class Simulator:
     def flush(self):
         self._call(self.lib.qcs_ext_flush(self.h), "flush")
 
+    def set_plan(self, tile_bits=None, k=3, tiles=2, window=64, rule=0):
+        """Fixed fusion plan for every flush (qcs_ext_set_plan); tile_bits None: default."""
+        en = tile_bits is not None
+        self._call(self.lib.qcs_ext_set_plan(self.h, int(en), tile_bits or 0, k, tiles, window, rule), "set_plan")
+
     def fusion_stats(self):
         g, b = ctypes.c_uint64(), ctypes.c_uint64()
         self._call(self.lib.qcs_ext_fusion_stats(self.h, ctypes.byref(g), ctypes.byref(b)), "fusion_stats")
@@ -461,6 +469,24 @@ def planner_record(sim, count, seed):
     y = np.empty(count, dtype=np.float32)
     sim._call(sim.lib.qcs_ext_planner_record(sim.h, count, seed, X.ctypes.data, y.ctypes.data), "planner_record")
     return X, y
+
+
+def planner_record_plans(sim, capacity=4096):
+    """Times the launches of every stage plan the planner considers for the gates queued on
+    `sim` (applied once per plan): (features [count, F], log2 microseconds [count])."""
+    F = planner_num_features(sim.L)
+    X = np.empty((capacity, F), dtype=np.float32)
+    y = np.empty(capacity, dtype=np.float32)
+    c = ctypes.c_int()
+    sim._call(sim.lib.qcs_ext_planner_record_plans(sim.h, capacity, X.ctypes.data, y.ctypes.data, ctypes.byref(c)), "planner_record_plans")
+    return X[: c.value].copy(), y[: c.value].copy()
+
+
+def planner_last_choice(sim):
+    """(stage tile bits, block qubits, tile choice, window, rule) of the last learned flush."""
+    out = (ctypes.c_int * 5)()
+    sim._call(sim.lib.qcs_ext_planner_last_choice(sim.h, out), "planner_last_choice")
+    return tuple(out)
 
 
 def set_planner(sim, model):

@@ -18,6 +18,8 @@ Workloads:
 * ``variational`` -- ``depth`` layers of RY and RZ (U[-pi, pi]) on every qubit plus a
   CNOT ladder q_i -> q_{i+1}.
 * ``qft`` -- H plus controlled-phase ladder plus the final swaps.
+* ``qaoa_ring`` -- QAOA rounds of RZZ on a ring and RX mixers (planner workload).
+* ``narrow_random`` -- a deep random circuit on the top qubits only (planner workload).
 """
 from __future__ import annotations
 
@@ -142,6 +144,38 @@ def qft(n: int):
             circ.append(("p", (j,), (math.pi / (1 << (j - k)),), (k,), ()))
     for i in range(n // 2):
         circ.append(("swap", (i, n - 1 - i), (), (), ()))
+    return circ
+
+
+def qaoa_ring(n: int, p: int, seed: int):
+    """QAOA on a ring: H on every qubit, then p rounds of RZZ(gamma) on the pairs (q, q+1
+    mod n) and RX(beta) on every qubit, gamma and beta ~ U[0, pi) per round."""
+    rng = SplitMix64(seed)
+    circ = [("h", (q,), (), (), ()) for q in range(n)]
+    for _ in range(p):
+        gamma, beta = math.pi * rng.uniform(), math.pi * rng.uniform()
+        circ += [("rzz", (q, (q + 1) % n), (gamma,), (), ()) for q in range(n)]
+        circ += [("rx", (q,), (beta,), (), ()) for q in range(n)]
+    return circ
+
+
+def narrow_random(n: int, width: int, count: int, seed: int):
+    """`count` random gates on the top `width` qubits only: U3 (angles ~ U[0, 2pi)) or CX
+    on a random ordered pair, each with probability 1/2 (a deep circuit on a register at the
+    high end of the index, the case where a wide tile of high qubits serves a whole flush)."""
+    rng = SplitMix64(seed)
+    qs = list(range(n - width, n))
+    circ = []
+    for _ in range(count):
+        if rng.uniform() < 0.5:
+            q = qs[rng.randbelow(width)]
+            circ.append(("u3", (q,), tuple(2 * math.pi * rng.uniform() for _ in range(3)), (), ()))
+        else:
+            a = qs[rng.randbelow(width)]
+            b = a
+            while b == a:
+                b = qs[rng.randbelow(width)]
+            circ.append(("x", (b,), (), (a,), ()))
     return circ
 
 

@@ -34,4 +34,10 @@ def random_state(n, seed):
 
 
 def fidelity(a, b):
+    """Normalised fidelity |<a|b>|^2 / (<a|a> <b|b>): blind to a uniform scale of either."""
     return abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+def overlap_raw(a, b):
+    """Unnormalised |<a|b>|^2 against a unit-norm reference b: a norm drift of a counts."""
+    return abs(np.vdot(a, b)) ** 2 / np.vdot(b, b).real

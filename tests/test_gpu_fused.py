@@ -3,6 +3,8 @@
 
 Bars: fidelity >= 1 - 1e-6 against the FP64 oracle (and against the GPU FP64 plain path at
 sizes the oracle does not reach); max|d amp| is reported and bounded by 1e-5 (FP32 level).
+Where a norm drift could hide behind the normalised fidelity (BF16x3 products), the
+unnormalised |<psi128|psi64>|^2 and |sum |a|^2 - 1| are held to 1e-6 as well.
 """
 import math
 import sys
@@ -12,7 +14,7 @@ import pytest
 
 import pyqcs
 from pyqcs import circuits
-from conftest import random_state, fidelity
+from conftest import random_state, fidelity, overlap_raw
 
 pytestmark = pytest.mark.gpu
 
@@ -135,6 +137,31 @@ def test_window_in_fused_mode(gpu, oracle):
 
 
 @pytest.mark.parametrize("precision", [1, 3])
+@pytest.mark.parametrize("workload", ["brick", "qft"])
+def test_unnormalised_overlap_22q(gpu, precision, workload):
+    # 22 qubits, brick depth 32 (the 32-qubit benchmark family: ~7 blocks per layer) or the
+    # QFT on the state of a random brick layer, stages on 2^13 tiles, against the plain
+    # complex128 per-gate path: normalised and unnormalised overlap, and the norm itself
+    n = 22
+    prep = circuits.brick(n, 1, seed=8)
+    circ = circuits.brick(n, 32, seed=9) if workload == "brick" else prep + circuits.qft(n)
+    say(f"unnormalised overlap n={n} {workload} precision={precision}")
+    with pyqcs.Simulator(gpu, n) as plain, pyqcs.Simulator(gpu, n, precision=precision, block_qubits=5, stage_bits=13) as fused:
+        plain.run(circ)
+        fused.run(circ)
+        ov, md = fused.overlap(plain)
+        norm64, norm128 = fused.norm(), plain.norm()
+        fid = abs(ov) ** 2 / (norm64 * norm128)
+        raw = abs(ov) ** 2 / norm128
+        say(f"  fidelity {fid:.12f} unnormalised {raw:.12f} |norm-1| {abs(norm64 - 1):.2e} max|d| {md:.2e}")
+        assert abs(norm128 - 1) <= 1e-12
+        assert fid >= 1 - FID_TOL and raw >= 1 - FID_TOL and abs(norm64 - 1) <= FID_TOL
+        assert md <= AMP_TOL
+        psi = fused.statevector()
+        assert abs(float(np.vdot(psi, psi).real) - 1) <= FID_TOL
+
+
+@pytest.mark.parametrize("precision", [1, 3])
 def test_overlap_with_plain_gpu_path(gpu, precision):
     n = 22
     circ = circuits.brick(n, 10, seed=7, mcx=4)
@@ -212,9 +239,10 @@ def test_stages_match_oracle(gpu, oracle, precision, m):
             o.run(circ)
             want = o.statevector()
         fid = fidelity(got, want)
+        raw = overlap_raw(got, want)
         err = np.max(np.abs(got - want))
-        say(f"  fidelity {fid:.12f} max|d| {err:.2e} gates {gates} launches {launches}")
-        assert fid >= 1 - FID_TOL and err <= AMP_TOL
+        say(f"  fidelity {fid:.12f} unnormalised {raw:.12f} max|d| {err:.2e} gates {gates} launches {launches}")
+        assert fid >= 1 - FID_TOL and raw >= 1 - FID_TOL and err <= AMP_TOL
         assert gates == len(circ)
 
 
