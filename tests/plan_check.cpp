@@ -6,8 +6,11 @@
 // stages of diagonal gates (qcore::prepare_stage128 and prepare_stage with tables): entry
 // idx against the product of the gates' own phases on the tile qubits' bits of idx, from
 // the gate definitions (P(t) = diag(1, e^{it}), RZ(t) = diag(e^{-it/2}, e^{it/2}), Z, S, T;
-// controls: the phase only where every control bit is 1).  Prints one line per case and
-// "ok" at the end; exits 1 at the first violation.
+// controls: the phase only where every control bit is 1); and the trailing tables of
+// stages of dense gates followed by diagonal ones (the greedy blocks after the last dense
+// block, applied as a table at the store): the LDS block count, the table's alignment and
+// every entry against the trailing gates' phases.  Prints one line per case and "ok" at
+// the end; exits 1 at the first violation.
 // Usage: plan_check <cases> <seed>
 #include "fused.hpp"
 
@@ -226,6 +229,133 @@ int main(int argc, char **argv)
                 fail("phase table differs from the gates' phases", c);
             std::printf("table case %d n %d window %d gates %d kq %d: max|d| %.1e (c128) %.1e (c64)\n", c, n, w0, ng, kq, e128, e64);
         }
+    }
+    // trailing tables: dense gates, then a run of diagonal ones; the greedy blocks up to the
+    // last one with a dense gate run through LDS (nblocks), the diagonal blocks after it
+    // become one table applied at the store (trail_offset, 4-float aligned for complex64)
+    for (int c = 0; c < cases; ++c)
+    {
+        const int n = 16 + int(mix(seed) % 6);
+        const int m = 12;
+        std::vector<int> tile = {0, 1, 2, 3};
+        const int w0 = 4 + int(mix(seed) % (n - 11));
+        for (int q = w0; q < w0 + 8; ++q)
+            tile.push_back(q);
+        const char *names[] = {"p", "rz", "z", "s", "t"};
+        std::vector<ResolvedGate> gates;
+        std::vector<double> angle;
+        auto pick = [&]() {
+            std::vector<int> pool = tile;
+            for (int q = m - 1; q > 0; --q)
+                std::swap(pool[q], pool[mix(seed) % (q + 1)]);
+            return pool;
+        };
+        const int ndense = 1 + int(mix(seed) % 6);
+        for (int i = 0; i < ndense; ++i)
+        {
+            ResolvedGate r;
+            double prm[4] = {0.4 + 0.1 * i, 1.3, -0.7, 0};
+            const auto pool = pick();
+            r.spec = make_gate(i % 2 ? "x" : "u3", prm);
+            r.tphys = {pool[0]};
+            if (i % 2)
+                r.pphys = {pool[1]};
+            gates.push_back(r);
+            angle.push_back(0.0);
+        }
+        const int ndiag = 1 + int(mix(seed) % 30);
+        for (int i = 0; i < ndiag; ++i)
+        {
+            ResolvedGate r;
+            const int kind = int(mix(seed) % 5);
+            double prm[4] = {6.283 * double(mix(seed) >> 11) * 0x1.0p-53, 0, 0, 0};
+            r.spec = make_gate(names[kind], prm);
+            const auto pool = pick();
+            r.tphys = {pool[0]};
+            const int nc = kind == 1 ? 0 : int(mix(seed) % 3);
+            for (int j = 0; j < nc; ++j)
+                r.pphys.push_back(pool[1 + j]);
+            gates.push_back(r);
+            angle.push_back(prm[0]);
+        }
+        PlannedStage st;
+        st.tile = tile;
+        for (int i = 0; i < (int)gates.size(); ++i)
+            st.gates.push_back(i);
+        // product of the phases of the listed (diagonal) gates on tile index idx
+        auto expect = [&](int idx, const std::vector<int> &which) {
+            std::complex<double> v(1.0, 0.0);
+            auto bit = [&](int q) { return (idx >> (std::find(tile.begin(), tile.end(), q) - tile.begin())) & 1; };
+            for (int i : which)
+            {
+                const ResolvedGate &g = gates[i];
+                if (!diagonal(g))
+                    fail("a dense gate in the trailing table", c);
+                bool on = true;
+                for (int q : g.pphys)
+                    on = on && bit(q);
+                if (!on)
+                    continue;
+                const int t = bit(g.tphys[0]);
+                const double a = angle[i];
+                const std::string nm = g.spec.name;
+                if (nm == "rz")
+                    v *= std::polar(1.0, t ? a / 2 : -a / 2);
+                else if (t)
+                    v *= std::polar(1.0, nm == "p" ? a : nm == "z" ? M_PI : nm == "s" ? M_PI / 2 : M_PI / 4);
+            }
+            return v;
+        };
+        for (int kq : {3, 4})
+            for (bool f64 : {true, false})
+            {
+                const int kb = f64 ? kq : kq + 1; // complex64 stages take 4- or 5-qubit blocks
+                const auto plan = plan_greedy(gates, kb);
+                const int lds = f64 ? stage128_lds_blocks(gates, kb) : stage64_lds_blocks(gates, kb);
+                std::vector<int> trail;
+                for (int b = lds; b < (int)plan.size(); ++b)
+                    trail.insert(trail.end(), plan[b].gates.begin(), plan[b].gates.end());
+                if (lds < 1)
+                    fail("a stage with dense gates has no LDS block", c);
+                size_t toff, nblocks;
+                std::vector<double> mats64;
+                std::vector<float> mats;
+                if (f64)
+                {
+                    const auto p = prepare_stage128(gates, st, n, mats64, kb);
+                    toff = p.trail_offset;
+                    nblocks = p.nblocks;
+                    if (p.table_offset != SIZE_MAX)
+                        fail("a stage with dense gates ran as a table", c);
+                }
+                else
+                {
+                    const auto p = prepare_stage(gates, st, kb, m, n, mats, false, true);
+                    toff = p.trail_offset;
+                    nblocks = p.nblocks;
+                    if (p.table_offset != SIZE_MAX)
+                        fail("a stage with dense gates ran as a table", c);
+                    if (toff != SIZE_MAX && toff % 4 != 0)
+                        fail("complex64 trailing table not 4-float aligned", c);
+                }
+                if ((int)nblocks != lds)
+                    fail("LDS block count differs from the blocks up to the last dense one", c);
+                if (trail.empty() != (toff == SIZE_MAX))
+                    fail("trailing table present without trailing diagonal blocks, or missing", c);
+                double err = 0;
+                if (!trail.empty())
+                    for (int idx = 0; idx < (1 << m); ++idx)
+                    {
+                        const std::complex<double> want = expect(idx, trail);
+                        const std::complex<double> got = f64 ? std::complex<double>(mats64[toff + 2 * idx], mats64[toff + 2 * idx + 1])
+                                                             : std::complex<double>(mats[toff + 2 * idx], mats[toff + 2 * idx + 1]);
+                        err = std::max(err, std::abs(got - want));
+                    }
+                if (err > (f64 ? 1e-12 : 1e-6))
+                    fail("trailing table differs from the trailing gates' phases", c);
+                std::printf("trail case %d n %d dense %d diagonal %d k %d %s: %d LDS blocks, %zu trailing gates, max|d| %.1e\n",
+                            c, n, ndense, ndiag, kb, f64 ? "c128" : "c64", lds, trail.size(), err);
+            }
     }
     std::printf("ok\n");
     return 0;
