@@ -338,7 +338,10 @@ SUBSYSTEM_GATES = (
     ("dcx", 0, "mono", lambda n: [(0, 1), (14, 15), (n - 2, n - 1)]),
     ("rccx", 0, "mono", lambda n: [(0, 1, 2), (14, 15, 16), (n - 3, n - 2, n - 1)]),
     ("swap", 0, "pair", lambda n: [(0, 1), (1, 2), (0, n - 1), (14, 15), (n - 2, n - 1)]),
-    ("u3", 3, "pair", lambda n: [(0,), (2,), (6,), (12,), (20,), (n - 1,)]),
+    ("u3", 3, "pair", lambda n: [(t,) for t in range(n)]),
+    # (targets, controls): a control on qubit 0 (the GHZ chain's CX), and the MCX of the
+    # headline circuit with its controls high and low
+    ("x", 0, "pair", lambda n: [((n - 1,), (0,)), ((5,), (0,)), ((n - 1,), (n - 4, n - 3, n - 2)), ((n - 1,), (0, 1, 2))]),
 )
 
 
@@ -369,17 +372,21 @@ def subsystem_leg(n=30, reps=4):
     for name, npar, cls, ops in SUBSYSTEM_GATES:
         rows = []
         for opnd in ops(n):
+            tg, ctl = opnd if isinstance(opnd[0], tuple) else (opnd, ())
             params = tuple(0.3 + 0.1 * i for i in range(npar))
             sim.profile_reset()
             sim.profile(True)
             for _ in range(reps):
-                sim.gate(name, opnd, params)
+                sim.gate(name, tg, params, ctrls=ctl)
             r = _class_rate(sim, cls)
             sim.profile(False)
-            r["operands"] = list(opnd)
+            r["operands"] = list(tg)
+            if ctl:
+                r["controls"] = list(ctl)
             rows.append(r)
-            log(f"subsystems: {name}{tuple(opnd)} {cls}: {r['avg_ms']:.3f} ms, {r['GBps']:.0f} GB/s ({r['frac']:.2f})")
-        gates[name] = rows
+            log(f"subsystems: {name}{tuple(tg)}{' ctrl ' + str(tuple(ctl)) if ctl else ''} {cls}: "
+                f"{r['avg_ms']:.3f} ms, {r['GBps']:.0f} GB/s ({r['frac']:.2f})")
+        gates[name if name != "x" else "controlled_x"] = rows
     out["gates"] = gates
     ident = list(range(n))
     rng = np.random.default_rng(CIRCUIT_SEED)
@@ -441,79 +448,160 @@ def subsystem_leg(n=30, reps=4):
     return out
 
 
-def planner_leg(n, record_qubits, record_count, depth, k):
-    """Learned fusion planner on complex128 fused blocks: record launch timings (single
-    blocks and multi-block stages) on states of `record_qubits` qubits, train the cost-model
-    MLP on the GPU, then run a QFT and a random brick circuit at n qubits three ways: the
-    greedy fusion fallback (one sweep per block), the plan the model picks among block plans
-    and stage plans (kept only when predicted 3 % faster than the greedy plan), and the
-    stage heuristic alone (stages on 2^12-amplitude LDS tiles) for reference.  The
-    complex128 results must agree (<Z_q> in FP64); agreement with the FP64 oracle is covered
-    by the tests at smaller n."""
+def _timed_run(sim, circ, prep=(), reps=1):
+    """Median seconds of: gates queued and applied (flush) plus a norm; the state is reset
+    to |0...0> and `prep` applied (untimed) before each repetition."""
+    ts = []
+    for _ in range(reps):
+        sim.call("set_zero_state")
+        if prep:
+            sim.run(prep)
+        sim.flush()
+        sim.norm()
+        t0 = time.perf_counter()
+        sim.run(circ)
+        sim.flush()
+        sim.norm()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def planner_leg(n, depth, record_count, k=4):
+    """Learned fusion planner on complex128 fused blocks (the workloads of the planner
+    configuration): a cost model trained on the GPU on recorded launch timings (random
+    blocks and stages at 26-32 qubits, and every candidate stage plan's launches on
+    training circuits at 26-30 qubits, none of them the evaluated circuits), then an n-qubit
+    QFT on the state of one random brick layer and an n-qubit brick circuit of the given
+    depth, each run three ways -- the greedy fusion fallback (one sweep per block), the
+    default stage plan, and the plan the model picks (kept only when predicted 3 % faster
+    than the default) -- and each output compared with the plain per-gate complex128 path
+    (one kernel per gate, the reference's method) on a second state in HBM."""
     lib = pyqcs.gpu_library()
-    F = pyqcs.planner_num_features(lib)
-    Xs, ys = [], []
-    t0 = time.perf_counter()
-    # launches per size fall with the state size (a 32-qubit complex128 launch streams
-    # 128 GiB) so that recording stays within tens of seconds
-    weights = [0.5 ** i for i in range(len(record_qubits))]
-    counts = [max(1, int(record_count * w / sum(weights))) for w in weights]
-    for i, (m, cnt) in enumerate(zip(record_qubits, counts)):
-        with pyqcs.Simulator(lib, m, precision=2, block_qubits=k) as s:
-            X, y = pyqcs.planner_record(s, cnt, seed=77 + i)
-        Xs.append(X)
-        ys.append(y)
-    X = np.concatenate(Xs)
-    y = np.concatenate(ys)
-    t_rec = time.perf_counter() - t0
-    model = pyqcs.Mlp(lib, F, seed=5)
-    t0 = time.perf_counter()
-    loss = model.train(X, y, epochs=300, batch=256, lr=2e-3, seed=2)
-    t_train = time.perf_counter() - t0
-    log(f"planner leg: recorded {len(y)} launches ({int(X[:, 4].sum())} stages) at "
-        f"{list(record_qubits)} qubits in {t_rec:.1f}s; trained in {t_train:.2f}s, mse {loss:.4f} "
-        f"(variance {np.var(y):.4f})")
-    out = {"config": f"complex128 fused blocks (k<={k}), {n}-qubit QFT and {n}-qubit brick "
-                     f"depth {depth}; cost model trained on {len(y)} recorded launches at "
-                     f"{min(record_qubits)}-{max(record_qubits)} qubits",
-           "recorded_launches": int(len(y)), "recorded_stages": int(X[:, 4].sum()),
-           "record_s": t_rec, "train_s": t_train,
-           "train_mse_log2us": float(loss), "target_variance": float(np.var(y))}
-    sim = pyqcs.Simulator(lib, n, seed=1, precision=2, block_qubits=k)
+    model, info = pyqcs.train_cost_model(lib, 2, k, (26, 28, 30, 32), record_count, (26, 28, 30), log=log)
+    out = {"config": f"complex128 fused blocks (k<={k}), {n}-qubit QFT on a random brick layer and "
+                     f"{n}-qubit brick depth {depth}; outputs against the plain per-gate complex128 path",
+           "cost_model": info}
+    prep_qft = circuits.brick(n, 1, seed=CIRCUIT_SEED + 9)
+    work = {"qft": (prep_qft, circuits.qft(n)), "brick": ((), circuits.brick(n, depth, seed=CIRCUIT_SEED + 5))}
     try:
-        sim.run(circuits.brick(n, 1, seed=1)[: n + 2])
-        for name, circ in (("qft", circuits.qft(n)), ("brick", circuits.brick(n, depth, seed=CIRCUIT_SEED + 5))):
+        fused = pyqcs.Simulator(lib, n, seed=1, precision=2, block_qubits=k)
+        plain = pyqcs.Simulator(lib, n, seed=1)
+    except pyqcs.QcsError:
+        n -= 1  # two complex128 states must fit in HBM
+        log(f"planner leg: two {n + 1}-qubit states do not fit; {n} qubits")
+        out["qubits_fallback"] = n
+        prep_qft = circuits.brick(n, 1, seed=CIRCUIT_SEED + 9)
+        work = {"qft": (prep_qft, circuits.qft(n)), "brick": ((), circuits.brick(n, depth, seed=CIRCUIT_SEED + 5))}
+        fused = pyqcs.Simulator(lib, n, seed=1, precision=2, block_qubits=k)
+        plain = pyqcs.Simulator(lib, n, seed=1)
+    out["qubits"] = n
+    try:
+        for name, (prep, circ) in work.items():
             res = {"gates": len(circ)}
-            zs = {}
-            # greedy: the greedy fusion fallback (one sweep per block); learned: the model
-            # chooses among block plans and stage plans, judged by its prediction for the
-            # greedy plan; stages: the stage heuristic alone, for reference
-            for mode, stage_bits, use_model in (("greedy", 0, False), ("learned", 0, True), ("stages", 12, False)):
-                sim.set_stage_bits(stage_bits)
-                pyqcs.set_planner(sim, model if use_model else None)
-                sim.call("set_zero_state")
-                sim.norm()
-                _, b0 = sim.fusion_stats()
-                t0 = time.perf_counter()
-                sim.run(circ)
-                total = sim.probability_total()
-                dt = time.perf_counter() - t0
-                _, b1 = sim.fusion_stats()
-                zs[mode] = sim.expect_z()
+            plain.call("set_zero_state")
+            plain.run(prep)
+            plain.norm()
+            t0 = time.perf_counter()
+            plain.run(circ)
+            plain.norm()
+            res["plain_per_gate_s"] = time.perf_counter() - t0
+            for mode in ("greedy", "stages", "learned"):
+                pyqcs.set_planner(fused, model if mode == "learned" else None)
+                if mode == "greedy":
+                    fused.set_plan(0, k, window=64, rule=0)
+                else:
+                    fused.set_plan(None)
+                _, b0 = fused.fusion_stats()
+                st0 = pyqcs.planner_stats(fused)
+                dt = _timed_run(fused, circ, prep)
+                _, b1 = fused.fusion_stats()
+                ov, maxdiff = fused.overlap(plain)
+                fid = abs(ov) ** 2 / (fused.norm() * plain.norm())
                 res[f"{mode}_s"] = dt
                 res[f"{mode}_sweeps"] = b1 - b0
-                res[f"{mode}_probability_total"] = total
-                if use_model:
-                    st = pyqcs.planner_stats(sim)
-                    res["pred_greedy_us"] = st["pred_greedy_us"]
-                    res["pred_chosen_us"] = st["pred_chosen_us"]
-                    res["learned_choices"] = st["learned_choices"]
-                    res["flushes"] = st["flushes"]
-                log(f"planner leg: {name} {mode}: {dt:.3f}s, {b1 - b0} sweeps")
+                res[f"{mode}_fidelity_vs_per_gate"] = fid
+                res[f"{mode}_max_abs_diff_vs_per_gate"] = maxdiff
+                if mode == "learned":
+                    st1 = pyqcs.planner_stats(fused)
+                    res["learned_choices"] = st1["learned_choices"] - st0["learned_choices"]
+                    res["flushes"] = st1["flushes"] - st0["flushes"]
+                    res["learned_plan"] = list(pyqcs.planner_last_choice(fused))
+                log(f"planner leg: {name} {mode}: {dt:.3f}s, {b1 - b0} sweeps, fidelity vs per-gate "
+                    f"{fid:.15f}, max|d| {maxdiff:.2e}")
+            pyqcs.set_planner(fused, None)
+            fused.set_plan(None)
+            res["learned_vs_stages"] = res["stages_s"] / res["learned_s"]
             res["learned_vs_greedy"] = res["greedy_s"] / res["learned_s"]
             res["stages_vs_greedy"] = res["greedy_s"] / res["stages_s"]
-            res["max_abs_diff_expz"] = float(max(np.max(np.abs(zs[a] - zs["greedy"])) for a in ("stages", "learned")))
+            log(f"planner leg: {name}: learned vs default stages {res['learned_vs_stages']:.3f}x, "
+                f"vs greedy {res['learned_vs_greedy']:.2f}x; plain per-gate {res['plain_per_gate_s']:.2f}s")
             out[name] = res
+    finally:
+        pyqcs.set_planner(fused, None)
+        fused.close()
+        plain.close()
+        model.close()
+    return out
+
+
+PRECISION_PLANNER_WORKLOADS = ("qft", "brick", "qaoa", "narrow", "random2q", "diagonal")
+
+
+def precision_planner_workloads(n):
+    """Evaluation circuits of the precision-form planner leg (seeds outside the training
+    set): prep and circuit per workload."""
+    return {"qft": (circuits.brick(n, 1, seed=CIRCUIT_SEED + 9), circuits.qft(n)),
+            "brick": ((), circuits.brick(n, 32, seed=CIRCUIT_SEED + 3)),
+            "qaoa": ((), circuits.qaoa_ring(n, 3, CIRCUIT_SEED + 11)),
+            "narrow": ((), circuits.narrow_random(n, 10, 300, CIRCUIT_SEED + 12)),
+            "random2q": ((), circuits.random_two_qubit(n, 400, CIRCUIT_SEED + 13)),
+            "diagonal": ((), circuits.diagonal_heavy(n, 400, CIRCUIT_SEED + 14))}
+
+
+def precision_planner_leg(n, record_count, fixed_sweep=False):
+    """The learned planner in the precision form (complex64 state, BF16x3 stages of 4- or
+    5-qubit blocks on 2^11-2^14-amplitude tiles): cost model trained as in the complex128
+    leg, then six n-qubit workloads run with the default stage plan (2^13 tiles, 5-qubit
+    blocks, fewer-stage tile choice) and with the model's choice (median of 3 runs each).
+    With fixed_sweep, every fixed stage plan is timed as well."""
+    lib = pyqcs.gpu_library()
+    model, info = pyqcs.train_cost_model(lib, 3, 5, (26, 28, 30, 32), record_count // 2, (26, 28, 30), log=log)
+    out = {"config": f"complex64 BF16x3 stages, {n} qubits; workloads {', '.join(PRECISION_PLANNER_WORKLOADS)}",
+           "cost_model": info}
+    sim = pyqcs.Simulator(lib, n, seed=1, precision=3, block_qubits=5)
+    try:
+        for name, (prep, circ) in precision_planner_workloads(n).items():
+            res = {"gates": len(circ)}
+            pyqcs.set_planner(sim, None)
+            res["default_s"] = _timed_run(sim, circ, prep, reps=3)
+            pyqcs.set_planner(sim, model)
+            res["learned_s"] = _timed_run(sim, circ, prep, reps=3)
+            res["learned_plan"] = list(pyqcs.planner_last_choice(sim))
+            pyqcs.set_planner(sim, None)
+            res["learned_vs_default"] = res["default_s"] / res["learned_s"]
+            if fixed_sweep:
+                fixed = {}
+                for m in (11, 12, 13, 14):
+                    for kb in (4, 5):
+                        for tiles in (0, 1):
+                            sim.set_plan(m, kb, tiles=tiles)
+                            fixed[f"{m}/{kb}/{'window' if tiles else 'grow'}"] = _timed_run(sim, circ, prep, reps=3)
+                sim.set_plan(None)
+                res["fixed_plans_s"] = fixed
+            log(f"precision planner leg: {name}: default {res['default_s'] * 1e3:.1f} ms, learned "
+                f"{res['learned_s'] * 1e3:.1f} ms ({res['learned_vs_default']:.3f}x), plan {res['learned_plan']}")
+            out[name] = res
+        if fixed_sweep:
+            # the best single fixed plan over all workloads, against the learned choices
+            names = list(out[PRECISION_PLANNER_WORKLOADS[0]]["fixed_plans_s"])
+            totals = {p: sum(out[w]["fixed_plans_s"][p] for w in PRECISION_PLANNER_WORKLOADS) for p in names}
+            best = min(totals, key=totals.get)
+            out["best_fixed_plan"] = best
+            out["best_fixed_total_s"] = totals[best]
+        out["default_total_s"] = sum(out[w]["default_s"] for w in PRECISION_PLANNER_WORKLOADS)
+        out["learned_total_s"] = sum(out[w]["learned_s"] for w in PRECISION_PLANNER_WORKLOADS)
+        log(f"precision planner leg: total default {out['default_total_s']:.3f}s, learned {out['learned_total_s']:.3f}s"
+            + (f", best fixed plan {out['best_fixed_plan']} {out['best_fixed_total_s']:.3f}s" if fixed_sweep else ""))
     finally:
         pyqcs.set_planner(sim, None)
         sim.close()
@@ -551,8 +639,11 @@ def main():
     ap.add_argument("--planner-depth", type=int, default=40)
     ap.add_argument("--planner-k", type=int, default=4)
     ap.add_argument("--planner-record", type=int, default=10000,
-                    help="blocks timed for the cost model over 26, 28, 30 and 32 qubits "
-                         "(8:4:2:1)")
+                    help="random blocks and stages timed for the complex128 cost model over 26, 28, "
+                         "30 and 32 qubits (8:4:2:1; half as many for the precision-form model), "
+                         "besides the launches of the candidate plans of the training circuits")
+    ap.add_argument("--planner-fixed-sweep", action="store_true",
+                    help="also time every fixed stage plan on the precision-form planner workloads")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one write-only fill and one read-only norm pass of the "
                          "full state (known HBM bytes, for scaling PMC byte counters)")
@@ -657,8 +748,9 @@ def main():
     if not args.no_batch:
         result["batch"] = batch_leg(args.batch_circuits, direct=args.batch_no_graph, tuning=args.batch_tuning)
     if not args.no_planner:
-        result["planner"] = planner_leg(args.planner_qubits, (26, 28, 30, 32), args.planner_record,
-                                        args.planner_depth, args.planner_k)
+        result["planner"] = planner_leg(args.planner_qubits, args.planner_depth, args.planner_record, args.planner_k)
+        result["planner_precision"] = precision_planner_leg(args.fused_qubits, args.planner_record,
+                                                            fixed_sweep=args.planner_fixed_sweep)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(n, circ, args.cpu_budget)
     print(json.dumps(result), flush=True)

@@ -1241,7 +1241,12 @@ namespace qsv
             }
         }
         const bool nt = use_nt(n);
-        if (items >= (uint64_t(1) << 16) && nt)
+        // states beyond the memory-side cache: one pair per thread (30 qubits, every target
+        // position from 3 to 29: 3-4 % faster than two pairs per thread, e.g. target 20
+        // 6.28 -> 6.05 ms, target 6 5.76 -> 5.52 ms)
+        if (items >= (uint64_t(1) << 16) && nt && n > kCachedMaxQubits)
+            hipLaunchKernelGGL((k_pair<1, true>), dim3(grid_for(items, kBlock)), dim3(kBlock), 0, s, state, items, sub, op);
+        else if (items >= (uint64_t(1) << 16) && nt)
             hipLaunchKernelGGL((k_pair<2, true>), dim3(grid_for(items, kBlock * 2)), dim3(kBlock), 0, s, state, items, sub, op);
         else if (items >= (uint64_t(1) << 16))
             hipLaunchKernelGGL((k_pair<2, false>), dim3(grid_for(items, kBlock * 2)), dim3(kBlock), 0, s, state, items, sub, op);

@@ -15,6 +15,7 @@ callback.
 from __future__ import annotations
 
 import ctypes
+import time
 import os
 import pathlib
 
@@ -487,6 +488,51 @@ def planner_last_choice(sim):
     out = (ctypes.c_int * 5)()
     sim._call(sim.lib.qcs_ext_planner_last_choice(sim.h, out), "planner_last_choice")
     return tuple(out)
+
+
+def train_cost_model(library, precision, block_qubits, random_sizes, random_count, plan_sizes,
+                     plan_seeds=(1001, 1002, 1003), epochs=300, seed=5, log=None):
+    """Records launch timings on the GPU and trains a cost model on them with the GPU
+    kernels: `random_count` random blocks and stages spread over `random_sizes` qubits
+    (halving per size step), plus the launches of every stage plan the planner considers
+    for the circuits of circuits.planner_training_set(m, s) at each m in `plan_sizes` and s
+    in `plan_seeds`.  Returns (model, info)."""
+    from . import circuits
+
+    F = planner_num_features(library)
+    Xs, ys = [], []
+    t0 = time.perf_counter()
+    weights = [0.5 ** i for i in range(len(random_sizes))]
+    for i, m in enumerate(random_sizes):
+        cnt = max(1, int(random_count * weights[i] / sum(weights)))
+        with Simulator(library, m, precision=precision, block_qubits=block_qubits) as s:
+            X, y = planner_record(s, cnt, seed=77 + i)
+        Xs.append(X)
+        ys.append(y)
+    n_random = sum(len(y) for y in ys)
+    t_random = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for m in plan_sizes:
+        with Simulator(library, m, precision=precision, block_qubits=block_qubits) as s:
+            for sd in plan_seeds:
+                for circ in circuits.planner_training_set(m, sd).values():
+                    s.run(circ)
+                    X, y = planner_record_plans(s, capacity=8192)
+                    Xs.append(X)
+                    ys.append(y)
+    t_plans = time.perf_counter() - t0
+    X = np.concatenate(Xs)
+    y = np.concatenate(ys)
+    model = Mlp(library, F, seed=seed)
+    t0 = time.perf_counter()
+    loss = model.train(X, y, epochs=epochs, batch=256, lr=2e-3, seed=2)
+    info = {"random_rows": int(n_random), "plan_rows": int(len(y) - n_random), "record_random_s": t_random,
+            "record_plans_s": t_plans, "train_s": time.perf_counter() - t0, "train_mse_log2us": float(loss),
+            "target_variance": float(np.var(y)), "random_sizes": list(random_sizes), "plan_sizes": list(plan_sizes)}
+    if log:
+        log(f"cost model (precision {precision}): {info['random_rows']} random + {info['plan_rows']} plan launches "
+            f"recorded in {t_random + t_plans:.1f}s, trained in {info['train_s']:.1f}s, mse {loss:.4f} (variance {np.var(y):.4f})")
+    return model, info
 
 
 def set_planner(sim, model):

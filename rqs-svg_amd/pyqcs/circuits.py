@@ -179,6 +179,59 @@ def narrow_random(n: int, width: int, count: int, seed: int):
     return circ
 
 
+def random_two_qubit(n: int, count: int, seed: int):
+    """U3 (40 %), CX (30 %) and controlled phase (30 %) on random qubits (pairs anywhere)."""
+    rng = SplitMix64(seed)
+    circ = []
+    for _ in range(count):
+        r = rng.uniform()
+        a = rng.randbelow(n)
+        b = a
+        while b == a:
+            b = rng.randbelow(n)
+        if r < 0.4:
+            circ.append(("u3", (a,), tuple(2 * math.pi * rng.uniform() for _ in range(3)), (), ()))
+        elif r < 0.7:
+            circ.append(("x", (b,), (), (a,), ()))
+        else:
+            circ.append(("p", (b,), (2 * math.pi * rng.uniform(),), (a,), ()))
+    return circ
+
+
+def diagonal_heavy(n: int, count: int, seed: int):
+    """H on every qubit, then controlled phases (45 %), RZ (45 %) and H (10 %) on random
+    qubits: stages that are mostly phase tables."""
+    rng = SplitMix64(seed)
+    circ = [("h", (q,), (), (), ()) for q in range(n)]
+    for _ in range(count):
+        r = rng.uniform()
+        a = rng.randbelow(n)
+        b = a
+        while b == a:
+            b = rng.randbelow(n)
+        if r < 0.45:
+            circ.append(("p", (b,), (2 * math.pi * rng.uniform(),), (a,), ()))
+        elif r < 0.9:
+            circ.append(("rz", (a,), (2 * math.pi * rng.uniform(),), (), ()))
+        else:
+            circ.append(("h", (a,), (), (), ()))
+    return circ
+
+
+def planner_training_set(n: int, seed: int):
+    """Circuits whose queued gates the planner's plan recorder times (one of each family,
+    sizes and depths varied by the seed): QFT on a brick layer, brick, QAOA ring, a narrow
+    register, random two-qubit gates, diagonal-heavy."""
+    return {
+        "qft": brick(n, 1, seed=seed) + qft(n),
+        "brick": brick(n, 8 + seed % 16, seed=seed),
+        "qaoa": qaoa_ring(n, 1 + seed % 4, seed),
+        "narrow": narrow_random(n, 6 + seed % 8, 100 + 40 * (seed % 6), seed),
+        "random2q": random_two_qubit(n, 200 + 50 * (seed % 5), seed),
+        "diagonal": diagonal_heavy(n, 200 + 50 * (seed % 5), seed),
+    }
+
+
 def random_all_gates(n: int, count: int, seed: int, max_ctrls: int = 2):
     """Random circuit over the whole gate set, random positive/negative controls."""
     from . import GATE_PARAMS, GATE_TARGETS

@@ -132,6 +132,32 @@ def test_gpu_batch_bf16x3(gpu, oracle):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", [1, 3])
+def test_gpu_batch_large_circuits_vs_oracle(gpu, oracle, mode):
+    # the upper end of the benchmark batch, 19-22 qubits (complex64 states in HBM, stage
+    # rounds), beside small LDS-resident circuits, against the oracle's FP64 plain method;
+    # also the GPU plain loop (complex128, one kernel per gate) at those sizes
+    work = circuits.variational_batch(5, seed=41, nmin=19, nmax=22, dmin=2, dmax=5)
+    work += circuits.variational_batch(4, seed=42, nmin=6, nmax=12, dmin=2, dmax=5)
+    assert max(n for n, _ in work) >= 21
+    ref = pyqcs.Batch(oracle)
+    b = pyqcs.Batch(gpu)
+    for n, g in work:
+        ref.add(n, g)
+        b.add(n, g)
+    want, _ = ref.run()
+    b.set_precision(mode)
+    got, secs = b.run()
+    err = max(np.max(np.abs(x - y)) for x, y in zip(got, want))
+    print(f"19-22 qubit batch mode {mode}: max |d<Z>| {err:.2e} device {secs * 1e3:.2f} ms", flush=True)
+    assert err < (2e-5 if mode == 1 else 5e-5)
+    if mode == 1:
+        plain, _ = b.run_plain()
+        assert max(np.max(np.abs(x - y)) for x, y in zip(plain, want)) < 1e-12
+    b.close()
+
+
+@pytest.mark.gpu
 def test_gpu_batch_traffic(gpu, oracle):
     # a gate-free 16-qubit circuit in HBM: one fill (8 B per amplitude) and one <Z>
     # reduction (8 B), 3 launches; an LDS-resident circuit: no HBM bytes, 2 launches

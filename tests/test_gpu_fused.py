@@ -247,6 +247,26 @@ def test_stages_match_oracle(gpu, oracle, precision, m):
 
 
 @pytest.mark.parametrize("precision", [1, 3])
+def test_window_tile_brick_stages_vs_oracle(gpu, oracle, precision):
+    # the benchmark's brick family on window tiles of 2^13 amplitudes (qubits 0..3 and a run
+    # of 9 consecutive qubits), 5-qubit blocks, against the FP64 oracle at 20 qubits
+    n = 20
+    circ = circuits.brick(n, 12, seed=44, mcx=2)
+    with pyqcs.Simulator(gpu, n, precision=precision, block_qubits=5) as f:
+        f.set_plan(13, 5, tiles=1)
+        f.run(circ)
+        got = f.statevector()
+        gates, launches = f.fusion_stats()
+    with pyqcs.Simulator(oracle, n) as o:
+        o.run(circ)
+        want = o.statevector()
+    fid, raw, err = fidelity(got, want), overlap_raw(got, want), np.max(np.abs(got - want))
+    say(f"window-tile brick n={n} precision={precision}: fidelity {fid:.12f} unnormalised {raw:.12f} max|d| {err:.2e} launches {launches}")
+    assert fid >= 1 - FID_TOL and raw >= 1 - FID_TOL and err <= AMP_TOL
+    assert gates == len(circ) and launches * 10 <= len(circ)
+
+
+@pytest.mark.parametrize("precision", [1, 3])
 def test_stages_vs_block_launches(gpu, precision):
     n = 20
     circ = circuits.brick(n, 12, seed=5, mcx=3)

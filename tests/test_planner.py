@@ -136,3 +136,54 @@ def test_planner_record_train_and_plan(gpu, oracle, precision, tol):
         want = o.statevector()
     for s in states:
         assert fidelity(s, want) >= 1 - tol
+
+
+def _median_run(sim, circ, reps=5):
+    import time
+
+    ts = []
+    for _ in range(reps):
+        sim.call("set_zero_state")
+        sim.flush()
+        sim.norm()
+        t0 = time.perf_counter()
+        sim.run(circ)
+        sim.flush()
+        sim.norm()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+@pytest.mark.gpu
+def test_learned_planner_measured_gain(gpu):
+    # A cost model trained on the GPU (random launches at 26-30 qubits, and the launches of
+    # every candidate stage plan on training circuits at 26 and 28 qubits, other circuits
+    # than these) chooses per flush among the stage plans: tile bits 11-14, 4- or 5-qubit
+    # blocks, grown or window tiles.  On a QAOA ring and a deep circuit on the top 10
+    # qubits the default plan (2^13 tiles, 5-qubit blocks) is not the fastest; the learned
+    # choice must be measurably faster on at least one of them (median of 5 runs) and
+    # within noise of the default on both, with the same output state.
+    n = 30
+    model, info = pyqcs.train_cost_model(gpu, 3, 5, (26, 28, 30), 2000, (26, 28))
+    print(f"cost model: {info}", flush=True)
+    work = {"qaoa": circuits.qaoa_ring(n, 3, 7), "narrow": circuits.narrow_random(n, 10, 300, 8)}
+    ratios = {}
+    try:
+        with pyqcs.Simulator(gpu, n, seed=1, precision=3, block_qubits=5) as d, \
+                pyqcs.Simulator(gpu, n, seed=1, precision=3, block_qubits=5) as learned:
+            pyqcs.set_planner(learned, model)
+            for name, circ in work.items():
+                t_default = _median_run(d, circ)
+                t_learned = _median_run(learned, circ)
+                choice = pyqcs.planner_last_choice(learned)
+                ov, md = learned.overlap(d)
+                fid = abs(ov) ** 2 / (learned.norm() * d.norm())
+                ratios[name] = t_learned / t_default
+                print(f"{name}: default {t_default * 1e3:.1f} ms, learned {t_learned * 1e3:.1f} ms "
+                      f"(plan {choice}), fidelity between them {fid:.12f}", flush=True)
+                assert fid >= 1 - 1e-6 and md <= 1e-5
+            pyqcs.set_planner(learned, None)
+    finally:
+        model.close()
+    assert max(ratios.values()) <= 1.05
+    assert min(ratios.values()) <= 0.95
