@@ -466,7 +466,7 @@ def _timed_run(sim, circ, prep=(), reps=1):
     return float(np.median(ts))
 
 
-def planner_leg(n, depth, record_count, k=4):
+def planner_leg(n, depth, record_count, k=4, fixed_sweep=False):
     """Learned fusion planner on complex128 fused blocks (the workloads of the planner
     configuration): a cost model trained on the GPU on recorded launch timings (random
     blocks and stages at 26-32 qubits, and every candidate stage plan's launches on
@@ -529,6 +529,17 @@ def planner_leg(n, depth, record_count, k=4):
                 log(f"planner leg: {name} {mode}: {dt:.3f}s, {b1 - b0} sweeps, fidelity vs per-gate "
                     f"{fid:.15f}, max|d| {maxdiff:.2e}")
             pyqcs.set_planner(fused, None)
+            if fixed_sweep:
+                fixed = {}
+                for m in (11, 12, 13):
+                    for kb in (3, 4):
+                        for tiles in (0, 1):
+                            fused.set_plan(m, kb, tiles=tiles)
+                            fixed[f"{m}/{kb}/{'window' if tiles else 'grow'}"] = _timed_run(fused, circ, prep)
+                res["fixed_plans_s"] = fixed
+                best = min(fixed, key=fixed.get)
+                log(f"planner leg: {name}: best fixed stage plan {best} {fixed[best]:.3f}s; "
+                    + " ".join(f"{p}={t:.3f}" for p, t in fixed.items()))
             fused.set_plan(None)
             res["learned_vs_stages"] = res["stages_s"] / res["learned_s"]
             res["learned_vs_greedy"] = res["greedy_s"] / res["learned_s"]
@@ -643,7 +654,7 @@ def main():
                          "30 and 32 qubits (8:4:2:1; half as many for the precision-form model), "
                          "besides the launches of the candidate plans of the training circuits")
     ap.add_argument("--planner-fixed-sweep", action="store_true",
-                    help="also time every fixed stage plan on the precision-form planner workloads")
+                    help="also time every fixed stage plan on the planner workloads (both legs)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one write-only fill and one read-only norm pass of the "
                          "full state (known HBM bytes, for scaling PMC byte counters)")
@@ -727,7 +738,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_pair (U3 / MCX)",
+            "kernel": "k_pair<1, true> + k_pair_low<2, true> (U3 / MCX launches, HIP events over the timed steps)",
             "achieved": achieved,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
@@ -748,7 +759,8 @@ def main():
     if not args.no_batch:
         result["batch"] = batch_leg(args.batch_circuits, direct=args.batch_no_graph, tuning=args.batch_tuning)
     if not args.no_planner:
-        result["planner"] = planner_leg(args.planner_qubits, args.planner_depth, args.planner_record, args.planner_k)
+        result["planner"] = planner_leg(args.planner_qubits, args.planner_depth, args.planner_record, args.planner_k,
+                                        fixed_sweep=args.planner_fixed_sweep)
         result["planner_precision"] = precision_planner_leg(args.fused_qubits, args.planner_record,
                                                             fixed_sweep=args.planner_fixed_sweep)
     if not args.no_cpu_baseline:

@@ -122,7 +122,7 @@ int qcs_ext_set_precision(qcs_simulator *sim, int mode, int max_block_qubits);
 
 /* Fused precision modes: blocks of a flush are grouped into stages, each applied to
  * LDS-resident tiles of 2^tile_bits amplitudes in one sweep of the state (tile_bits 11, 12,
- * 13 or 14, default 13; complex128 tiles hold 2^12 and take 3-qubit FP64 blocks; used when
+ * 13 or 14, default 13; complex128 stages default to 2^12 tiles and 3-qubit FP64 blocks; used when
  * the state has at least tile_bits + 4 qubits), or launched one sweep per block (tile_bits
  * 0).  Any time; queued gates are flushed first. */
 int qcs_ext_set_stage_bits(qcs_simulator *sim, int tile_bits);
@@ -130,7 +130,7 @@ int qcs_ext_set_stage_bits(qcs_simulator *sim, int tile_bits);
 /* Fused precision modes: a fixed plan for every flush in place of the default one (stages
  * on the configured tile size with the default block size and tile choice) when no
  * planner model is attached.  tile_bits > 0: stages on tiles of 2^tile_bits amplitudes
- * (12 for complex128, 11..14 for complex64, state of at least tile_bits + 4 qubits) with
+ * (11..13 for complex128, 11..14 for complex64, state of at least tile_bits + 4 qubits) with
  * blocks of block_qubits qubits (complex128 3 or 4, complex64 4 or 5, at most the block
  * limit) and tiles 0 (grown from qubits 0..3 in gate order), 1 (the best window of
  * consecutive qubits) or 2 (whichever gives fewer stages, the default).  tile_bits 0: one

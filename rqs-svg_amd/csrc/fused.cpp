@@ -333,8 +333,12 @@ namespace qcore
         // greedy blocks)
         const int n = sim_.n_;
         const int m0 = tile_bits();
+        // complex128 stages on 2^13 tiles (one 512-thread workgroup per CU) stay out of the
+        // candidates: at 33 qubits they ran the depth-40 brick in 5.2-6.2 s against 4.7 s for
+        // 2^12 tiles, and a model trained at up to 32 qubits took them for faster; they remain
+        // available through a fixed plan (qcs_ext_set_plan)
         std::vector<int> tiles;
-        for (int m : f64_ ? std::vector<int>{12} : std::vector<int>{11, 12, 13, 14})
+        for (int m : f64_ ? std::vector<int>{11, 12} : std::vector<int>{11, 12, 13, 14})
             if (n >= m + 4)
                 tiles.push_back(m);
         std::vector<PlanOption> opts;
@@ -553,9 +557,9 @@ namespace qcore
         }
         if (bits != 0)
         {
-            const bool ok_bits = f64_ ? bits == 12 : (bits >= 11 && bits <= 14);
+            const bool ok_bits = bits >= 11 && bits <= (f64_ ? 13 : 14);
             if (!ok_bits || sim_.n_ < bits + 4)
-                throw std::runtime_error("plan: stage tile bits must be 12 (complex128) or 11..14 (complex64), with at least tile_bits + 4 qubits");
+                throw std::runtime_error("plan: stage tile bits must be 11..13 (complex128) or 11..14 (complex64), with at least tile_bits + 4 qubits");
             const auto ks = stage_block_sizes();
             if (std::find(ks.begin(), ks.end(), k) == ks.end())
                 throw std::runtime_error("plan: this arithmetic's stage kernels take other block sizes");
@@ -594,7 +598,7 @@ namespace qcore
         };
         auto unif = [&]() { return double(rnd() >> 11) * 0x1.0p-53; };
         std::vector<int> tiles;
-        for (int m : f64_ ? std::vector<int>{12} : std::vector<int>{11, 12, 13, 14})
+        for (int m : f64_ ? std::vector<int>{11, 12} : std::vector<int>{11, 12, 13, 14})
             if (n >= m + 4)
                 tiles.push_back(m);
         // 1..24 random gates (U3, or CX/CZ with probability 0.3) on the qubit set qs
@@ -1301,12 +1305,12 @@ namespace qcore
 
     PreparedStage128 prepare_stage128(const std::vector<ResolvedGate> &gates, const PlannedStage &st, int n, std::vector<double> &mats, int kq)
     {
-        constexpr int m = 12;
+        const int m = (int)st.tile.size();
         if (kq != 3 && kq != 4)
             throw std::runtime_error("stage128: block qubits must be 3 or 4");
         const int K = 1 << kq;
-        if ((int)st.tile.size() != m)
-            throw std::runtime_error("stage128: tile size mismatch");
+        if (m < 11 || m > 13)
+            throw std::runtime_error("stage128: tile bits must be 11, 12 or 13");
         std::vector<ResolvedGate> sub;
         sub.reserve(st.gates.size());
         for (int gi : st.gates)

@@ -133,7 +133,7 @@ namespace qcore
     // Launches ntasks tasks (stages of one kq, m and x3 on several states) at task_offset.
     void launch_stage_tasks(hipStream_t s, int kq, int m, bool x3, const float *mats_dev, size_t task_offset, int ntasks, uint64_t total_tiles);
 
-    // complex128 stages (tile of 2^12, FP64 blocks of kq = 3 or 4 qubits): matrices and
+    // complex128 stages (tiles of 2^11, 2^12 or 2^13, FP64 blocks of kq = 3 or 4 qubits): matrices and
     // descriptors appended to `mats` as doubles.
     struct PreparedStage128
     {
@@ -189,11 +189,11 @@ namespace qcore
         // LDS tile bits of the stage kernel (11, 12 or 13), or 0 for one launch per block.
         void set_stage_bits(int m);
         int stage_bits() const { return stage_bits_; }
-        // tile bits of the stage kernel for this state (complex128 tiles hold 2^12), 0 if
+        // tile bits of the default stage plan for this state (complex128: 2^12), 0 if
         // per-block launches are used; stages need at least 16 tiles
         int tile_bits() const
         {
-            const int m = f64_ ? 12 : stage_bits_; // complex128 tiles are 2^12
+            const int m = f64_ ? 12 : stage_bits_; // complex128 default tiles are 2^12
             return (stage_bits_ > 0 && sim_.n_ >= m + 4) ? m : 0;
         }
         bool uses_stages() const { return tile_bits() > 0; }

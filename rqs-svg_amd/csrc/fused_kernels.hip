@@ -873,32 +873,37 @@ namespace qsv
             }
         }
 
-        // complex128 stage: a tile of 2^12 amplitudes (64 KiB) in LDS, tile bits as in
-        // k_stage; a block of KQ = 3 or 4 qubits has 2^(12-KQ) columns, one or two per thread,
-        // applied in FP64
-        // with the 16x16 matrix read through uniform loads; the next tile is prefetched into
-        // registers while the blocks run.
-        constexpr int kStage128Bits = 12;
+        // complex128 stage: a tile of 2^M amplitudes (M = 11, 12, 13: 32-128 KiB) in LDS, tile
+        // bits as in k_stage; a block of KQ = 3 or 4 qubits has 2^(M-KQ) columns, one or two
+        // per thread, applied in FP64 with the 2^KQ x 2^KQ matrix read through uniform loads;
+        // the next tile is prefetched into registers while the blocks run.  Threads per
+        // workgroup: 2^(M-4) (16 prefetch registers per thread), at most one per column.
+        constexpr int kStage128Bits = 12; // the default tile
+        template <int KQ, int M>
+        constexpr int stage128_threads() { return (1 << (M - 4)) < (1 << (M - KQ)) ? (1 << (M - 4)) : (1 << (M - KQ)); }
 
-        template <int KQ>
-        __global__ __launch_bounds__(kThreads) void k_stage128(c128 *__restrict__ s, uint64_t ntiles, Subspace tsub, Stage128Op op)
+        template <int KQ, int M>
+        __global__ __launch_bounds__((stage128_threads<KQ, M>())) void k_stage128(c128 *__restrict__ s, uint64_t ntiles, Subspace tsub, Stage128Op op)
         {
             typedef double v2d __attribute__((ext_vector_type(2)));
-            constexpr int NT = 1 << kStage128Bits;
+            constexpr int kThreads = stage128_threads<KQ, M>();
+            constexpr int TB = __builtin_ctz(kThreads); // tile bits set by the thread index
+            constexpr int NT = 1 << M;
             constexpr int LOADS = NT / kThreads;
             constexpr int K = 1 << KQ;
             constexpr int PER_THREAD = (NT >> KQ) / kThreads; // columns per thread and block
+            static_assert(PER_THREAD >= 1 && LOADS >= 1, "stage128 shape");
             __shared__ __attribute__((aligned(16))) v2d tile[NT];
             const int t = threadIdx.x;
             uint64_t g_t = 0;
 #pragma unroll
-            for (int b = 0; b < 8; ++b)
+            for (int b = 0; b < TB; ++b)
                 if ((t >> b) & 1)
                     g_t += op.gw[b];
             auto tile_offset = [&](int i) {
                 uint64_t gi = 0;
 #pragma unroll
-                for (int b = 8; b < kStage128Bits; ++b)
+                for (int b = TB; b < M; ++b)
                     if (((i * kThreads) >> b) & 1)
                         gi += op.gw[b];
                 return gi;
@@ -941,7 +946,7 @@ namespace qsv
                             const int col = t + cc * kThreads;
                             int cbase = 0;
 #pragma unroll
-                            for (int j = 0; j < kStage128Bits - KQ; ++j)
+                            for (int j = 0; j < M - KQ; ++j)
                                 if ((col >> j) & 1)
                                     cbase += B.w[KQ + j];
 #pragma unroll
@@ -966,7 +971,7 @@ namespace qsv
                         const int col = t + cc * kThreads;
                         int cbase = 0;
 #pragma unroll
-                        for (int j = 0; j < kStage128Bits - KQ; ++j)
+                        for (int j = 0; j < M - KQ; ++j)
                             if ((col >> j) & 1)
                                 cbase += B.w[KQ + j];
                         int off[K];
@@ -1025,24 +1030,25 @@ namespace qsv
             }
         }
         // complex128 stage whose blocks are all diagonal: their product is one table over the
-        // 2^12 tile indices, and each amplitude is multiplied by its entry.  The state is
+        // 2^M tile indices, and each amplitude is multiplied by its entry.  The state is
         // streamed in order (no tiles): amplitude a's tile index is a's bits 0..3 (tile qubits
-        // 0..3) and its bits hi[0..7] (the other tile qubits) as bits 4..11.  Each workgroup
-        // copies the 64 KiB table into LDS once; the 8 XCDs each stream one contiguous eighth.
-        // 1024-thread workgroups, two per CU (LDS), keep as many loads in flight as the pair
-        // kernels' 256-thread ones.
+        // 0..3) and its bits hi[0..M-5] (the other tile qubits) as bits 4..M-1.  Each workgroup
+        // copies the 32-128 KiB table into LDS once; the 8 XCDs each stream one contiguous
+        // eighth.  1024-thread workgroups (two per CU with the 64 KiB table) keep as many loads
+        // in flight as the pair kernels' 256-thread ones.
         constexpr int kTablePer = 4, kTableThreads = 1024;
+        template <int M>
         __global__ __launch_bounds__(kTableThreads) void k_stage128_table(c128 *__restrict__ s, uint64_t count, Subspace tsub, const double *__restrict__ table)
         {
             typedef double v2d __attribute__((ext_vector_type(2)));
             constexpr int kThreads = kTableThreads;
-            __shared__ __attribute__((aligned(16))) v2d tab[1 << kStage128Bits];
-            for (int i = threadIdx.x; i < (1 << kStage128Bits); i += kThreads)
+            __shared__ __attribute__((aligned(16))) v2d tab[1 << M];
+            for (int i = threadIdx.x; i < (1 << M); i += kThreads)
                 tab[i] = *reinterpret_cast<const v2d *>(table + 2 * i);
             __syncthreads();
-            int hi[kStage128Bits - 4];
+            int hi[M - 4];
 #pragma unroll
-            for (int b = 0; b < kStage128Bits - 4; ++b)
+            for (int b = 0; b < M - 4; ++b)
                 hi[b] = tsub.ops[4 + b];
             const uint64_t per_block = uint64_t(kThreads) * kTablePer;
             const uint64_t nchunks = count / per_block;
@@ -1061,7 +1067,7 @@ namespace qsv
                     const uint64_t i = c * per_block + uint64_t(j) * kThreads + threadIdx.x;
                     uint32_t e = uint32_t(i & 15);
 #pragma unroll
-                    for (int b = 0; b < kStage128Bits - 4; ++b)
+                    for (int b = 0; b < M - 4; ++b)
                         e |= uint32_t((i >> hi[b]) & 1) << (4 + b);
                     const v2d ph = tab[e];
                     const v2d y{__builtin_fma(ph.x, a[j].x, -ph.y * a[j].y), __builtin_fma(ph.x, a[j].y, ph.y * a[j].x)};
@@ -1246,8 +1252,8 @@ namespace qsv
 
     void launch_stage128(hipStream_t st, c128 *state, int n, const Subspace &tsub, const Stage128Op &op)
     {
-        if (op.m != kStage128Bits)
-            throw std::runtime_error("stage128: tile bits must be 12");
+        if (op.m < 11 || op.m > 13)
+            throw std::runtime_error("stage128: tile bits must be 11, 12 or 13");
         if (op.kq != 3 && op.kq != 4)
             throw std::runtime_error("stage128: block qubits must be 3 or 4");
         if (n <= op.m || tsub.nops != op.m)
@@ -1266,21 +1272,31 @@ namespace qsv
         const uint64_t ntiles = uint64_t(1) << (n - op.m);
         if (op.table != nullptr)
         {
-            // a streaming pass in amplitude order; 2^n is a multiple of the chunk (n > 12)
+            // a streaming pass in amplitude order; 2^n is a multiple of the chunk (n > 13)
             const uint64_t count = uint64_t(1) << n;
+            if (count % (uint64_t(kTableThreads) * kTablePer) != 0)
+                throw std::runtime_error("stage128 table: state smaller than one chunk");
             const uint64_t chunks = count / (uint64_t(kTableThreads) * kTablePer);
             const uint64_t grid = chunks < 4096 ? chunks : 4096;
-            hipLaunchKernelGGL(k_stage128_table, dim3(grid), dim3(kTableThreads), 0, st, state, count, tsub, op.table);
+            if (op.m == 11)
+                hipLaunchKernelGGL(k_stage128_table<11>, dim3(grid), dim3(kTableThreads), 0, st, state, count, tsub, op.table);
+            else if (op.m == 12)
+                hipLaunchKernelGGL(k_stage128_table<12>, dim3(grid), dim3(kTableThreads), 0, st, state, count, tsub, op.table);
+            else
+                hipLaunchKernelGGL(k_stage128_table<13>, dim3(grid), dim3(kTableThreads), 0, st, state, count, tsub, op.table);
             check(hipGetLastError(), "stage128 table kernel");
             return;
         }
         if (op.nblocks < 1 || op.blocks == nullptr || op.M == nullptr)
             throw std::runtime_error("stage128: no blocks");
         const uint64_t grid = ntiles < 8192 ? ntiles : 8192;
-        if (op.kq == 4)
-            hipLaunchKernelGGL(k_stage128<4>, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+        auto go = [&](auto kern, int threads) { hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, st, state, ntiles, tsub, op); };
+        if (op.m == 11)
+            op.kq == 4 ? go(k_stage128<4, 11>, stage128_threads<4, 11>()) : go(k_stage128<3, 11>, stage128_threads<3, 11>());
+        else if (op.m == 12)
+            op.kq == 4 ? go(k_stage128<4, 12>, stage128_threads<4, 12>()) : go(k_stage128<3, 12>, stage128_threads<3, 12>());
         else
-            hipLaunchKernelGGL(k_stage128<3>, dim3(grid), dim3(kThreads), 0, st, state, ntiles, tsub, op);
+            op.kq == 4 ? go(k_stage128<4, 13>, stage128_threads<4, 13>()) : go(k_stage128<3, 13>, stage128_threads<3, 13>());
         check(hipGetLastError(), "stage128 kernel");
     }
 

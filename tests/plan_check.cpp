@@ -151,14 +151,14 @@ int main(int argc, char **argv)
             std::printf("case %d n %d m %d k %d gates %d tiles %d: %zu stages\n", c, n, m, k, ngates, tiles, plan.size());
         }
     }
-    // phase tables of all-diagonal stages
+    // phase tables of all-diagonal stages, on tiles of 2^11, 2^12 and 2^13
     for (int c = 0; c < cases; ++c)
     {
-        const int n = 16 + int(mix(seed) % 6);
-        const int m = 12;
+        const int n = 17 + int(mix(seed) % 6);
+        const int m = 11 + c % 3;
         std::vector<int> tile = {0, 1, 2, 3};
-        const int w0 = 4 + int(mix(seed) % (n - 11)); // a window of 8 qubits above qubit 3
-        for (int q = w0; q < w0 + 8; ++q)
+        const int w0 = 4 + int(mix(seed) % (n - m + 1)); // a window of m - 4 qubits above qubit 3
+        for (int q = w0; q < w0 + m - 4; ++q)
             tile.push_back(q);
         const char *names[] = {"p", "rz", "z", "s", "t"};
         std::vector<ResolvedGate> gates;
@@ -227,7 +227,7 @@ int main(int argc, char **argv)
             }
             if (e128 > 1e-12 || e64 > 1e-6)
                 fail("phase table differs from the gates' phases", c);
-            std::printf("table case %d n %d window %d gates %d kq %d: max|d| %.1e (c128) %.1e (c64)\n", c, n, w0, ng, kq, e128, e64);
+            std::printf("table case %d n %d m %d window %d gates %d kq %d: max|d| %.1e (c128) %.1e (c64)\n", c, n, m, w0, ng, kq, e128, e64);
         }
     }
     // trailing tables: dense gates, then a run of diagonal ones; the greedy blocks up to the
@@ -235,11 +235,11 @@ int main(int argc, char **argv)
     // become one table applied at the store (trail_offset, 4-float aligned for complex64)
     for (int c = 0; c < cases; ++c)
     {
-        const int n = 16 + int(mix(seed) % 6);
-        const int m = 12;
+        const int n = 17 + int(mix(seed) % 6);
+        const int m = 11 + c % 3;
         std::vector<int> tile = {0, 1, 2, 3};
-        const int w0 = 4 + int(mix(seed) % (n - 11));
-        for (int q = w0; q < w0 + 8; ++q)
+        const int w0 = 4 + int(mix(seed) % (n - m + 1));
+        for (int q = w0; q < w0 + m - 4; ++q)
             tile.push_back(q);
         const char *names[] = {"p", "rz", "z", "s", "t"};
         std::vector<ResolvedGate> gates;

@@ -315,6 +315,31 @@ def test_stages_complex128_match_oracle(gpu, oracle, k):
         assert gates == len(circ)
 
 
+@pytest.mark.parametrize("m", [11, 13])
+@pytest.mark.parametrize("k", [3, 4])
+def test_stages_complex128_tile_sizes(gpu, oracle, m, k):
+    # complex128 stages on 2^11 and 2^13 tiles (128 and 512 threads per workgroup), grown
+    # and window tiles, dense and diagonal stages (the QFT's phase ladders become tables);
+    # FP64 bars against the oracle
+    for n, seed, tiles in ((m + 4, 31, 0), (m + 5, 32, 1)):
+        circ = circuits.random_all_gates(n, 200, seed=seed, max_ctrls=3) + circuits.qft(n)
+        mapping = [int(q) for q in np.random.default_rng(seed).permutation(n)]
+        psi = random_state(n, seed)
+        with pyqcs.Simulator(gpu, n, precision=2, block_qubits=4, mapping=mapping) as f:
+            f.set_plan(m, k, tiles=tiles)
+            f.set_statevector(psi)
+            f.run(circ)
+            got = f.statevector()
+            gates, launches = f.fusion_stats()
+        with pyqcs.Simulator(oracle, n, mapping=mapping) as o:
+            o.set_statevector(psi)
+            o.run(circ)
+            want = o.statevector()
+        say(f"c128 stages m={m} k={k} n={n}: max|d| {np.max(np.abs(got - want)):.2e} gates {gates} launches {launches}")
+        assert np.max(np.abs(got - want)) <= 1e-13
+        assert gates == len(circ) and launches < len(circ)
+
+
 @pytest.mark.parametrize("k", [3, 4])
 def test_stages_complex128_diagonal(gpu, oracle, k):
     # diagonal gates only (controlled phases, rz, rzz, z, s, t, ... with positive and negative

@@ -90,3 +90,37 @@ def test_gpu_load_errors(gpu, tmp_path):
         with pytest.raises(pyqcs.QcsError):
             s.load(tmp_path / "missing.bin")
         assert abs(s.norm() - 1) < 1e-15  # state untouched by the failed loads
+
+
+def _measure_then_set(lib, n, psi):
+    # measure q2 (it joins the measured list), overwrite the state, then a gate on q4 and a
+    # measurement of q2: with the list cleared, q2 is no implicit control of the gate and
+    # is measured afresh (one engine draw) on the new state
+    with pyqcs.Simulator(lib, n, seed=11) as s:
+        s.gate("h", [2])
+        first = s.measure(2)
+        s.set_statevector(psi)
+        s.gate("h", [4])
+        second = s.measure(2)
+        return first, second, s.statevector()
+
+
+def test_oracle_set_statevector_clears_measurement(oracle):
+    n = 6
+    psi = np.full(1 << n, 2 ** (-n / 2), dtype=np.complex128)
+    first, second, out = _measure_then_set(oracle, n, psi)
+    # H on q4 of the uniform state gives the state with q4 = 0 (amplitude sqrt(2) 2^(-n/2)),
+    # then q2's measurement keeps half of it
+    want = np.where(((np.arange(1 << n) >> 4) & 1) == 0, 2 ** (-n / 2) * np.sqrt(2), 0.0)
+    want = np.where(((np.arange(1 << n) >> 2) & 1) == second, want, 0.0)
+    assert np.max(np.abs(out - want)) < 1e-15
+
+
+@pytest.mark.gpu
+def test_gpu_set_statevector_clears_measurement(gpu, oracle):
+    n = 13
+    psi = random_state(n, 21)
+    g = _measure_then_set(gpu, n, psi)
+    o = _measure_then_set(oracle, n, psi)
+    assert g[0] == o[0] and g[1] == o[1]
+    assert np.max(np.abs(g[2] - o[2])) <= 1e-13
