@@ -669,8 +669,16 @@ namespace qsv
         // next chunk are issued before the current chunk is summed, so each wave keeps one
         // chunk (16 KiB) in flight while it works.  4 waves per workgroup, 8.5 KiB of LDS
         // per wave.
-        constexpr int kFoldWaves = 4;
-        constexpr int kFoldRow = 17; // doubles per LDS row (16 + 1 of padding)
+#ifndef QSV_FOLD_WAVES
+#define QSV_FOLD_WAVES 4
+#endif
+#ifndef QSV_FOLD_CHUNK
+#define QSV_FOLD_CHUNK 16
+#endif
+        constexpr int kFoldWaves = QSV_FOLD_WAVES;
+        constexpr int kFoldChunk = QSV_FOLD_CHUNK;   // columns per chunk (8 or 16)
+        constexpr int kFoldChunks = 64 / kFoldChunk;
+        constexpr int kFoldRow = kFoldChunk + 1; // doubles per LDS row (a chunk + 1 of padding)
 
         template <bool FROM_STATE, class A>
         __global__ __launch_bounds__(64 * kFoldWaves) void k_fold64(const void *__restrict__ src, uint64_t count, double *__restrict__ out, uint64_t nout, uint64_t ngroups)
@@ -685,11 +693,11 @@ namespace qsv
             // 8-byte entries (complex64 amplitudes, level sums) go two per lane so that each
             // load moves 16 B: lane = (row, column pair); 16-byte amplitudes one per lane
             constexpr bool PAIRS = !(FROM_STATE && sizeof(A) == 16);
-            constexpr int NL = PAIRS ? 8 : 16;   // 16-byte loads per lane and chunk
-            constexpr int RUNS = PAIRS ? 8 : 4;  // rows covered by one load instruction
+            constexpr int NL = PAIRS ? kFoldChunk / 2 : kFoldChunk;    // 16-byte loads per lane and chunk
+            constexpr int RUNS = PAIRS ? 128 / kFoldChunk : 64 / kFoldChunk; // rows covered by one load instruction
             constexpr int ES = FROM_STATE ? int(sizeof(A)) : 8; // bytes per entry
-            const int run = PAIRS ? (lane >> 3) : (lane >> 4);
-            const int col = PAIRS ? 2 * (lane & 7) : (lane & 15); // the lane's first column
+            const int run = PAIRS ? (lane / (kFoldChunk / 2)) : (lane / kFoldChunk);
+            const int col = PAIRS ? 2 * (lane % (kFoldChunk / 2)) : (lane % kFoldChunk); // the lane's first column
             const char *bytes = static_cast<const char *>(src);
             // raw 16 bytes per load: one complex128, two complex64 or two level sums; entries
             // at or past `count` read as zero
@@ -697,7 +705,7 @@ namespace qsv
 #pragma unroll
                 for (int j = 0; j < NL; ++j)
                 {
-                    const uint64_t e = group * 4096 + uint64_t(RUNS * j + run) * 64 + 16 * chunk + col;
+                    const uint64_t e = group * 4096 + uint64_t(RUNS * j + run) * 64 + kFoldChunk * chunk + col;
                     const v2d *p = reinterpret_cast<const v2d *>(bytes + e * ES);
                     if (e + (PAIRS ? 1 : 0) < count)
                         r[j] = FROM_STATE ? __builtin_nontemporal_load(p) : *p;
@@ -742,16 +750,16 @@ namespace qsv
             load(0, ra);
             double acc = 0.0;
 #pragma unroll
-            for (int chunk = 0; chunk < 4; ++chunk)
+            for (int chunk = 0; chunk < kFoldChunks; ++chunk)
             {
                 v2d(&cur)[NL] = (chunk & 1) ? rb : ra;
                 v2d(&nxt)[NL] = (chunk & 1) ? ra : rb;
-                if (chunk < 3)
+                if (chunk < kFoldChunks - 1)
                     load(chunk + 1, nxt);
                 put(cur);
                 wave_sync();
 #pragma unroll
-                for (int c = 0; c < 16; ++c)
+                for (int c = 0; c < kFoldChunk; ++c)
                 {
                     const double x = tl[lane * kFoldRow + c];
                     acc = (chunk == 0 && c == 0) ? x : acc + x;
