@@ -127,6 +127,13 @@ int qcs_ext_set_precision(qcs_simulator *sim, int mode, int max_block_qubits);
  * 0).  Any time; queued gates are flushed first. */
 int qcs_ext_set_stage_bits(qcs_simulator *sim, int tile_bits);
 
+/* Fused complex64 modes: stages on 2^13-amplitude tiles of a state of 22 to 33 qubits run
+ * in the persistent streaming kernel (on = 1, the default: one workgroup per CU, the next
+ * tile and the next block matrix loaded into LDS while the blocks run) or with one
+ * workgroup per tile (on = 0).  Both give bit-identical states.  Any time; queued gates are
+ * flushed first. */
+int qcs_ext_set_stage_stream(qcs_simulator *sim, int on);
+
 /* Fused precision modes: a fixed plan for every flush in place of the default one (stages
  * on the configured tile size with the default block size and tile choice) when no
  * planner model is attached.  tile_bits > 0: stages on tiles of 2^tile_bits amplitudes

@@ -747,7 +747,7 @@ namespace qcore
                     if (L.tile_bits && f64_)
                         launch_prepared_stage128(sim_.stream_, state128_, n, L.st128, devd);
                     else if (L.tile_bits)
-                        launch_prepared_stage(sim_.stream_, L.st, devf);
+                        launch_prepared_stage(sim_.stream_, L.st, devf, stage_stream_);
                     else if (f64_)
                         launch_prepared128(sim_.stream_, state128_, n, L.b128, devd);
                     else
@@ -1286,12 +1286,14 @@ namespace qcore
         p.state = state;
     }
 
-    void launch_prepared_stage(hipStream_t s, const PreparedStage &p, const float *mats_dev)
+    void launch_prepared_stage(hipStream_t s, const PreparedStage &p, const float *mats_dev, bool stream)
     {
         if (p.task_offset == 0)
             throw std::runtime_error("stage: launched without a task");
         if (p.table_offset != SIZE_MAX)
             qsv::launch_stage_table64(s, p.state, p.n, p.tsub, mats_dev + p.table_offset);
+        else if (stream && qsv::stage_stream_supported(p.kq, p.m, p.n))
+            qsv::launch_stage_stream(s, p.kq, p.m, p.x3, mats_dev, reinterpret_cast<const qsv::StageTask *>(mats_dev + p.task_offset), uint64_t(1) << (p.n - p.m));
         else
             launch_stage_tasks(s, p.kq, p.m, p.x3, mats_dev, p.task_offset, 1, uint64_t(1) << (p.n - p.m));
     }
@@ -1444,6 +1446,12 @@ namespace qcore
         qsv::launch_stage128(s, state, n, p.tsub, p.op);
     }
 
+    void FusedEngine::set_stage_stream(bool on)
+    {
+        flush();
+        stage_stream_ = on;
+    }
+
     void FusedEngine::set_stage_bits(int m)
     {
         if (m != 0 && (m < 11 || m > 14))
@@ -1512,7 +1520,7 @@ namespace qcore
                 if (f64_)
                     launch_prepared_stage128(sim_.stream_, state128_, n, it.st128, static_cast<const double *>(rdev_));
                 else
-                    launch_prepared_stage(sim_.stream_, it.st, static_cast<const float *>(rdev_));
+                    launch_prepared_stage(sim_.stream_, it.st, static_cast<const float *>(rdev_), stage_stream_);
                 launch_mark_end();
                 sim_.prof_end();
                 blocks_applied_ += 1;

@@ -129,7 +129,9 @@ namespace qcore
     // Appends the stage's one-state task on `state` (a device address) to `mats`.
     void attach_stage(PreparedStage &p, qsv::c64 *state, std::vector<float> &mats);
     // Launches an attached stage; mats_dev holds `mats` in device memory.
-    void launch_prepared_stage(hipStream_t s, const PreparedStage &p, const float *mats_dev);
+    // stream: the persistent streaming kernel where it applies (2^13 tiles of a state of 22
+    // to 33 qubits), else one workgroup per tile; both give the same bits
+    void launch_prepared_stage(hipStream_t s, const PreparedStage &p, const float *mats_dev, bool stream);
     // Launches ntasks tasks (stages of one kq, m and x3 on several states) at task_offset.
     void launch_stage_tasks(hipStream_t s, int kq, int m, bool x3, const float *mats_dev, size_t task_offset, int ntasks, uint64_t total_tiles);
 
@@ -188,6 +190,7 @@ namespace qcore
         bool has_queued() const { return !queue_.empty(); }
         // LDS tile bits of the stage kernel (11, 12 or 13), or 0 for one launch per block.
         void set_stage_bits(int m);
+        void set_stage_stream(bool on);
         int stage_bits() const { return stage_bits_; }
         // tile bits of the default stage plan for this state (complex128: 2^12), 0 if
         // per-block launches are used; stages need at least 16 tiles
@@ -256,6 +259,7 @@ namespace qcore
         int k_;  // planner limit (qubits per block)
         int kq_; // kernel block size (4 or 5)
         int stage_bits_ = 13;
+        bool stage_stream_ = true;
         bool f64_ = false;
         bool x3_ = false;
         qsv::c64 *state_ = nullptr;

@@ -61,6 +61,13 @@ namespace qsv
 // BF16x3 stage blocks issue the next column groups' LDS gathers before the current groups'
 // MFMA work, and take two column groups per pass with 5-qubit blocks (measured on 32-qubit
 // brick stages of ~5 blocks: 27.3 -> 25.9 ms; FP32 stages gain nothing from either)
+// streaming stage kernel: parts of a tile exchange, and parts issued per block
+#ifndef QSV_STREAM_PARTS
+#define QSV_STREAM_PARTS 4
+#endif
+#ifndef QSV_STREAM_PER_BLOCK
+#define QSV_STREAM_PER_BLOCK 1
+#endif
 #ifndef QSV_STAGE_GATHER_AHEAD
 #define QSV_STAGE_GATHER_AHEAD 1
 #endif
@@ -92,7 +99,31 @@ namespace qsv
         struct BlockMatrix<KQ, false>
         {
             static constexpr int D = 2 << KQ, MT = D / 32, H = D / 2;
+            // LDS copy (streaming stage kernel): 16-B unit j * 64 + lane holds what lane
+            // `lane` loads as its j-th v4f, so each of a wave's UNITS reads is one
+            // contiguous, conflict-free 1 KiB
+            static constexpr int UNITS = MT * (H / 4);
             float a[MT][H];
+            // the 16-B unit of R (from its start) that LDS unit j * 64 + lane holds
+            static __device__ __forceinline__ int source_unit(int j, int lane)
+            {
+                const int mt = j / (H / 4), q = j % (H / 4);
+                return ((32 * mt + (lane & 31)) * D + (lane >> 5) * H) / 4 + q;
+            }
+            __device__ __forceinline__ void load_lds(const v4f *m, int lane)
+            {
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int q = 0; q < H / 4; ++q)
+                    {
+                        const v4f v = m[(mt * (H / 4) + q) * 64 + lane];
+                        a[mt][4 * q + 0] = v.x;
+                        a[mt][4 * q + 1] = v.y;
+                        a[mt][4 * q + 2] = v.z;
+                        a[mt][4 * q + 3] = v.w;
+                    }
+            }
             __device__ __forceinline__ void load(const float *R, int c, int h)
             {
 #pragma unroll
@@ -142,7 +173,26 @@ namespace qsv
         struct BlockMatrix<KQ, true>
         {
             static constexpr int D = 2 << KQ, MT = D / 32, H = D / 2, ST = H / 8;
+            // LDS copy, as in the FP32 form: the hi units, then the lo units
+            static constexpr int UNITS = 2 * MT * ST;
             bf16x8 hi[MT][ST], lo[MT][ST];
+            static __device__ __forceinline__ int source_unit(int j, int lane)
+            {
+                const int part = j / (MT * ST), mt = (j % (MT * ST)) / ST, st = j % ST;
+                return part * (D * D / 8) + ((32 * mt + (lane & 31)) * D + (lane >> 5) * H) / 8 + st;
+            }
+            __device__ __forceinline__ void load_lds(const v4f *m, int lane)
+            {
+                const bf16x8 *u = reinterpret_cast<const bf16x8 *>(m);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int st = 0; st < ST; ++st)
+                    {
+                        hi[mt][st] = u[(mt * ST + st) * 64 + lane];
+                        lo[mt][st] = u[((MT + mt) * ST + st) * 64 + lane];
+                    }
+            }
             __device__ __forceinline__ void load(const float *R, int c, int h)
             {
                 const bf16x8 *rh = reinterpret_cast<const bf16x8 *>(R);
@@ -578,9 +628,9 @@ namespace qsv
         }
 
         // One block of a stage on the tile in LDS: CWAVES waves (index `wave`) share the
-        // 2^(M-KQ) columns, 32 per MFMA column group.  The caller separates blocks with a
-        // workgroup barrier.
-        template <int KQ, int M, bool X3, int CWAVES>
+        // 2^(M-KQ) columns, 32 per MFMA column group, GCAP (0: the tuned default) groups per
+        // pass at most.  The caller separates blocks with a workgroup barrier.
+        template <int KQ, int M, bool X3, int CWAVES, int GCAP = 0>
         __device__ __forceinline__ void stage_block_with(v2f *tile, const BlockMatrix<KQ, X3> &A, const __attribute__((address_space(4))) StageBlock &B, int wave, int c, int h)
         {
             constexpr int D = 2 << KQ;
@@ -592,7 +642,7 @@ namespace qsv
             constexpr int PER_WAVE = COLS / 32 / CWAVES;
             static_assert(PER_WAVE >= 1, "tile too small");
             // column groups per pass (each adds MT accumulators and H floats per lane)
-            constexpr int GMAX = KQ == 5 ? (X3 ? QSV_STAGE_GROUPS_KQ5_X3 : QSV_STAGE_GROUPS_KQ5) : QSV_STAGE_GROUPS_KQ4;
+            constexpr int GMAX = GCAP > 0 ? GCAP : KQ == 5 ? (X3 ? QSV_STAGE_GROUPS_KQ5_X3 : QSV_STAGE_GROUPS_KQ5) : QSV_STAGE_GROUPS_KQ4;
             constexpr int G = PER_WAVE < GMAX ? PER_WAVE : GMAX;
             static_assert(PER_WAVE % G == 0, "groups per pass must divide the groups per wave");
             constexpr bool AHEAD = X3 && QSV_STAGE_GATHER_AHEAD;
@@ -870,6 +920,246 @@ namespace qsv
                 // the LDS tile is reused only when this workgroup takes another tile
                 if (g + gridDim.x < total_tiles)
                     stage_barrier<W>();
+            }
+        }
+
+        // Streaming stage kernel: a persistent workgroup per CU with two LDS tile buffers and
+        // two LDS block-matrix buffers (160 KiB), and three kinds of wave:
+        //   * 4 compute waves run the blocks of tile i on one tile buffer, exactly as
+        //     k_stage<KQ, M, X3, 4> does (same columns per wave, same products: the same
+        //     bits), reading each block's matrix from LDS;
+        //   * 2 tile waves store tile i - 1 from the other tile buffer and refill it with
+        //     tile i + 1 through direct-to-LDS loads (no registers hold the data in flight);
+        //   * 2 matrix waves copy the next block's matrix into the other matrix buffer, also
+        //     direct to LDS.
+        // A wave's vector-memory counter completes in issue order and counts stores as well as
+        // loads; giving each kind of traffic its own waves means that no wait for one kind
+        // waits for the other.  The compute waves issue no global memory operations at all.
+        // All waves pass the same barriers: one after the first tile's load and the first
+        // matrix, then one per block.
+        //
+        // Tile wave tw owns the 1-KiB chunks CH*tw .. CH*tw + CH - 1 of both tile buffers (64
+        // amplitude pairs each, CH = 2^(M-7)/2); lane l of chunk j holds LDS pair slot
+        // 64 j + l, whose tile index is swz64(128 j + 2 l).  The per-lane global offsets of
+        // those tile indices are the same for every tile, so they are computed once.
+        typedef __attribute__((address_space(3))) void lds_void;
+        typedef __attribute__((address_space(1))) void glb_void;
+        constexpr int kStreamCompute = 4, kStreamTile = 2, kStreamMatrix = 2;
+        constexpr int kStreamWaves = kStreamCompute + kStreamTile + kStreamMatrix;
+
+        template <int M>
+        struct StreamChunks
+        {
+            static constexpr int CH = (1 << M) / 128 / kStreamTile;
+            uint32_t off[CH]; // global offset of the lane's pair in pairs (the state has at most 2^33 amplitudes), per owned chunk
+            int first;        // first owned chunk
+            int lane;
+            __device__ __forceinline__ int tile_index(int k) const { return swz64(128 * (first + k) + 2 * lane); }
+            __device__ __forceinline__ void init(TaskPtr T, int tw, int l)
+            {
+                first = CH * tw;
+                lane = l;
+#pragma unroll
+                for (int k = 0; k < CH; ++k)
+                {
+                    const int e = tile_index(k);
+                    uint64_t o = 0;
+#pragma unroll
+                    for (int b = 1; b < M; ++b)
+                        o += T->gw[b] & (uint64_t(0) - uint64_t((e >> b) & 1));
+                    off[k] = uint32_t(o >> 1);
+                }
+            }
+            // owned chunks K0 .. K0 + P - 1
+            template <int K0, int P>
+            __device__ __forceinline__ void load(v2f *buf, const c64 *s, uint64_t base) const
+            {
+#pragma unroll
+                for (int k = K0; k < K0 + P; ++k)
+                    __builtin_amdgcn_global_load_lds((const glb_void *)(s + (base + 2 * uint64_t(off[k]))),
+                                                     (lds_void *)(buf + 128 * (first + k)), 16, 0, 0);
+            }
+            // owned chunks of a finished tile from LDS into v, times the trailing table
+            template <int K0, int P>
+            __device__ __forceinline__ void read(const v2f *buf, TaskPtr T, const float *__restrict__ mats, v4f (&v)[P]) const
+            {
+#pragma unroll
+                for (int k = 0; k < P; ++k)
+                    v[k] = *reinterpret_cast<const v4f *>(buf + 128 * (first + K0 + k) + 2 * lane);
+                if (T->flags & kStageTrailTable) // uniform: the diagonal blocks after the others
+                {
+                    const v4f *tab = reinterpret_cast<const v4f *>(mats + T->trail);
+#pragma unroll
+                    for (int k = 0; k < P; ++k)
+                    {
+                        const v4f p = tab[tile_index(K0 + k) >> 1];
+                        const v4f x = v[k];
+                        v[k] = v4f{__builtin_fmaf(p.x, x.x, -p.y * x.y), __builtin_fmaf(p.x, x.y, p.y * x.x),
+                                   __builtin_fmaf(p.z, x.z, -p.w * x.w), __builtin_fmaf(p.z, x.w, p.w * x.z)};
+                    }
+                }
+            }
+            template <int K0, int P>
+            __device__ __forceinline__ void write(c64 *s, uint64_t base, const v4f (&v)[P]) const
+            {
+#pragma unroll
+                for (int k = 0; k < P; ++k)
+                    __builtin_nontemporal_store(v[k], reinterpret_cast<v4f *>(s + (base + 2 * uint64_t(off[K0 + k]))));
+            }
+            // Tile `out` (if any) from buf to HBM and tile `in` (if any) from HBM into buf, in
+            // PARTS parts: each part's LDS reads, then its loads, then its stores.  Waiting
+            // for the loads (issue order) then waits for all but the last part's stores.
+            // The parts are spread over the blocks of the tile (PER_BLOCK parts per block, the
+            // rest with the last block), so that no barrier waits for the whole exchange.
+            static constexpr int PARTS = QSV_STREAM_PARTS, PART = CH / PARTS, PER_BLOCK = QSV_STREAM_PER_BLOCK;
+            static_assert(CH % PARTS == 0, "parts must divide the chunks");
+            // the parts due at block bi of nblocks
+            template <int Q = 0>
+            __device__ __forceinline__ void exchange_at(int bi, int nblocks, v2f *buf, c64 *s, TaskPtr T, const float *__restrict__ mats, bool has_out, uint64_t out, bool has_in, uint64_t in) const
+            {
+                if constexpr (Q < PARTS)
+                {
+                    const int due = Q / PER_BLOCK < nblocks - 1 ? Q / PER_BLOCK : nblocks - 1;
+                    if (due == bi)
+                        part<Q * PART>(buf, s, T, mats, has_out, out, has_in, in);
+                    exchange_at<Q + 1>(bi, nblocks, buf, s, T, mats, has_out, out, has_in, in);
+                }
+            }
+            template <int K0>
+            __device__ __forceinline__ void part(v2f *buf, c64 *s, TaskPtr T, const float *__restrict__ mats, bool has_out, uint64_t out, bool has_in, uint64_t in) const
+            {
+                v4f v[PART];
+                if (has_out)
+                    read<K0, PART>(buf, T, mats, v);
+                if (has_in)
+                {
+                    // the LDS reads have returned (the table products, when there are any,
+                    // waited for the table loads); nothing may move the loads above them.
+                    // lgkmcnt(0) only: a vector-memory wait here would wait for the earlier
+                    // parts' stores.
+                    __builtin_amdgcn_s_waitcnt(0xc07f);
+                    __builtin_amdgcn_sched_barrier(0);
+                    load<K0, PART>(buf, s, in);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (has_out)
+                    write<K0, PART>(s, out, v);
+            }
+        };
+
+        // The barrier of the tile and matrix waves.  lds_barrier's release fence would make
+        // the wave wait for its direct-to-LDS loads at every barrier (they write LDS); these
+        // waves wait for them explicitly where needed, and their own LDS reads complete
+        // before the stores that use them.  The acquire fence keeps the LDS reads of the next
+        // tile's stores after the barrier.
+        __device__ __forceinline__ void stream_barrier()
+        {
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        }
+
+        // s_waitcnt immediate that waits until at most n vector-memory operations are
+        // outstanding (lgkmcnt and expcnt at their maxima: no wait on them)
+        constexpr int vm_wait(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
+
+        // tile number -> its first amplitude (single task, tile_begin 0)
+        __device__ __forceinline__ uint64_t stream_base(TaskPtr T, uint64_t g) { return expand_task(g, T); }
+
+        template <int KQ, int M, bool X3>
+        __global__ __launch_bounds__(64 * kStreamWaves) void k_stage_stream(const float *__restrict__ mats, const StageTask *__restrict__ task, uint64_t total_tiles)
+        {
+            typedef BlockMatrix<KQ, X3> Mat;
+            __shared__ __attribute__((aligned(16))) v2f buf0[1 << M];
+            __shared__ __attribute__((aligned(16))) v2f buf1[1 << M];
+            __shared__ __attribute__((aligned(16))) v4f mat0[Mat::UNITS * 64];
+            __shared__ __attribute__((aligned(16))) v4f mat1[Mat::UNITS * 64];
+            const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+            const int lane = threadIdx.x & 63;
+            const TaskPtr T = as_constant(task);
+            const int nblocks = T->nblocks;
+            const __attribute__((address_space(4))) StageBlock *blocks = as_constant(reinterpret_cast<const StageBlock *>(mats + T->blocks));
+            const uint64_t stride = gridDim.x;
+            // this workgroup's tiles: blockIdx.x + i * gridDim.x, i < count (gridDim.x <= total_tiles)
+            const uint64_t count = (total_tiles - blockIdx.x + stride - 1) / stride;
+            // steps: tile k / nblocks, block k % nblocks; step k reads matrix buffer k & 1
+            const uint64_t steps = count * uint64_t(nblocks);
+            if (wave >= kStreamCompute + kStreamTile)
+            {
+                // matrix waves: wave mw copies units j = mw, mw + 2, ... of each matrix
+                const int mw = wave - kStreamCompute - kStreamTile;
+                auto copy = [&](v4f *dst, int bi) {
+                    const char *src = reinterpret_cast<const char *>(mats + blocks[bi].matrix);
+#pragma unroll
+                    for (int j = 0; j < Mat::UNITS; j += kStreamMatrix)
+                        __builtin_amdgcn_global_load_lds((const glb_void *)(src + 16 * Mat::source_unit(j + mw, lane)),
+                                                         (lds_void *)(dst + 64 * (j + mw)), 16, 0, 0);
+                };
+                copy(mat0, 0);
+                __builtin_amdgcn_s_waitcnt(0);
+                stream_barrier();
+                int bi = 0;
+                for (uint64_t k = 0; k < steps; ++k)
+                {
+                    bi = bi + 1 == nblocks ? 0 : bi + 1;
+                    // the buffer the compute waves read in step k - 1 (finished before the
+                    // barrier that began step k)
+                    if (k + 1 < steps)
+                        copy((k & 1) ? mat0 : mat1, bi);
+                    __builtin_amdgcn_s_waitcnt(0);
+                    stream_barrier();
+                }
+                return;
+            }
+            if (wave >= kStreamCompute)
+            {
+                StreamChunks<M> io;
+                io.init(T, wave - kStreamCompute, lane);
+                c64 *__restrict__ s = T->state;
+                io.template load<0, StreamChunks<M>::CH>(buf0, s, stream_base(T, blockIdx.x));
+                __builtin_amdgcn_s_waitcnt(0);
+                stream_barrier();
+                for (uint64_t i = 0; i < count; ++i)
+                {
+                    // the other buffer: tile i - 1 out, tile i + 1 in, part by part during the
+                    // blocks of tile i (the stores of each part after its loads, so that they
+                    // drain during the next tile)
+                    v2f *other = (i & 1) ? buf0 : buf1;
+                    const bool has_out = i > 0, has_in = i + 1 < count;
+                    const uint64_t out = has_out ? stream_base(T, blockIdx.x + (i - 1) * stride) : 0;
+                    const uint64_t in = has_in ? stream_base(T, blockIdx.x + (i + 1) * stride) : 0;
+                    for (int bi = 0; bi < nblocks; ++bi)
+                    {
+                        // opaque per block: keeps the compiler from computing every part's
+                        // addresses ahead of the loop (they would not fit in registers)
+                        uint64_t out_b = out, in_b = in;
+                        asm volatile("" : "+s"(out_b), "+s"(in_b));
+                        io.exchange_at(bi, nblocks, other, s, T, mats, has_out, out_b, has_in, in_b);
+                        if (bi == nblocks - 1) // tile i + 1 in LDS; the last part's stores may be in flight
+                        {
+                            if (i > 0)
+                                __builtin_amdgcn_s_waitcnt(vm_wait(StreamChunks<M>::PART));
+                            else
+                                __builtin_amdgcn_s_waitcnt(0);
+                        }
+                        stream_barrier();
+                    }
+                }
+                io.exchange_at(0, 1, (count & 1) ? buf0 : buf1, s, T, mats, true, stream_base(T, blockIdx.x + (count - 1) * stride), false, 0);
+                return;
+            }
+            const int c = lane & 31;
+            const int h = lane >> 5;
+            lds_barrier();
+            uint64_t i = 0;
+            int bi = 0;
+            for (uint64_t k = 0; k < steps; ++k)
+            {
+                Mat A;
+                A.load_lds((k & 1) ? mat1 : mat0, lane);
+                stage_block_with<KQ, M, X3, kStreamCompute, 1>((i & 1) ? buf1 : buf0, A, blocks[bi], wave, c, h);
+                lds_barrier();
+                bi = bi + 1 == nblocks ? 0 : bi + 1;
+                i += bi == 0;
             }
         }
 
@@ -1247,6 +1537,27 @@ namespace qsv
         else
             x3 ? go(k_stage<4, 11, true, X11>, X11) : go(k_stage<4, 11, false, W11>, W11);
         check(hipGetLastError(), "stage kernel");
+    }
+
+    bool stage_stream_supported(int kq, int m, int n) { return (kq == 4 || kq == 5) && m == 13 && n >= m + 9 && n <= 33; }
+
+    void launch_stage_stream(hipStream_t st, int kq, int m, bool x3, const float *base, const StageTask *task, uint64_t total_tiles)
+    {
+        if (base == nullptr || task == nullptr || total_tiles < 1 || (total_tiles & (total_tiles - 1)))
+            throw std::runtime_error("stage stream: no task, or a tile count that is not a power of two");
+        if (!stage_stream_supported(kq, m, m + 63 - __builtin_clzll(total_tiles)))
+            throw std::runtime_error("stage stream: kq must be 4 or 5, the tile 2^13 amplitudes and the state 22 to 33 qubits");
+        int dev = 0, cus = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        // one resident workgroup per CU (2 x 64 KiB of LDS); every workgroup gets a tile
+        const uint64_t grid = std::min<uint64_t>(total_tiles, uint64_t(std::max(cus, 1)));
+        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kStreamWaves), 0, st, base, task, total_tiles); };
+        if (kq == 5)
+            x3 ? go(k_stage_stream<5, 13, true>) : go(k_stage_stream<5, 13, false>);
+        else
+            x3 ? go(k_stage_stream<4, 13, true>) : go(k_stage_stream<4, 13, false>);
+        check(hipGetLastError(), "stage stream kernel");
     }
 
 

@@ -70,6 +70,11 @@ namespace qsv
     // tasks: device array of ntasks tasks with ascending tile_begin, the first 0, all checked
     // with check_stage_task; total_tiles = the tiles of all tasks.
     void launch_stage(hipStream_t s, int kq, int m, bool x3, const float *base, const StageTask *tasks, int ntasks, uint64_t total_tiles);
+    // The same stage on one task by a persistent kernel (one workgroup per CU, two tile
+    // buffers, the next tile loaded while the blocks run).  Same products as launch_stage,
+    // so the same bits.  The task must not carry kStageZeroState.
+    bool stage_stream_supported(int kq, int m, int n);
+    void launch_stage_stream(hipStream_t s, int kq, int m, bool x3, const float *base, const StageTask *task, uint64_t total_tiles);
     // complex64 stage whose blocks are all diagonal: every amplitude times table[tile index],
     // the tile index made of the amplitude's bits at tsub.ops[0..m-1] (m = tsub.nops, 11..14,
     // ops[0..3] = 0..3); table holds 2^m complex floats in device memory; 2^n >= 2^13.

@@ -68,6 +68,7 @@ namespace qcore
         void permute(const std::vector<int> &want_p2l);
         void set_precision(int mode, int k);
         void set_stage_bits(int m);
+        void set_stage_stream(int on);
         void flush();
         void overlap(Simulator &other, double out[3]);
         void set_planner_model(CostModel *m);
@@ -124,6 +125,7 @@ namespace qcore
         int n_local_ = 0;
         int precision_ = 0;
         int stage_bits_ = 13;
+        int stage_stream_ = 1;
         bool c64_state() const { return precision_ == 1 || precision_ == 3; }
         int block_qubits_ = 5;
         qsv::c128 *state_ = nullptr; // complex128 state (precision 0)

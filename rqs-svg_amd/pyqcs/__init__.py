@@ -126,6 +126,7 @@ class Library:
         lib.qcs_ext_get_mapping.argtypes = [vp, P(bit_num_t), bit_num_t]
         lib.qcs_ext_permute.argtypes = [vp, P(bit_num_t), bit_num_t]
         lib.qcs_ext_set_stage_bits.argtypes = [vp, ctypes.c_int]
+        lib.qcs_ext_set_stage_stream.argtypes = [vp, ctypes.c_int]
         lib.qcs_ext_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.qcs_ext_flush.argtypes = [vp]
         lib.qcs_ext_set_plan.argtypes = [vp] + [ctypes.c_int] * 6
@@ -348,6 +349,11 @@ class Simulator:
 
     def set_stage_bits(self, m):
         self._call(self.lib.qcs_ext_set_stage_bits(self.h, m), "set_stage_bits")
+
+    def set_stage_stream(self, on):
+        """Stages on 2^13 tiles of 22-33 qubit states: streaming kernel (True) or one
+        workgroup per tile (False); the same bits either way."""
+        self._call(self.lib.qcs_ext_set_stage_stream(self.h, int(bool(on))), "set_stage_stream")
 
     def flush(self):
         self._call(self.lib.qcs_ext_flush(self.h), "flush")

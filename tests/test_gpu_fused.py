@@ -410,3 +410,49 @@ def test_stage_launch_chunks(gpu, oracle, precision):
     say(f"chunks n={n} precision={precision}: fidelity {fidelity(got, want):.12f} launches {launches}")
     assert fidelity(got, want) >= 1 - FID_TOL and np.max(np.abs(got - want)) <= AMP_TOL
     assert gates == len(circ)
+
+
+@pytest.mark.parametrize("precision", [1, 3])
+@pytest.mark.parametrize("k", [4, 5])
+def test_stage_stream_matches_tile_kernel(gpu, oracle, precision, k):
+    # Stages on 2^13 tiles of 22-33 qubit states run by default in the streaming kernel (one
+    # workgroup per CU; the next tile and the next block matrix are loaded into LDS while
+    # the blocks run).  With one workgroup per tile instead, every state must have the same
+    # bits: brick circuits (dense blocks; 2 and 4 tiles per workgroup at 22 and 23 qubits),
+    # the QFT on a prepared state (trailing diagonal tables applied as the tiles are
+    # stored), and a short random circuit (stages of one or two blocks).  The 22-qubit
+    # brick is also held to the FP64 oracle.
+    work = [
+        (22, "brick", circuits.brick(22, 6, seed=7), None),
+        (23, "brick", circuits.brick(23, 4, seed=9, mcx=2), None),
+        (23, "qft", circuits.qft(23), circuits.brick(23, 1, seed=8)),
+        (22, "short", circuits.random_all_gates(22, 12, seed=3, max_ctrls=1), None),
+    ]
+    for n, name, circ, prep in work:
+        outs = []
+        for stream in (False, True):
+            with pyqcs.Simulator(gpu, n, precision=precision, block_qubits=k, stage_bits=13) as f:
+                f.set_stage_stream(stream)
+                if prep:
+                    f.run(prep)
+                f.run(circ)
+                outs.append(f.statevector())
+        same = np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+        say(f"stream n={n} {name} precision={precision} k={k}: identical bits {same}")
+        assert same
+        if name == "brick" and n == 22:
+            with pyqcs.Simulator(oracle, n) as o:
+                o.run(circ)
+                want = o.statevector()
+            fid, raw, err = fidelity(outs[1], want), overlap_raw(outs[1], want), np.max(np.abs(outs[1] - want))
+            say(f"  vs oracle: fidelity {fid:.12f} unnormalised {raw:.12f} max|d| {err:.2e}")
+            assert fid >= 1 - FID_TOL and raw >= 1 - FID_TOL and err <= AMP_TOL
+
+
+def test_stage_stream_validation(gpu, oracle):
+    for lib in (gpu, oracle):
+        with pyqcs.Simulator(lib, 8) as sim:
+            for bad in (2, -1):
+                assert sim.lib.qcs_ext_set_stage_stream(sim.h, bad) == 0
+            for good in (False, True):
+                sim.set_stage_stream(good)
