@@ -178,6 +178,26 @@ def fused_leg(n, depth, k, shots, variants=FUSED_VARIANTS):
         norm64, norm128 = fused.norm(), plain.norm()
         fid = abs(ov) ** 2 / (norm64 * norm128)
         raw = abs(ov) ** 2 / norm128  # unnormalised: a norm drift of the complex64 state counts
+        stream = stage_bits == 13 and 22 <= n <= 33
+        if stream:
+            # the same flushes with one workgroup per tile in place of the streaming kernel
+            # (bit-identical states; tests/test_gpu_fused.py checks the bits)
+            fused.set_stage_stream(False)
+            fused.call("set_zero_state")
+            fused.norm()
+            fused.profile_reset()
+            fused.profile(True)
+            t0 = time.perf_counter()
+            fused.run(circ)
+            fused.probability_total()
+            t_tile = time.perf_counter() - t0
+            fused.profile(False)
+            tb = fused.profile_get().get("fused", {"launches": 1, "seconds": 1.0})
+            waves = 2 if mode == 3 else 4  # waves per workgroup of k_stage on 2^13 tiles
+            res["tile_per_workgroup"] = {"kernel": f"k_stage<{k}, {stage_bits}, {'true' if mode == 3 else 'false'}, {waves}>",
+                                         "fused_c64_s": t_tile, "avg_ms": tb["seconds"] / tb["launches"] * 1e3,
+                                         "streaming_speedup": t_tile / t_fused}
+            log(f"fused leg: {key} with one workgroup per tile {t_tile:.3f}s ({t_tile / t_fused:.3f}x the streaming kernel's time)")
         fused.close()
         log(f"fused leg: {key} {t_fused:.3f}s ({b1 - b0} sweeps), fidelity {fid:.12f}, unnormalised "
             f"{raw:.12f}, |norm-1| {abs(norm64 - 1):.2e}, max|d| {maxdiff:.2e}")
@@ -190,7 +210,8 @@ def fused_leg(n, depth, k, shots, variants=FUSED_VARIANTS):
         # per launch (2 * 8 B * 2^n for a stage); each block costs 8 * 2^k flops per amplitude
         gbs = blk["bytes"] / blk["seconds"] / 1e9
         x3 = "true" if mode == 3 else "false"
-        kern = f"k_stage<{k}, {stage_bits}, {x3}>" if stage_bits else f"k_block_lds<{k}, {x3}>"
+        kern = (f"k_stage_stream<{k}, {stage_bits}, {x3}>" if stream else f"k_stage<{k}, {stage_bits}, {x3}>") if stage_bits \
+            else f"k_block_lds<{k}, {x3}>"
         res["roofline"] = {"bound": "hbm", "kernel": kern,
                            "achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                            "frac": gbs / (HBM_PEAK / 1e9), "launches": blk["launches"],
