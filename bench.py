@@ -597,7 +597,9 @@ def precision_planner_leg(n, record_count, fixed_sweep=False):
     blocks, fewer-stage tile choice) and with the model's choice (median of 3 runs each).
     With fixed_sweep, every fixed stage plan is timed as well."""
     lib = pyqcs.gpu_library()
-    model, info = pyqcs.train_cost_model(lib, 3, 5, (26, 28, 30, 32), record_count // 2, (26, 28, 30), log=log)
+    # candidate-plan timings at 30 and 32 qubits (one seed): the launches of the sizes the
+    # model chooses for, at the recording cost of three seeds at 26-30 qubits
+    model, info = pyqcs.train_cost_model(lib, 3, 5, (26, 28, 30, 32), record_count // 2, (30, 32), plan_seeds=(1001,), log=log)
     out = {"config": f"complex64 BF16x3 stages, {n} qubits; workloads {', '.join(PRECISION_PLANNER_WORKLOADS)}",
            "cost_model": info}
     sim = pyqcs.Simulator(lib, n, seed=1, precision=3, block_qubits=5)

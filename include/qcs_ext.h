@@ -204,9 +204,11 @@ int qcs_ext_batch_run_direct(qcs_ext_batch *batch, double *expz, uint64_t count,
  * libqcs.so trains and evaluates it with its own GPU kernels; the oracle library computes
  * the same functions on the CPU.  With a model attached (precision mode 1 or 2), every
  * flush predicts the time of candidate fusion plans (features per block, see
- * qcs_ext_planner_record) and takes the cheapest unless it does not beat the greedy plan
- * by 3 %; without a model the greedy plan is used.  The caller keeps the model alive while
- * it is attached. */
+ * qcs_ext_planner_record) and takes the cheapest only if its predicted time beats the
+ * default plan's by the model's own typical error: a factor 2^-rms, rms = the root mean
+ * squared error (log2 microseconds) of the model's last qcs_ext_mlp_train on its data,
+ * within [0.75, 0.97] (0.97 for parameters set by qcs_ext_mlp_set_params).  Without a model
+ * the default plan is used.  The caller keeps the model alive while it is attached. */
 typedef struct qcs_ext_mlp qcs_ext_mlp;
 int qcs_ext_mlp_create(qcs_ext_mlp **out, int num_features, uint64_t seed);
 int qcs_ext_mlp_destroy(qcs_ext_mlp *model);

@@ -460,7 +460,7 @@ namespace qcore
                 best = oi;
         stat_flushes_ += 1;
         stat_pred_greedy_ += cost[0];
-        if (best != 0 && std::isfinite(cost[0]) && cost[best] < 0.97 * cost[0])
+        if (best != 0 && std::isfinite(cost[0]) && cost[best] < model_->choice_factor() * cost[0])
         {
             stat_learned_ += 1;
             stat_pred_chosen_ += cost[best];

@@ -176,6 +176,14 @@ namespace qcore
     {
         qsv::check(hipMemcpy(params_, in, P_ * sizeof(float), hipMemcpyHostToDevice), "mlp params");
         step_ = 0;
+        fit_mse_ = -1.f;
+    }
+
+    double CostModel::choice_factor() const
+    {
+        if (!(fit_mse_ > 0.f))
+            return 0.97;
+        return std::min(0.97, std::max(0.75, std::exp2(-std::sqrt(double(fit_mse_)))));
     }
 
     void CostModel::predict(const float *X, int B, float *Y)
@@ -251,7 +259,8 @@ namespace qcore
                 loss += double(y[i] - ts[i]) * double(y[i] - ts[i]);
             (void)hipFree(dX);
             (void)hipFree(dT);
-            return float(loss / N);
+            fit_mse_ = float(loss / N);
+            return fit_mse_;
         }
         catch (...)
         {

@@ -39,6 +39,12 @@ namespace qcore
         float gradients(const float *X, const float *T, int B, float *G);
         // Adam on shuffled mini-batches; returns the final mean squared error on all data
         float train(const float *X, const float *T, int N, int epochs, int batch, float lr, uint64_t seed);
+        // The factor a candidate plan's predicted time must beat the default plan's by to be
+        // chosen: 2^-rms, rms = the root mean squared error (log2 microseconds) of the last
+        // training on its own data -- a plan is taken only when its predicted gain exceeds
+        // the model's typical error -- kept within [0.75, 0.97]; 0.97 when the parameters
+        // were set directly (no fit known).
+        double choice_factor() const;
 
     private:
         void reserve(int B);
@@ -49,6 +55,7 @@ namespace qcore
         float *dH1_ = nullptr, *dH2_ = nullptr, *dY_ = nullptr;
         int cap_ = 0;
         int step_ = 0;
+        float fit_mse_ = -1.f; // of the last train(); -1 after set_params
     };
 
     std::vector<PlannedBlock> plan_with(const std::vector<ResolvedGate> &gates, const PlanOption &opt);
