@@ -1376,12 +1376,25 @@ namespace qsv
             for (int i = threadIdx.x; i < (1 << M); i += kTable64Threads)
                 tab[i] = *reinterpret_cast<const v2f *>(table + 2 * i);
             __syncthreads();
-            int hi[M - 4];
-#pragma unroll
-            for (int b = 0; b < M - 4; ++b)
-                hi[b] = tsub.ops[4 + b];
-            const uint64_t per_block = uint64_t(kTable64Threads) * kTable64Per * 2;
+            constexpr int kChunkBits = __builtin_ctz(kTable64Threads * kTable64Per * 2);
+            const uint64_t per_block = uint64_t(1) << kChunkBits;
             const uint64_t nchunks = count / per_block;
+            // An amplitude's tile index takes its bits 0..3 and its bits at the other tile
+            // qubits; the bits below kChunkBits come from the thread's offset in a chunk (the
+            // same in every chunk), the others from the chunk's base (uniform).  Both parts
+            // are computed once: per thread here, per chunk below.
+            uint32_t eoff[kTable64Per];
+#pragma unroll
+            for (int j = 0; j < kTable64Per; ++j)
+            {
+                const uint32_t o = 2 * (uint32_t(j) * kTable64Threads + threadIdx.x);
+                uint32_t e = o & 15;
+#pragma unroll
+                for (int b = 0; b < M - 4; ++b)
+                    if (tsub.ops[4 + b] < kChunkBits)
+                        e |= ((o >> tsub.ops[4 + b]) & 1) << (4 + b);
+                eoff[j] = e;
+            }
             uint64_t nb = gridDim.x, b0 = blockIdx.x;
             if ((nb & 7) == 0)
                 b0 = (b0 & 7) * (nb >> 3) + (b0 >> 3);
@@ -1391,15 +1404,21 @@ namespace qsv
 #pragma unroll
                 for (int j = 0; j < kTable64Per; ++j)
                     a[j] = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + (c * per_block + 2 * (uint64_t(j) * kTable64Threads + threadIdx.x))));
+                const uint64_t cbase = c * per_block;
+                uint32_t ebase = 0;
+#pragma unroll
+                for (int b = 0; b < M - 4; ++b)
+                    if (tsub.ops[4 + b] >= kChunkBits)
+                        ebase |= uint32_t((cbase >> tsub.ops[4 + b]) & 1) << (4 + b);
 #pragma unroll
                 for (int j = 0; j < kTable64Per; ++j)
                 {
-                    const uint64_t i = c * per_block + 2 * (uint64_t(j) * kTable64Threads + threadIdx.x);
-                    uint32_t e = uint32_t(i & 15);
-#pragma unroll
-                    for (int b = 0; b < M - 4; ++b)
-                        e |= uint32_t((i >> hi[b]) & 1) << (4 + b);
-                    const v2f p0 = tab[e], p1 = tab[e | 1];
+                    const uint64_t i = cbase + 2 * (uint64_t(j) * kTable64Threads + threadIdx.x);
+                    // e is even: entries e and e + 1 in one 16-B read (a wave's reads of
+                    // consecutive pairs are then one contiguous, conflict-free KiB)
+                    const uint32_t e = ebase | eoff[j];
+                    const v4f p = *reinterpret_cast<const v4f *>(&tab[e]);
+                    const v2f p0{p.x, p.y}, p1{p.z, p.w};
                     const v4f y{__builtin_fmaf(p0.x, a[j].x, -p0.y * a[j].y), __builtin_fmaf(p0.x, a[j].y, p0.y * a[j].x),
                                 __builtin_fmaf(p1.x, a[j].z, -p1.y * a[j].w), __builtin_fmaf(p1.x, a[j].w, p1.y * a[j].z)};
                     __builtin_nontemporal_store(y, reinterpret_cast<v4f *>(s + i));
