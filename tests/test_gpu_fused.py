@@ -456,3 +456,27 @@ def test_stage_stream_validation(gpu, oracle):
                 assert sim.lib.qcs_ext_set_stage_stream(sim.h, bad) == 0
             for good in (False, True):
                 sim.set_stage_stream(good)
+
+
+def test_stage_stream_33q_edge(gpu):
+    # The streaming kernel keeps each lane's offsets within a tile as 32-bit indices of
+    # amplitude pairs, exact up to 2^33 amplitudes: at 33 qubits (two 64 GiB complex64
+    # states) a brick circuit reaching the top qubit must match the per-tile kernel exactly
+    # (max |d amp| 0 over the whole state, compared on the GPU).
+    n = 33
+    circ = circuits.brick(n, 2, seed=17)
+    sims = []
+    try:
+        for stream in (False, True):
+            f = pyqcs.Simulator(gpu, n, seed=1, precision=3, block_qubits=5, stage_bits=13)
+            sims.append(f)
+            f.set_stage_stream(stream)
+            f.run(circ)
+            f.flush()
+        ov, md = sims[1].overlap(sims[0])
+        norm = sims[1].norm()
+        say(f"stream 33 q: <tile|stream> {ov.real:.12f}{ov.imag:+.2e}i, max|d amp| {md:.3e}, norm {norm:.12f}")
+        assert md == 0.0 and abs(ov - norm) <= 1e-9
+    finally:
+        for f in sims:
+            f.close()
