@@ -5,6 +5,7 @@
 #include "fused.hpp"
 #include "planner.hpp"
 
+#include <atomic>
 #include <exception>
 #include <mutex>
 #include <thread>
@@ -349,7 +350,7 @@ namespace qcore
         return opts;
     }
 
-    void FusedEngine::option_features(const std::vector<ResolvedGate> &gates, const PlanOption &o, std::vector<float> &X) const
+    void FusedEngine::option_features(const std::vector<ResolvedGate> &gates, const PlanOption &o, std::vector<float> &X, PlannedOption *keep) const
     {
         // one row per launch, in the order apply_option issues them
         const int n = sim_.n_;
@@ -362,11 +363,15 @@ namespace qcore
         };
         if (o.stage_bits == 0)
         {
-            for (const auto &b : plan_with(gates, o))
+            auto blocks = plan_with(gates, o);
+            for (const auto &b : blocks)
                 add_block(gates, b);
+            if (keep)
+                keep->blocks = std::move(blocks);
             return;
         }
-        for (const auto &st : plan_stages(gates, n, o.stage_bits, o.k, o.tiles))
+        auto stages = plan_stages(gates, n, o.stage_bits, o.k, o.tiles);
+        for (const auto &st : stages)
         {
             std::vector<ResolvedGate> sub;
             for (int gi : st.gates)
@@ -382,6 +387,8 @@ namespace qcore
             stage_features(n, math, o.k, st.tile, nb, (int)st.gates.size(), f);
             X.insert(X.end(), f, f + kPlanFeatures);
         }
+        if (keep)
+            keep->stages = std::move(stages);
     }
 
     void FusedEngine::apply_option(const std::vector<ResolvedGate> &gates, const PlanOption &o)
@@ -414,20 +421,24 @@ namespace qcore
         // Learned planner: predicted cost of each candidate plan = sum over its launches of
         // 2^prediction (the model predicts log2 microseconds).  Candidate 0 is the plan the
         // engine runs without a model, and it judges the others: it is kept unless a
-        // candidate is predicted 3 % faster.  The candidates are planned on host threads.
+        // candidate's predicted gain exceeds the model's own error (choice_factor).  The
+        // candidates are planned on up to 16 host threads: a circuit that runs as one flush
+        // waits for its planning with the GPU idle.
         const auto opts = plan_options();
         std::vector<std::vector<float>> rows(opts.size());
+        std::vector<PlannedOption> planned(opts.size());
         {
-            const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+            const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
             std::vector<std::thread> pool;
             std::exception_ptr err;
             std::mutex err_mu;
+            std::atomic<size_t> next{0}; // candidates to threads as they free up (window-tile plans cost most)
             for (unsigned t = 0; t < hw; ++t)
-                pool.emplace_back([&, t] {
+                pool.emplace_back([&] {
                     try
                     {
-                        for (size_t oi = t; oi < opts.size(); oi += hw)
-                            option_features(gates, opts[oi], rows[oi]);
+                        for (size_t oi = next++; oi < opts.size(); oi = next++)
+                            option_features(gates, opts[oi], rows[oi], &planned[oi]);
                     }
                     catch (...)
                     {
@@ -471,7 +482,11 @@ namespace qcore
             stat_pred_chosen_ += cost[0];
         }
         last_choice_ = opts[best];
-        apply_option(gates, opts[best]);
+        // the chosen plan as planned above
+        if (opts[best].stage_bits > 0)
+            apply_planned_stages(gates, planned[best].stages, opts[best].stage_bits, opts[best].k);
+        else
+            apply_blocks(gates, planned[best].blocks);
     }
 
     void FusedEngine::launch_mark_begin()
@@ -1462,8 +1477,12 @@ namespace qcore
 
     void FusedEngine::apply_stages(const std::vector<ResolvedGate> &gates, int m, int k, int tiles)
     {
+        apply_planned_stages(gates, plan_stages(gates, sim_.n_, m, k, tiles), m, k);
+    }
+
+    void FusedEngine::apply_planned_stages(const std::vector<ResolvedGate> &gates, const std::vector<PlannedStage> &stages, int m, int k)
+    {
         const int n = sim_.n_;
-        const auto stages = plan_stages(gates, n, m, k, tiles);
         // one matrix buffer for the whole flush: FP32 (complex64 modes) or FP64 entries
         std::vector<float> mats;
         std::vector<double> mats64;

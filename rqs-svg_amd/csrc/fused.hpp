@@ -243,8 +243,16 @@ namespace qcore
         void apply_plan(const std::vector<ResolvedGate> &gates);
         // candidate plans of the learned planner (index 0: the default plan)
         std::vector<PlanOption> plan_options() const;
-        // feature rows of a plan's launches, in the order apply_option issues them
-        void option_features(const std::vector<ResolvedGate> &gates, const PlanOption &o, std::vector<float> &X) const;
+        // A candidate plan as planned for its features, kept to be applied if chosen: the
+        // stages of a stage option, the blocks of a block option.
+        struct PlannedOption
+        {
+            std::vector<PlannedStage> stages;
+            std::vector<PlannedBlock> blocks;
+        };
+        // feature rows of a plan's launches, in the order apply_option issues them; the plan
+        // itself into *keep when given
+        void option_features(const std::vector<ResolvedGate> &gates, const PlanOption &o, std::vector<float> &X, PlannedOption *keep = nullptr) const;
         void apply_option(const std::vector<ResolvedGate> &gates, const PlanOption &o);
         // FP64 norm and rescale of a complex64 state (no-op for complex128)
         void renormalize();
@@ -253,6 +261,7 @@ namespace qcore
         void launch_mark_end();
         void apply_blocks(const std::vector<ResolvedGate> &gates, const std::vector<PlannedBlock> &plan);
         void apply_stages(const std::vector<ResolvedGate> &gates, int m, int k, int tiles = kTilesAuto);
+        void apply_planned_stages(const std::vector<ResolvedGate> &gates, const std::vector<PlannedStage> &stages, int m, int k);
         void reserve_matrices(size_t bytes);
 
         Simulator &sim_;
