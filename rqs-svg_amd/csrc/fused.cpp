@@ -1013,6 +1013,9 @@ namespace qcore
         // Gates are taken in order; a gate may pass a deferred gate only if they commute:
         // disjoint qubits, or both diagonal.  blocked[q]: 0 no deferred gate on q, 1 only
         // diagonal ones, 2 some other gate.
+        // Once every qubit of a tile that cannot grow is blocked by a non-diagonal deferred
+        // gate, no later gate can be taken (a gate needs all its qubits in the tile, none
+        // blocked): the rest is deferred without looking at it.
         StageTake take_stage(const std::vector<std::vector<int>> &qubits, const std::vector<char> &diag, const std::vector<int> &rem, int n, int m, int k, const std::vector<int> &start, bool grow)
         {
             StageTake r;
@@ -1022,8 +1025,15 @@ namespace qcore
             for (int q : start)
                 if (!r.in_tile[q])
                     r.in_tile[q] = 1, ++tile_size;
-            for (int gi : rem)
+            int closed = 0; // tile qubits blocked at level 2 (fixed tiles only)
+            for (size_t ri = 0; ri < rem.size(); ++ri)
             {
+                const int gi = rem[ri];
+                if (!grow && closed == tile_size)
+                {
+                    r.deferred.insert(r.deferred.end(), rem.begin() + ri, rem.end());
+                    break;
+                }
                 const auto &qs = qubits[gi];
                 const char level = diag[gi] ? 1 : 2;
                 bool take = (int)qs.size() <= k;
@@ -1047,10 +1057,49 @@ namespace qcore
                 {
                     r.deferred.push_back(gi);
                     for (int q : qs)
+                    {
+                        if (level == 2 && blocked[q] < 2 && r.in_tile[q])
+                            ++closed;
                         blocked[q] = std::max(blocked[q], level);
+                    }
                 }
             }
             return r;
+        }
+
+        // The number of gates take_stage(..., start, false) would take, by the same rule,
+        // without building its lists (the window search ranks every window by this count).
+        int count_stage(const std::vector<std::vector<int>> &qubits, const std::vector<char> &diag, const std::vector<int> &rem, int n, int k,
+                        const std::vector<int> &start)
+        {
+            char in_tile[qsv::kMaxQubits] = {}, blocked[qsv::kMaxQubits] = {};
+            int size = 0;
+            for (int q : start)
+                if (!in_tile[q])
+                    in_tile[q] = 1, ++size;
+            int taken = 0, closed = 0;
+            for (int gi : rem)
+            {
+                if (closed == size) // every tile qubit blocked: nothing more can be taken
+                    break;
+                const auto &qs = qubits[gi];
+                const char level = diag[gi] ? 1 : 2;
+                bool take = (int)qs.size() <= k;
+                for (int q : qs)
+                    if (blocked[q] > (diag[gi] ? 1 : 0) || !in_tile[q])
+                        take = false;
+                if (take)
+                    ++taken;
+                else
+                    for (int q : qs)
+                    {
+                        if (level == 2 && blocked[q] < 2 && in_tile[q])
+                            ++closed;
+                        blocked[q] = std::max(blocked[q], level);
+                    }
+            }
+            (void)n;
+            return taken;
         }
 
         std::vector<PlannedStage> plan_stages_with(const std::vector<std::vector<int>> &qubits, const std::vector<char> &diag, int n, int m, int k, bool windows)
@@ -1065,17 +1114,25 @@ namespace qcore
                 bool grown = true;
                 if (windows)
                 {
+                    // rank the windows by count, then take the first best one
+                    int best_count = (int)best.taken.size(), best_a = -1;
+                    std::vector<int> w;
                     for (int a = 4; a + (m - 4) <= n; ++a)
                     {
-                        std::vector<int> w = low;
+                        w = low;
                         for (int q = a; q < a + m - 4; ++q)
                             w.push_back(q);
-                        StageTake c = take_stage(qubits, diag, rem, n, m, k, w, false);
-                        if (c.taken.size() > best.taken.size())
-                        {
-                            best = std::move(c);
-                            grown = false;
-                        }
+                        const int c = count_stage(qubits, diag, rem, n, k, w);
+                        if (c > best_count)
+                            best_count = c, best_a = a;
+                    }
+                    if (best_a >= 0)
+                    {
+                        w = low;
+                        for (int q = best_a; q < best_a + m - 4; ++q)
+                            w.push_back(q);
+                        best = take_stage(qubits, diag, rem, n, m, k, w, false);
+                        grown = false;
                     }
                 }
                 PlannedStage st;
