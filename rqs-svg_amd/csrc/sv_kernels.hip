@@ -578,6 +578,12 @@ namespace qsv
             tile_tree<2>(y, out + blockIdx.x, out_stride);
         }
 
+        // Reduction units: one amplitude (complex128), or an aligned pair of amplitudes
+        // (complex64: 16 B per lane, the pair's two probabilities added first), so that every
+        // lane load moves 16 B.
+        template <class A>
+        constexpr uint64_t reduce_per_unit() { return sizeof(A) == 8 ? 2 : 1; }
+
         template <class A>
         __global__ __launch_bounds__(kBlock) void k_reduce_probs(const A *__restrict__ s, uint64_t count, uint64_t zbit, double *__restrict__ out)
         {
@@ -586,9 +592,25 @@ namespace qsv
 #pragma unroll
             for (int j = 0; j < 16; ++j)
             {
-                uint64_t i = base + uint64_t(j) * kBlock;
-                double p = i < count ? prob(lda(s + i)) : 0.0;
-                y[0][j] = (zbit && (i & zbit)) ? -p : p;
+                uint64_t u = base + uint64_t(j) * kBlock;
+                if constexpr (reduce_per_unit<A>() == 2)
+                {
+                    // amplitudes 2u and 2u + 1 (count is even: 2^n, n >= 1)
+                    const uint64_t i = 2 * u;
+                    double p = 0.0;
+                    if (i < count)
+                    {
+                        const v4f f = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(s + i));
+                        const double p0 = prob(v2d{double(f.x), double(f.y)}), p1 = prob(v2d{double(f.z), double(f.w)});
+                        p = ((zbit && (i & zbit)) ? -p0 : p0) + ((zbit && ((i + 1) & zbit)) ? -p1 : p1);
+                    }
+                    y[0][j] = p;
+                }
+                else
+                {
+                    double p = u < count ? prob(lda(s + u)) : 0.0;
+                    y[0][j] = (zbit && (u & zbit)) ? -p : p;
+                }
             }
             tile_tree<1>(y, out + blockIdx.x, 0);
         }
@@ -1469,7 +1491,9 @@ namespace qsv
     void reduce_probs(hipStream_t s, const A *state, uint64_t count, uint64_t zbit, double *work, double *out)
     {
         double *dev_out = work;
-        uint64_t g = (count + kTile - 1) / kTile;
+        if (reduce_per_unit<A>() == 2 && (count & 1))
+            throw std::runtime_error("reduce: complex64 states have an even amplitude count");
+        uint64_t g = (count / reduce_per_unit<A>() + kTile - 1) / kTile;
         double *lvl = work + 2;
         if (g == 1)
         {
@@ -1490,7 +1514,9 @@ namespace qsv
     void renormalize_async(hipStream_t s, A *state, uint64_t count, double *work)
     {
         double *dev_out = work;
-        const uint64_t g = (count + kTile - 1) / kTile;
+        if (reduce_per_unit<A>() == 2 && (count & 1))
+            throw std::runtime_error("renormalize: complex64 states have an even amplitude count");
+        const uint64_t g = (count / reduce_per_unit<A>() + kTile - 1) / kTile;
         double *lvl = work + 2;
         if (g == 1)
         {
