@@ -480,3 +480,26 @@ def test_stage_stream_33q_edge(gpu):
     finally:
         for f in sims:
             f.close()
+
+
+@pytest.mark.parametrize("n", [9, 13, 16])
+def test_complex64_reductions_match_exact_sums(gpu, n):
+    # The complex64 reductions (norm, <Z_q>) take aligned pairs of amplitudes per lane: each
+    # result must equal the exact sum over the state the simulator holds (read back, FP64
+    # products, math.fsum) to FP64 rounding, for a partial tile (9 qubits), one tile (13)
+    # and several (16), and for <Z_0>, where the two amplitudes of a pair have opposite signs.
+    import math
+    psi = random_state(n, 40 + n)
+    with pyqcs.Simulator(gpu, n, precision=1, block_qubits=4) as f:
+        f.set_statevector(psi)
+        norm = f.norm()
+        ez = f.expect_z()
+        state = f.statevector()
+    p = (state.real.astype(np.float64) ** 2 + state.imag.astype(np.float64) ** 2)
+    want = math.fsum(p)
+    say(f"complex64 reductions n={n}: norm {norm!r} exact {want!r}")
+    assert abs(norm - want) <= 2e-15 * want  # a tree of 2^n terms: at most ~n roundings deep
+    idx = np.arange(1 << n)
+    for q in range(n):
+        zq = math.fsum(np.where((idx >> q) & 1, -p, p))
+        assert abs(ez[q] - zq) <= 2e-15 * want, q
