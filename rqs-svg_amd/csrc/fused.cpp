@@ -1261,6 +1261,8 @@ namespace qcore
             for (int b = 0; b < m; ++b)
                 if (!is_block[b])
                     sb.w[j++] = 1 << b;
+            for (int b = 0; b < m; ++b)
+                sb.lds[b] = qsv::stage_lds_offset(uint32_t(sb.w[b]));
             desc.push_back(sb);
         }
         static_assert(sizeof(qsv::StageBlock) % 16 == 0, "StageBlock keeps 16-B alignment");

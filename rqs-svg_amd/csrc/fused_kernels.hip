@@ -103,6 +103,8 @@ namespace qsv
             // `lane` loads as its j-th v4f, so each of a wave's UNITS reads is one
             // contiguous, conflict-free 1 KiB
             static constexpr int UNITS = MT * (H / 4);
+            // the products of K values 8p .. 8p + 7 need part p: units 2p, 2p + 1 of each row tile
+            static constexpr int PARTS = H / 8;
             float a[MT][H];
             // the 16-B unit of R (from its start) that LDS unit j * 64 + lane holds
             static __device__ __forceinline__ int source_unit(int j, int lane)
@@ -110,12 +112,12 @@ namespace qsv
                 const int mt = j / (H / 4), q = j % (H / 4);
                 return ((32 * mt + (lane & 31)) * D + (lane >> 5) * H) / 4 + q;
             }
-            __device__ __forceinline__ void load_lds(const v4f *m, int lane)
+            __device__ __forceinline__ void load_lds_part(const v4f *m, int lane, int p)
             {
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-                    for (int q = 0; q < H / 4; ++q)
+                    for (int q = 2 * p; q < 2 * p + 2; ++q)
                     {
                         const v4f v = m[(mt * (H / 4) + q) * 64 + lane];
                         a[mt][4 * q + 0] = v.x;
@@ -123,6 +125,12 @@ namespace qsv
                         a[mt][4 * q + 2] = v.z;
                         a[mt][4 * q + 3] = v.w;
                     }
+            }
+            __device__ __forceinline__ void load_lds(const v4f *m, int lane)
+            {
+#pragma unroll
+                for (int p = 0; p < PARTS; ++p)
+                    load_lds_part(m, lane, p);
             }
             __device__ __forceinline__ void load(const float *R, int c, int h)
             {
@@ -149,17 +157,25 @@ namespace qsv
                     for (int mt = 0; mt < MT; ++mt)
                         d[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt][st], x[st], d[mt], 0, 0, 0);
             }
-            // G column groups at once: G * MT independent accumulation chains
+            // G column groups at once: G * MT independent accumulation chains; part p = the
+            // products of K values 8p .. 8p + 7
             template <int G>
-            __device__ __forceinline__ void apply_n(const float (&x)[G][H], v16f (&d)[G][MT]) const
+            __device__ __forceinline__ void apply_part(const float (&x)[G][H], v16f (&d)[G][MT], int p) const
             {
 #pragma unroll
-                for (int st = 0; st < H; ++st)
+                for (int st = 8 * p; st < 8 * p + 8; ++st)
 #pragma unroll
                     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                         for (int g = 0; g < G; ++g)
                             d[g][mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt][st], x[g][st], d[g][mt], 0, 0, 0);
+            }
+            template <int G>
+            __device__ __forceinline__ void apply_n(const float (&x)[G][H], v16f (&d)[G][MT]) const
+            {
+#pragma unroll
+                for (int p = 0; p < PARTS; ++p)
+                    apply_part<G>(x, d, p);
             }
         };
 
@@ -175,23 +191,29 @@ namespace qsv
             static constexpr int D = 2 << KQ, MT = D / 32, H = D / 2, ST = H / 8;
             // LDS copy, as in the FP32 form: the hi units, then the lo units
             static constexpr int UNITS = 2 * MT * ST;
+            // the products of K step st (K values 8 st .. 8 st + 7) need part st
+            static constexpr int PARTS = ST;
             bf16x8 hi[MT][ST], lo[MT][ST];
             static __device__ __forceinline__ int source_unit(int j, int lane)
             {
                 const int part = j / (MT * ST), mt = (j % (MT * ST)) / ST, st = j % ST;
                 return part * (D * D / 8) + ((32 * mt + (lane & 31)) * D + (lane >> 5) * H) / 8 + st;
             }
-            __device__ __forceinline__ void load_lds(const v4f *m, int lane)
+            __device__ __forceinline__ void load_lds_part(const v4f *m, int lane, int st)
             {
                 const bf16x8 *u = reinterpret_cast<const bf16x8 *>(m);
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
+                {
+                    hi[mt][st] = u[(mt * ST + st) * 64 + lane];
+                    lo[mt][st] = u[((MT + mt) * ST + st) * 64 + lane];
+                }
+            }
+            __device__ __forceinline__ void load_lds(const v4f *m, int lane)
+            {
 #pragma unroll
-                    for (int st = 0; st < ST; ++st)
-                    {
-                        hi[mt][st] = u[(mt * ST + st) * 64 + lane];
-                        lo[mt][st] = u[((MT + mt) * ST + st) * 64 + lane];
-                    }
+                for (int st = 0; st < ST; ++st)
+                    load_lds_part(m, lane, st);
             }
             __device__ __forceinline__ void load(const float *R, int c, int h)
             {
@@ -241,12 +263,11 @@ namespace qsv
                     }
                 }
             }
-            // G column groups at once: G * MT independent accumulation chains
+            // G column groups at once: G * MT independent accumulation chains; part st = the
+            // products of step st
             template <int G>
-            __device__ __forceinline__ void apply_n(const float (&x)[G][H], v16f (&d)[G][MT]) const
+            __device__ __forceinline__ void apply_part(const float (&x)[G][H], v16f (&d)[G][MT], int st) const
             {
-#pragma unroll
-                for (int st = 0; st < ST; ++st)
                 {
                     bf16x8 xh[G], xl[G];
 #pragma unroll
@@ -268,6 +289,13 @@ namespace qsv
                         for (int g = 0; g < G; ++g)
                             d[g][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi[mt][st], xh[g], d[g][mt], 0, 0, 0);
                 }
+            }
+            template <int G>
+            __device__ __forceinline__ void apply_n(const float (&x)[G][H], v16f (&d)[G][MT]) const
+            {
+#pragma unroll
+                for (int st = 0; st < ST; ++st)
+                    apply_part<G>(x, d, st);
             }
         };
 
@@ -334,19 +362,13 @@ namespace qsv
                     }
             }
         }
-        // LDS slot of tile index i in the stage kernels.  Blocks gather their columns at strides
-        // 2^b (b = the tile bits below the block's lowest qubit) and would hit one bank from
-        // all lanes; XOR-ing index bits 1..4 with bits 5..8 and 9..12 spreads such strides
-        // over the banks.  Bit 0 is kept, so amplitudes 2j and 2j + 1 stay one aligned 16-B
-        // pair for the tile's global loads and stores.
-        __device__ __forceinline__ int swz64(int i) { return i ^ ((((i >> 5) ^ (i >> 9)) & 15) << 1); }
+        // LDS slot of tile index i in the complex64 stage kernels (stage_lds_offset in slots).
+        // The map is linear over GF(2) (it XORs bits 1..4 with bits of i it leaves unchanged),
+        // so for indices built from disjoint bit sets, swz64(a | b) = swz64(a) ^ swz64(b): the
+        // stage kernel combines per-lane and uniform parts of an LDS address with one XOR.
+        __device__ __forceinline__ int swz64(int i) { return int(stage_lds_offset(uint32_t(i)) >> 3); }
         // complex128 (16-B slots): XOR bits 0..3 with bits 4..7 and 8..11
         __device__ __forceinline__ int swz128(int i) { return i ^ (((i >> 4) ^ (i >> 8)) & 15); }
-        // swz64 is linear over GF(2) (it XORs bits 1..4 with bits of i it leaves unchanged), so
-        // for indices built from disjoint bit sets, swz64(a | b) = swz64(a) ^ swz64(b): the
-        // stage kernel combines per-lane and uniform parts of an LDS address with one XOR.
-        // Byte offset of swz64(i) for 8-B slots.
-        __device__ __forceinline__ uint32_t swz64b(uint32_t i) { return uint32_t(swz64(int(i))) << 3; }
 
         // Read-only data of a launch (block descriptors, complex128 matrices) read through the
         // constant address space: uniform loads from it become scalar loads, which neither
@@ -627,11 +649,35 @@ namespace qsv
                 lds_barrier();
         }
 
+        // Swizzled LDS byte offsets of one block's tile bits: tb[j] of the block's j-th qubit,
+        // cb[j] of its j-th column bit; uniform, so kept in scalar registers.
+        template <int KQ, int M>
+        struct BlockOffsets
+        {
+            uint32_t tb[KQ], cb[M - KQ];
+            __device__ __forceinline__ void read(const __attribute__((address_space(4))) StageBlock &B)
+            {
+#pragma unroll
+                for (int j = 0; j < KQ; ++j)
+                    tb[j] = B.lds[j];
+#pragma unroll
+                for (int j = 0; j < M - KQ; ++j)
+                    cb[j] = B.lds[KQ + j];
+            }
+        };
+
         // One block of a stage on the tile in LDS: CWAVES waves (index `wave`) share the
         // 2^(M-KQ) columns, 32 per MFMA column group, GCAP (0: the tuned default) groups per
         // pass at most.  The caller separates blocks with a workgroup barrier.
-        template <int KQ, int M, bool X3, int CWAVES, int GCAP = 0>
-        __device__ __forceinline__ void stage_block_with(v2f *tile, const BlockMatrix<KQ, X3> &A, const __attribute__((address_space(4))) StageBlock &B, int wave, int c, int h)
+        //
+        // PIPE: the caller's load_part(p) reads part p of the matrix A from LDS (what the
+        // products of the gathered amplitudes 4p .. 4p + 3 need), and the first pass is
+        // software-pipelined over the parts: the gathers and matrix part p + 1 (and a quarter
+        // of the next pass's gathers) are issued before the products of part p, so the
+        // products start once the first part has arrived instead of after every LDS read of
+        // the block.  Same products in the same order either way.
+        template <int KQ, int M, bool X3, int CWAVES, int GCAP = 0, bool PIPE = false, class LoadPart>
+        __device__ __forceinline__ void stage_block_at(v2f *tile, const BlockMatrix<KQ, X3> &A, LoadPart &&load_part, const BlockOffsets<KQ, M> &O, int wave, int c, int h)
         {
             constexpr int D = 2 << KQ;
             constexpr int MT = D / 32;
@@ -646,15 +692,10 @@ namespace qsv
             constexpr int G = PER_WAVE < GMAX ? PER_WAVE : GMAX;
             static_assert(PER_WAVE % G == 0, "groups per pass must divide the groups per wave");
             constexpr bool AHEAD = X3 && QSV_STAGE_GATHER_AHEAD;
-            // LDS byte offsets (swizzled) of the block bits (amplitude index bits) and
-            // of the column bits; uniform, so kept in scalar registers
-            uint32_t tb[KQ], cb[M - KQ];
-#pragma unroll
-            for (int j = 0; j < KQ; ++j)
-                tb[j] = swz64b(uint32_t(B.w[j]));
-#pragma unroll
-            for (int j = 0; j < M - KQ; ++j)
-                cb[j] = swz64b(uint32_t(B.w[KQ + j]));
+            constexpr int PARTS = BlockMatrix<KQ, X3>::PARTS;
+            static_assert(AMPS == 4 * PARTS, "a matrix part per 4 gathered amplitudes");
+            const uint32_t(&tb)[KQ] = O.tb;
+            const uint32_t(&cb)[M - KQ] = O.cb;
             // amplitude a of a column: XOR of tb[j] over the set bits j of a
             auto amp_off = [&](int a) {
                 uint32_t o = 0;
@@ -678,7 +719,7 @@ namespace qsv
             // G column groups per pass (independent MFMA chains).  The next pass's gathers are
             // issued before this pass's products and scatters (its columns are disjoint from
             // this pass's), so their LDS latency hides under the MFMA work.
-            auto gather = [&](int gr0, float (&x)[G][H], uint32_t (&U)[G]) {
+            auto group_base = [&](int gr0, uint32_t (&U)[G]) {
 #pragma unroll
                 for (int g = 0; g < G; ++g)
                 {
@@ -689,9 +730,16 @@ namespace qsv
                     for (int j = 5; j < M - KQ; ++j)
                         if ((grp >> (j - 5)) & 1)
                             U[g] ^= cb[j];
+                }
+            };
+            // amplitudes 4p .. 4p + 3 of every group of a pass
+            auto gather_part = [&](const uint32_t (&U)[G], float (&x)[G][H], int p) {
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                {
                     const uint32_t gbase = g_lane ^ U[g];
 #pragma unroll
-                    for (int e = 0; e < AMPS; ++e)
+                    for (int e = 4 * p; e < 4 * p + 4; ++e)
                     {
                         const v2f a = *reinterpret_cast<const v2f *>(tbytes + (gbase ^ amp_off(e)));
                         x[g][2 * e] = a.x;
@@ -699,22 +747,54 @@ namespace qsv
                     }
                 }
             };
+            auto gather = [&](int gr0, float (&x)[G][H], uint32_t (&U)[G]) {
+                group_base(gr0, U);
+#pragma unroll
+                for (int p = 0; p < PARTS; ++p)
+                    gather_part(U, x, p);
+            };
             uint32_t U[G], Un[G];
             float x[G][H], xn[G][H];
-            gather(0, x, U);
+            if (!PIPE)
+                gather(0, x, U);
 #pragma unroll
             for (int gr0 = 0; gr0 < PER_WAVE; gr0 += G)
             {
                 const bool more = AHEAD && gr0 + G < PER_WAVE;
-                if (more)
-                    gather(gr0 + G, xn, Un);
                 v16f d[G][MT];
 #pragma unroll
                 for (int g = 0; g < G; ++g)
 #pragma unroll
                     for (int mt = 0; mt < MT; ++mt)
                         d[g][mt] = v16f{0.f};
-                A.template apply_n<G>(x, d);
+                if (PIPE && gr0 == 0)
+                {
+                    group_base(0, U);
+                    if (more)
+                        group_base(G, Un);
+                    gather_part(U, x, 0);
+                    load_part(0);
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int p = 0; p < PARTS; ++p)
+                    {
+                        if (p + 1 < PARTS)
+                        {
+                            gather_part(U, x, p + 1);
+                            load_part(p + 1);
+                        }
+                        if (more)
+                            gather_part(Un, xn, p);
+                        A.template apply_part<G>(x, d, p);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                }
+                else
+                {
+                    if (more)
+                        gather(gr0 + G, xn, Un);
+                    A.template apply_n<G>(x, d);
+                }
                 // accumulator register r of tile mt holds real row 32*mt + 8*(r/4) + 4*h + r%4,
                 // i.e. amplitudes 16*mt + 4*(r/4) + 2*h and the next one
 #pragma unroll
@@ -745,6 +825,15 @@ namespace qsv
                     }
                 }
             }
+        }
+
+        // The same with the matrix already in registers.
+        template <int KQ, int M, bool X3, int CWAVES, int GCAP = 0>
+        __device__ __forceinline__ void stage_block_with(v2f *tile, const BlockMatrix<KQ, X3> &A, const __attribute__((address_space(4))) StageBlock &B, int wave, int c, int h)
+        {
+            BlockOffsets<KQ, M> O;
+            O.read(B);
+            stage_block_at<KQ, M, X3, CWAVES, GCAP, false>(tile, A, [](int) {}, O, wave, c, h);
         }
 
         template <int KQ, int M, bool X3, int CWAVES>
@@ -1149,17 +1238,25 @@ namespace qsv
             }
             const int c = lane & 31;
             const int h = lane >> 5;
+            // The next block's LDS offsets are read at the end of each step, where the
+            // barrier waits for the step's LDS accesses anyway: a scalar load outstanding
+            // during the step would make each wait for its LDS reads a wait for all of them.
+            BlockOffsets<KQ, M> O;
+            O.read(blocks[0]);
             lds_barrier();
             uint64_t i = 0;
             int bi = 0;
             for (uint64_t k = 0; k < steps; ++k)
             {
                 Mat A;
-                A.load_lds((k & 1) ? mat1 : mat0, lane);
-                stage_block_with<KQ, M, X3, kStreamCompute, 1>((i & 1) ? buf1 : buf0, A, blocks[bi], wave, c, h);
-                lds_barrier();
+                const v4f *m = (k & 1) ? mat1 : mat0;
+                // the matrix is read from LDS part by part, pipelined with the first products
+                stage_block_at<KQ, M, X3, kStreamCompute, 1, true>((i & 1) ? buf1 : buf0, A, [&](int p) { A.load_lds_part(m, lane, p); }, O, wave, c, h);
                 bi = bi + 1 == nblocks ? 0 : bi + 1;
                 i += bi == 0;
+                __builtin_amdgcn_sched_barrier(0);
+                O.read(blocks[bi]);
+                lds_barrier();
             }
         }
 

@@ -42,13 +42,23 @@ namespace qsv
     // Block i's matrix is at R + blocks[i].matrix (layout of BlockOp for x3); w[j] (j < kq)
     // is the LDS index weight (1 << tile bit position) of the block's j-th qubit and
     // w[kq + j] that of the j-th column bit (the other tile bits, ascending).  blocks
-    // points to device memory.
+    // points to device memory.  Complex64 stages also carry lds[j] = stage_lds_offset(w[j]),
+    // the byte offset of that weight in the swizzled LDS tile, so that the kernels read the
+    // offsets ready-made instead of swizzling them at every block.
     struct StageBlock
     {
         int w[16];
         uint32_t matrix; // offset in floats from the launch's matrix base (complex128 stages: doubles from M)
         uint32_t pad[3];
+        uint32_t lds[16];
     };
+    // LDS slot of tile index i in the complex64 stage kernels, as a byte offset (8-B slots).
+    // Blocks gather their columns at strides 2^b (b = the tile bits below the block's lowest
+    // qubit) and would hit one bank from all lanes; XOR-ing index bits 1..4 with bits 5..8
+    // and 9..12 spreads such strides over the banks.  Bit 0 is kept, so amplitudes 2j and
+    // 2j + 1 stay one aligned 16-B pair for the tile's global loads and stores.  The map is
+    // linear over GF(2), so the offset of a | b (disjoint bits) is the XOR of theirs.
+    constexpr uint32_t stage_lds_offset(uint32_t i) { return (i ^ ((((i >> 5) ^ (i >> 9)) & 15u) << 1)) << 3; }
     // One state's part of a stage launch (device memory, read by the kernel): its tiles are
     // the launch's tiles [tile_begin, tile_begin + 2^(n - m)), its nblocks StageBlock
     // descriptors start `blocks` floats after the launch's matrix base (a multiple of 4).
