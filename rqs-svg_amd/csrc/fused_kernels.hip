@@ -77,6 +77,15 @@ namespace qsv
 #ifndef QSV_STAGE_GROUPS_KQ5
 #define QSV_STAGE_GROUPS_KQ5 1
 #endif
+#ifndef QSV_STREAM_GROUPS
+#define QSV_STREAM_GROUPS 1 // column groups per pass in the streaming stage kernel (FP32)
+#endif
+#ifndef QSV_STREAM_GROUPS_X3
+#define QSV_STREAM_GROUPS_X3 1 // the same, BF16x3
+#endif
+#ifndef QSV_STREAM_AHEAD_FP32
+#define QSV_STREAM_AHEAD_FP32 1 // the streaming kernel gathers the next FP32 pass during the products too
+#endif
 #ifndef QSV_STAGE_GROUPS_KQ4
 #define QSV_STAGE_GROUPS_KQ4 1
 #endif
@@ -691,7 +700,7 @@ namespace qsv
             constexpr int GMAX = GCAP > 0 ? GCAP : KQ == 5 ? (X3 ? QSV_STAGE_GROUPS_KQ5_X3 : QSV_STAGE_GROUPS_KQ5) : QSV_STAGE_GROUPS_KQ4;
             constexpr int G = PER_WAVE < GMAX ? PER_WAVE : GMAX;
             static_assert(PER_WAVE % G == 0, "groups per pass must divide the groups per wave");
-            constexpr bool AHEAD = X3 && QSV_STAGE_GATHER_AHEAD;
+            constexpr bool AHEAD = (X3 || (PIPE && QSV_STREAM_AHEAD_FP32)) && QSV_STAGE_GATHER_AHEAD;
             constexpr int PARTS = BlockMatrix<KQ, X3>::PARTS;
             static_assert(AMPS == 4 * PARTS, "a matrix part per 4 gathered amplitudes");
             const uint32_t(&tb)[KQ] = O.tb;
@@ -1251,7 +1260,7 @@ namespace qsv
                 Mat A;
                 const v4f *m = (k & 1) ? mat1 : mat0;
                 // the matrix is read from LDS part by part, pipelined with the first products
-                stage_block_at<KQ, M, X3, kStreamCompute, 1, true>((i & 1) ? buf1 : buf0, A, [&](int p) { A.load_lds_part(m, lane, p); }, O, wave, c, h);
+                stage_block_at<KQ, M, X3, kStreamCompute, X3 ? QSV_STREAM_GROUPS_X3 : QSV_STREAM_GROUPS, true>((i & 1) ? buf1 : buf0, A, [&](int p) { A.load_lds_part(m, lane, p); }, O, wave, c, h);
                 bi = bi + 1 == nblocks ? 0 : bi + 1;
                 i += bi == 0;
                 __builtin_amdgcn_sched_barrier(0);
