@@ -469,10 +469,11 @@ def subsystem_leg(n=30, reps=4):
     return out
 
 
-def _timed_run(sim, circ, prep=(), reps=1):
+def _timed_run(sim, circ, prep=(), reps=1, runs=None):
     """Median seconds of: gates queued and applied (flush) plus a norm; the state is reset
-    to |0...0> and `prep` applied (untimed) before each repetition."""
-    ts = []
+    to |0...0> and `prep` applied (untimed) before each repetition.  Every repetition's
+    seconds are appended to `runs` when given."""
+    ts = runs if runs is not None else []
     for _ in range(reps):
         sim.call("set_zero_state")
         if prep:
@@ -494,9 +495,11 @@ def planner_leg(n, depth, record_count, k=4, fixed_sweep=False):
     training circuits at 26-30 qubits, none of them the evaluated circuits), then an n-qubit
     QFT on the state of one random brick layer and an n-qubit brick circuit of the given
     depth, each run three ways -- the greedy fusion fallback (one sweep per block), the
-    default stage plan, and the plan the model picks (kept only when predicted 3 % faster
-    than the default) -- and each output compared with the plain per-gate complex128 path
-    (one kernel per gate, the reference's method) on a second state in HBM."""
+    default stage plan, and the plan the model picks (taken only when its predicted gain
+    over the default exceeds the model's error; one run each, so an override runs as the
+    model picked it, before the planner's measured check has compared it with the default)
+    -- and each output compared with the plain per-gate complex128 path (one kernel per
+    gate, the reference's method) on a second state in HBM."""
     lib = pyqcs.gpu_library()
     model, info = pyqcs.train_cost_model(lib, 2, k, (26, 28, 30, 32), record_count, (26, 28, 30), log=log)
     out = {"config": f"complex128 fused blocks (k<={k}), {n}-qubit QFT on a random brick layer and "
@@ -597,6 +600,9 @@ def precision_planner_leg(n, record_count, fixed_sweep=False):
     blocks, fewer-stage tile choice) and with the model's choice (median of 3 runs each).
     With fixed_sweep, every fixed stage plan is timed as well."""
     lib = pyqcs.gpu_library()
+    # With the planner's measured check (on by default), a circuit's first run takes the
+    # model's choice, its second the default plan, and the third the faster of the two
+    # (when the model keeps the default, all three run it): learned_s is the median.
     # candidate-plan timings at 30 and 32 qubits (one seed): the launches of the sizes the
     # model chooses for, at the recording cost of three seeds at 26-30 qubits
     model, info = pyqcs.train_cost_model(lib, 3, 5, (26, 28, 30, 32), record_count // 2, (30, 32), plan_seeds=(1001,), log=log)
@@ -609,8 +615,15 @@ def precision_planner_leg(n, record_count, fixed_sweep=False):
             pyqcs.set_planner(sim, None)
             res["default_s"] = _timed_run(sim, circ, prep, reps=3)
             pyqcs.set_planner(sim, model)
-            res["learned_s"] = _timed_run(sim, circ, prep, reps=3)
+            st0 = pyqcs.planner_stats(sim)
+            runs = []
+            res["learned_s"] = _timed_run(sim, circ, prep, reps=3, runs=runs)
+            st1 = pyqcs.planner_stats(sim)
+            res["learned_runs_s"] = runs
             res["learned_plan"] = list(pyqcs.planner_last_choice(sim))
+            # flushes that ran the model's (non-default) choice over the 3 runs and preps
+            res["learned_flushes"] = st1["learned_choices"] - st0["learned_choices"]
+            res["flushes"] = st1["flushes"] - st0["flushes"]
             pyqcs.set_planner(sim, None)
             res["learned_vs_default"] = res["default_s"] / res["learned_s"]
             if fixed_sweep:
@@ -623,7 +636,9 @@ def precision_planner_leg(n, record_count, fixed_sweep=False):
                 sim.set_plan(None)
                 res["fixed_plans_s"] = fixed
             log(f"precision planner leg: {name}: default {res['default_s'] * 1e3:.1f} ms, learned "
-                f"{res['learned_s'] * 1e3:.1f} ms ({res['learned_vs_default']:.3f}x), plan {res['learned_plan']}")
+                f"{res['learned_s'] * 1e3:.1f} ms ({res['learned_vs_default']:.3f}x; runs "
+                + ", ".join(f"{t * 1e3:.1f}" for t in runs)
+                + f"; {res['learned_flushes']:.0f} of {res['flushes']:.0f} flushes off the default plan), plan {res['learned_plan']}")
             out[name] = res
         if fixed_sweep:
             # the best single fixed plan over all workloads, against the learned choices

@@ -208,7 +208,10 @@ int qcs_ext_batch_run_direct(qcs_ext_batch *batch, double *expz, uint64_t count,
  * default plan's by the model's own typical error: a factor 2^-rms, rms = the root mean
  * squared error (log2 microseconds) of the model's last qcs_ext_mlp_train on its data,
  * within [0.75, 0.97] (0.97 for parameters set by qcs_ext_mlp_set_params).  Without a model
- * the default plan is used.  The caller keeps the model alive while it is attached. */
+ * the default plan is used.  With the measured check (qcs_ext_planner_set_judge, on by
+ * default) the default plan also judges each override by measurement: a flush structure
+ * keeps the model's choice only if it ran faster than the default plan.  The caller keeps
+ * the model alive while it is attached. */
 typedef struct qcs_ext_mlp qcs_ext_mlp;
 int qcs_ext_mlp_create(qcs_ext_mlp **out, int num_features, uint64_t seed);
 int qcs_ext_mlp_destroy(qcs_ext_mlp *model);
@@ -238,6 +241,16 @@ int qcs_ext_planner_record_plans(qcs_simulator *sim, int capacity, float *featur
 /* The plan the learned planner ran at the last flush: plan[0..4] = stage tile bits (0: one
  * launch per block), block qubits, tile choice, window, rule (as in qcs_ext_set_plan). */
 int qcs_ext_planner_last_choice(qcs_simulator *sim, int *plan);
+/* No counterpart in the reference.  Measured check of the learned planner's overrides, on
+ * (1, the default) or off (0).  On: per flush structure (the gates' names, kinds and qubits;
+ * parameters ignored), the first flush for which the model picks a plan other than the
+ * default runs that plan with timing events around it, the next flush of the same structure
+ * runs the default plan timed, and every later one runs whichever of the two was faster (the
+ * third flush waits for the second's events).  Off: every flush runs the model's choice.
+ * Attaching another model or changing the setting forgets the checks; at most 1024
+ * structures are checked per simulator (later ones run the model's choice).  Queued gates
+ * are flushed first. */
+int qcs_ext_planner_set_judge(qcs_simulator *sim, int on);
 
 /* Per-kernel-class timing with HIP events around every launch (oracle: host clock). */
 int qcs_ext_profile(qcs_simulator *sim, int enable);

@@ -256,6 +256,57 @@ namespace qcore
             (void)hipEventDestroy(copy_done_);
         for (hipEvent_t e : ev_pool_)
             (void)hipEventDestroy(e);
+        clear_verdicts();
+    }
+
+    void FusedEngine::clear_verdicts()
+    {
+        for (auto &kv : verdicts_)
+            for (hipEvent_t e : kv.second.ev)
+                if (e)
+                    (void)hipEventDestroy(e);
+        verdicts_.clear();
+    }
+
+    void FusedEngine::set_model(CostModel *m)
+    {
+        // measured checks belong to the model that made the choices
+        if (m != model_)
+            clear_verdicts();
+        model_ = m;
+    }
+
+    void FusedEngine::set_judge(bool on)
+    {
+        judge_ = on;
+        clear_verdicts();
+    }
+
+    uint64_t FusedEngine::structure_key(const std::vector<ResolvedGate> &gates)
+    {
+        // FNV-1a over what decides a plan: gate names and kinds, targets and controls
+        uint64_t h = 1469598103934665603ull;
+        auto mix = [&](uint64_t v) {
+            for (int b = 0; b < 8; ++b)
+            {
+                h ^= (v >> (8 * b)) & 0xff;
+                h *= 1099511628211ull;
+            }
+        };
+        mix(gates.size());
+        for (const auto &g : gates)
+        {
+            for (char ch : g.spec.name)
+                mix(uint64_t(uint8_t(ch)));
+            mix(uint64_t(g.spec.kind));
+            for (const auto *v : {&g.tphys, &g.pphys, &g.nphys})
+            {
+                mix(v->size());
+                for (int q : *v)
+                    mix(uint64_t(q));
+            }
+        }
+        return h;
     }
 
     void FusedEngine::reserve_matrices(size_t bytes)
@@ -469,24 +520,70 @@ namespace qcore
         for (size_t oi = 1; oi < opts.size(); ++oi)
             if (std::isfinite(cost[oi]) && cost[oi] < cost[best])
                 best = oi;
+        if (!(best != 0 && std::isfinite(cost[0]) && cost[best] < model_->choice_factor() * cost[0]))
+            best = 0;
+        // The default plan judges an override by measurement: the first flush of this
+        // structure runs the model's choice timed, the second the default timed, and the
+        // later ones the faster of the two.
+        size_t run = best;
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (best != 0 && judge_)
+        {
+            const uint64_t key = structure_key(gates);
+            auto it = verdicts_.find(key);
+            if (it == verdicts_.end() && verdicts_.size() < kMaxVerdicts)
+            {
+                it = verdicts_.emplace(key, Verdict{}).first;
+                for (hipEvent_t &e : it->second.ev)
+                    qsv::check(hipEventCreate(&e), "hipEventCreate");
+            }
+            if (it != verdicts_.end())
+            {
+                Verdict &v = it->second;
+                if (v.stage == 0 || v.option != best)
+                {
+                    v.stage = 1;
+                    v.option = best;
+                    t0 = v.ev[0];
+                    t1 = v.ev[1];
+                }
+                else if (v.stage == 1)
+                {
+                    v.stage = 2;
+                    run = 0;
+                    t0 = v.ev[2];
+                    t1 = v.ev[3];
+                }
+                else
+                {
+                    if (v.stage == 2)
+                    {
+                        float ms_learned = 0.f, ms_default = 0.f;
+                        qsv::check(hipEventSynchronize(v.ev[3]), "hipEventSynchronize");
+                        qsv::check(hipEventElapsedTime(&ms_learned, v.ev[0], v.ev[1]), "hipEventElapsedTime");
+                        qsv::check(hipEventElapsedTime(&ms_default, v.ev[2], v.ev[3]), "hipEventElapsedTime");
+                        v.learned = ms_learned < ms_default;
+                        v.stage = 3;
+                    }
+                    run = v.learned ? best : 0;
+                }
+            }
+        }
         stat_flushes_ += 1;
         stat_pred_greedy_ += cost[0];
-        if (best != 0 && std::isfinite(cost[0]) && cost[best] < model_->choice_factor() * cost[0])
-        {
+        stat_pred_chosen_ += cost[run];
+        if (run != 0)
             stat_learned_ += 1;
-            stat_pred_chosen_ += cost[best];
-        }
-        else
-        {
-            best = 0;
-            stat_pred_chosen_ += cost[0];
-        }
-        last_choice_ = opts[best];
+        last_choice_ = opts[run];
         // the chosen plan as planned above
-        if (opts[best].stage_bits > 0)
-            apply_planned_stages(gates, planned[best].stages, opts[best].stage_bits, opts[best].k);
+        if (t0)
+            qsv::check(hipEventRecord(t0, sim_.stream_), "hipEventRecord");
+        if (opts[run].stage_bits > 0)
+            apply_planned_stages(gates, planned[run].stages, opts[run].stage_bits, opts[run].k);
         else
-            apply_blocks(gates, planned[best].blocks);
+            apply_blocks(gates, planned[run].blocks);
+        if (t1)
+            qsv::check(hipEventRecord(t1, sim_.stream_), "hipEventRecord");
     }
 
     void FusedEngine::launch_mark_begin()

@@ -4,6 +4,7 @@
 #include "simulator.hpp"
 
 #include <algorithm>
+#include <unordered_map>
 #include <vector>
 
 namespace qcore
@@ -220,8 +221,15 @@ namespace qcore
         uint64_t gates_applied() const { return gates_applied_; }
 
         // Learned planner: with a model attached, each flush picks the candidate plan with
-        // the lowest predicted time unless it does not beat the greedy plan by 3 %.
-        void set_model(CostModel *m) { model_ = m; }
+        // the lowest predicted time unless it does not beat the default plan by the model's
+        // margin (CostModel::choice_factor).
+        void set_model(CostModel *m);
+        // Measured check of the model's overrides (on by default): per flush structure (the
+        // gates' names, kinds and qubits; parameters ignored), the first flush for which the
+        // model picks a non-default plan runs that plan with events around it, the next flush
+        // of the same structure runs the default plan timed, and later ones run whichever of
+        // the two was faster.  Off: every flush runs the model's choice.
+        void set_judge(bool on);
         // A fixed plan for every flush without a model: stages on tiles of 2^tile_bits
         // amplitudes with k-qubit blocks and tile choice `tiles` (StageTiles), or, with
         // tile_bits 0, one launch per block planned with k, window and rule (plan_rule).
@@ -283,6 +291,20 @@ namespace qcore
         uint64_t gates_applied_ = 0;
         CostModel *model_ = nullptr;
         PlanOption last_choice_{0, 0, 0, 0, kTilesAuto};
+        // one flush structure's measured check: stage 1 after the timed run of the model's
+        // choice (option), 2 after the timed run of the default plan, 3 once decided
+        struct Verdict
+        {
+            int stage = 0;
+            size_t option = 0;
+            bool learned = false; // decided: the model's choice was faster
+            hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+        };
+        static constexpr size_t kMaxVerdicts = 1024;
+        static uint64_t structure_key(const std::vector<ResolvedGate> &gates);
+        void clear_verdicts();
+        bool judge_ = true;
+        std::unordered_map<uint64_t, Verdict> verdicts_;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> *launch_log_ = nullptr;
         std::vector<hipEvent_t> ev_pool_;
         size_t ev_used_ = 0;

@@ -145,6 +145,16 @@ namespace qcore
             fused_->set_stage_stream(on != 0);
     }
 
+    void Simulator::set_planner_judge(int on)
+    {
+        if (on != 0 && on != 1)
+            throw std::runtime_error("planner judge setting must be 0 or 1");
+        flush();
+        planner_judge_ = on;
+        if (fused_)
+            fused_->set_judge(on != 0);
+    }
+
     void Simulator::reinitialize_mapping()
     { // src/qcs.cu:1860-1885
         state_changed();
@@ -193,6 +203,7 @@ namespace qcore
             fused_ = new FusedEngine(*this, block_qubits_, precision_);
             fused_->set_stage_bits(stage_bits_);
             fused_->set_stage_stream(stage_stream_ != 0);
+            fused_->set_judge(planner_judge_ != 0);
         }
         work_len_ = qsv::reduce_workspace(count) + 16;
         qsv::check(hipMalloc((void **)&work_, work_len_ * sizeof(double)), "hipMalloc workspace");

@@ -187,3 +187,56 @@ def test_learned_planner_measured_gain(gpu):
         model.close()
     assert max(ratios.values()) <= 1.05
     assert min(ratios.values()) <= 0.95
+
+
+@pytest.mark.gpu
+def test_planner_judge_keeps_the_faster_plan(gpu, oracle):
+    # A hand-set model whose prediction falls with the tile bits (feature 17 = m / 16, through
+    # one hidden unit of each layer: log2 us = b3 - 32 * m / 16; block launches, with feature
+    # 17 = 0, are priced at b3) rates a stage launch on 2^14 tiles 4x cheaper than one on
+    # 2^13, so it overrides the default plan (2^13 tiles, 5-qubit blocks, fewer-stage tile
+    # choice) with a 2^14-tile plan.  With the measured
+    # check on, the first run of the circuit takes the model's plan, the second the default
+    # plan, and the later runs the faster of the two; with it off, every run takes the
+    # model's plan.  Every run's state matches the oracle.
+    n = 20
+    F = pyqcs.planner_num_features(gpu)
+    model = pyqcs.Mlp(gpu, F, seed=1)
+    p = np.zeros(model.P, dtype=np.float32)
+    p[17] = 1.0                      # W1[0][17]
+    p[H * F + H] = 1.0               # W2[0][0]
+    p[H * F + H + H * H + H] = -32.0  # w3[0]
+    p[-1] = np.log2(100.0) + 28.0    # b3
+    model.set_params(p)
+    X = np.zeros((3, F), dtype=np.float32)
+    X[:, 17] = [0.0, 13 / 16, 14 / 16]
+    assert np.allclose(model.predict(X), [np.log2(100.0) + 28, np.log2(100.0) + 2, np.log2(100.0)], atol=1e-4)
+    circ = circuits.brick(n, 6, seed=11)
+    with pyqcs.Simulator(oracle, n) as o:
+        o.run(circ)
+        want = o.statevector()
+    try:
+        for judge in (True, False):
+            with pyqcs.Simulator(gpu, n, seed=1, precision=3, block_qubits=5) as sim:
+                pyqcs.set_planner(sim, model)
+                pyqcs.planner_set_judge(sim, judge)
+                choices = []
+                for r in range(4):
+                    sim.call("set_zero_state")
+                    sim.run(circ)
+                    assert fidelity(sim.statevector(), want) >= 1 - 1e-6
+                    choices.append(tuple(pyqcs.planner_last_choice(sim)))
+                st = pyqcs.planner_stats(sim)
+                pyqcs.set_planner(sim, None)
+            print(f"judge {judge}: choices {choices}, stats {st}", flush=True)
+            assert st["flushes"] == 4
+            if judge:
+                learned, default = choices[0], choices[1]
+                assert learned[0] == 14 and default[:3] == (13, 5, 2)
+                assert choices[2] == choices[3] and choices[2] in (learned, default)
+                assert st["learned_choices"] == (3 if choices[2] == learned else 1)
+            else:
+                assert len(set(choices)) == 1 and choices[0][0] == 14
+                assert st["learned_choices"] == 4
+    finally:
+        model.close()
