@@ -865,7 +865,10 @@ namespace qsv
         // output bit b.  Input and output use the same address map, so the tile is updated in
         // place and workgroups own disjoint addresses.
         constexpr int kPermThreads = 512;
-        constexpr int kPermLds = 65536; // bytes
+#ifndef QSV_PERM_LDS
+#define QSV_PERM_LDS 65536
+#endif
+        constexpr int kPermLds = QSV_PERM_LDS; // bytes
 
         struct PermuteOp
         {
@@ -1735,7 +1738,7 @@ namespace qsv
     }
 
     template <class A>
-    int permute_tile_bits() { return kPermLds / int(sizeof(A)) == 4096 ? 12 : 13; }
+    int permute_tile_bits() { return __builtin_ctz(unsigned(kPermLds / int(sizeof(A)))); }
 
     template <class A>
     void launch_permute(hipStream_t s, A *state, int n, const std::vector<int> &dst)
