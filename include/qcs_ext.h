@@ -247,7 +247,8 @@ int qcs_ext_planner_last_choice(qcs_simulator *sim, int *plan);
  * default runs that plan with timing events around it, the next flush of the same structure
  * runs the default plan timed, and every later one runs whichever of the two was faster (the
  * third flush waits for the second's events).  Off: every flush runs the model's choice.
- * Attaching another model or changing the setting forgets the checks; at most 1024
+ * Attaching another model, changing this setting, the stage tile bits or the stage kernel
+ * (qcs_ext_set_stage_bits, qcs_ext_set_stage_stream) forgets the checks; at most 1024
  * structures are checked per simulator (later ones run the model's choice).  Queued gates
  * are flushed first. */
 int qcs_ext_planner_set_judge(qcs_simulator *sim, int on);

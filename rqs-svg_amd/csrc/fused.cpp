@@ -1620,6 +1620,8 @@ namespace qcore
     void FusedEngine::set_stage_stream(bool on)
     {
         flush();
+        if (on != stage_stream_)
+            clear_verdicts(); // measured with the other kernel
         stage_stream_ = on;
     }
 
@@ -1628,6 +1630,8 @@ namespace qcore
         if (m != 0 && (m < 11 || m > 14))
             throw std::runtime_error("stage tile bits must be 0 (per-block launches), 11, 12, 13 or 14");
         flush();
+        if (m != stage_bits_)
+            clear_verdicts(); // the candidate plans (and the default) change
         stage_bits_ = m;
     }
 
