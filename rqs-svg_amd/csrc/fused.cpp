@@ -910,6 +910,29 @@ namespace qcore
         }
     } // namespace
 
+    // An FP32 block matrix in the layout the block kernels read: 5-qubit blocks as Re M,
+    // Im M and Re M + Im M (each 32 x 32 row-major, the sum formed in FP64; three real
+    // products per block on MFMA), smaller blocks as the real form R (2K x 2K on
+    // interleaved (re, im) vectors).  dst holds (2K)^2 floats.
+    static void write_fp32_matrix(const std::vector<cd> &M, const std::vector<double> &R, int kq, float *dst)
+    {
+        const int K = 1 << kq;
+        if (kq == 5)
+        {
+            for (int r = 0; r < K; ++r)
+                for (int c = 0; c < K; ++c)
+                {
+                    const cd z = M[size_t(r) * K + c];
+                    dst[r * K + c] = float(z.real());
+                    dst[K * K + r * K + c] = float(z.imag());
+                    dst[2 * K * K + r * K + c] = float(z.real() + z.imag());
+                }
+            return;
+        }
+        for (size_t i = 0; i < R.size(); ++i)
+            dst[i] = float(R[i]);
+    }
+
     void split_bf16(const double *R, size_t count, uint16_t *hi, uint16_t *lo)
     {
         for (size_t i = 0; i < count; ++i)
@@ -953,8 +976,7 @@ namespace qcore
             }
             else
             {
-                for (size_t i = 0; i < per; ++i)
-                    mats[b.matrix_offset + i] = float(R[i]);
+                write_fp32_matrix(M, R, kq, mats.data() + b.matrix_offset);
             }
             std::vector<int> outside = lay.pos_out;
             outside.insert(outside.end(), lay.neg_out.begin(), lay.neg_out.end());
@@ -1342,8 +1364,7 @@ namespace qcore
             }
             else
             {
-                for (size_t i = 0; i < per; ++i)
-                    mats[sb.matrix + i] = float(R[i]);
+                write_fp32_matrix(M, R, kq, mats.data() + sb.matrix);
             }
             std::vector<char> is_block(m, 0);
             for (int j = 0; j < kq; ++j)

@@ -108,6 +108,8 @@ namespace qsv
         struct BlockMatrix<KQ, false>
         {
             static constexpr int D = 2 << KQ, MT = D / 32, H = D / 2;
+            // accumulators per column group; output amplitude offset of lane half h
+            static constexpr int NACC = MT, OUT_H = 2;
             // LDS copy (streaming stage kernel): 16-B unit j * 64 + lane holds what lane
             // `lane` loads as its j-th v4f, so each of a wave's UNITS reads is one
             // contiguous, conflict-free 1 KiB
@@ -186,6 +188,120 @@ namespace qsv
                 for (int p = 0; p < PARTS; ++p)
                     apply_part<G>(x, d, p);
             }
+            // accumulator register r of tile mt holds real row 32*mt + 8*(r/4) + 4*h + r%4:
+            // amplitudes e = 16*mt + 4*(r/4) (+ OUT_H*h) and e + 1 as {re, im, re, im}
+            template <class W>
+            static __device__ __forceinline__ void outputs2(const v16f (&d)[NACC], W &&w)
+            {
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        w(16 * mt + 4 * q, v4f{d[mt][4 * q + 0], d[mt][4 * q + 1], d[mt][4 * q + 2], d[mt][4 * q + 3]});
+            }
+        };
+
+        // FP32 5-qubit blocks (32 x 32 complex) with three real products in place of four
+        // (Gauss): P1 = Ar xr, P2 = Ai xi, P3 = (Ar + Ai)(xr + xi), then Re = P1 - P2 and
+        // Im = P3 - P1 - P2, on v_mfma_f32_32x32x2f32: 48 products per 32 columns instead of
+        // 64 (the form is MFMA-bound).  Step st pairs amplitude st of lanes 0-31 with amplitude
+        // 16 + st of lanes 32-63, which are the lane half's x values 2st (re) and 2st + 1 (im).
+        // R holds Ar, Ai and Ar + Ai, each 32 x 32 row-major FP32 (the sum formed in FP64).
+        template <>
+        struct BlockMatrix<5, false>
+        {
+            static constexpr int H = 32, NACC = 3, OUT_H = 4;
+            // LDS copy: unit (4q + p) * 64 + lane holds the lane's Ar / Ai / As (q = 0, 1, 2)
+            // entries 4p .. 4p + 3; part p = the products of steps 4p .. 4p + 3
+            static constexpr int UNITS = 12, PARTS = 4;
+            float a[3][16]; // [Ar, Ai, Ar + Ai][step]: row c, column 16h + step
+            static __device__ __forceinline__ int source_unit(int j, int lane)
+            {
+                return (j / 4) * 256 + (lane & 31) * 8 + 4 * (lane >> 5) + j % 4;
+            }
+            __device__ __forceinline__ void load_lds_part(const v4f *m, int lane, int p)
+            {
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                {
+                    const v4f v = m[(4 * q + p) * 64 + lane];
+                    a[q][4 * p + 0] = v.x;
+                    a[q][4 * p + 1] = v.y;
+                    a[q][4 * p + 2] = v.z;
+                    a[q][4 * p + 3] = v.w;
+                }
+            }
+            __device__ __forceinline__ void load_lds(const v4f *m, int lane)
+            {
+#pragma unroll
+                for (int p = 0; p < PARTS; ++p)
+                    load_lds_part(m, lane, p);
+            }
+            __device__ __forceinline__ void load(const float *R, int c, int h)
+            {
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                {
+                    const v4f *rp = reinterpret_cast<const v4f *>(R + q * 1024 + c * 32 + 16 * h);
+#pragma unroll
+                    for (int p = 0; p < 4; ++p)
+                    {
+                        const v4f v = rp[p];
+                        a[q][4 * p + 0] = v.x;
+                        a[q][4 * p + 1] = v.y;
+                        a[q][4 * p + 2] = v.z;
+                        a[q][4 * p + 3] = v.w;
+                    }
+                }
+            }
+            template <int G>
+            __device__ __forceinline__ void apply_part(const float (&x)[G][H], v16f (&d)[G][NACC], int p) const
+            {
+#pragma unroll
+                for (int st = 4 * p; st < 4 * p + 4; ++st)
+#pragma unroll
+                    for (int g = 0; g < G; ++g)
+                    {
+                        const float xr = x[g][2 * st], xi = x[g][2 * st + 1];
+                        d[g][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[0][st], xr, d[g][0], 0, 0, 0);
+                        d[g][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[1][st], xi, d[g][1], 0, 0, 0);
+                        d[g][2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[2][st], xr + xi, d[g][2], 0, 0, 0);
+                    }
+            }
+            template <int G>
+            __device__ __forceinline__ void apply_n(const float (&x)[G][H], v16f (&d)[G][NACC]) const
+            {
+#pragma unroll
+                for (int p = 0; p < PARTS; ++p)
+                    apply_part<G>(x, d, p);
+            }
+            __device__ __forceinline__ void apply(const float (&x)[H], v16f (&d)[NACC]) const
+            {
+#pragma unroll
+                for (int st = 0; st < 16; ++st)
+                {
+                    const float xr = x[2 * st], xi = x[2 * st + 1];
+                    d[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[0][st], xr, d[0], 0, 0, 0);
+                    d[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[1][st], xi, d[1], 0, 0, 0);
+                    d[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[2][st], xr + xi, d[2], 0, 0, 0);
+                }
+            }
+            // accumulator register r holds complex row 8*(r/4) + 4*h + r%4 of P1, P2, P3:
+            // amplitudes e = 8*(r/4) + r%4 (r%4 = 0 or 2, + OUT_H*h) and e + 1
+            template <class W>
+            static __device__ __forceinline__ void outputs2(const v16f (&d)[NACC], W &&w)
+            {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int i = 0; i < 4; i += 2)
+                    {
+                        const int r = 4 * q + i;
+                        const float p10 = d[0][r], p20 = d[1][r], p30 = d[2][r];
+                        const float p11 = d[0][r + 1], p21 = d[1][r + 1], p31 = d[2][r + 1];
+                        w(8 * q + i, v4f{p10 - p20, (p30 - p10) - p20, p11 - p21, (p31 - p11) - p21});
+                    }
+            }
         };
 
         // BF16x3 split precision on v_mfma_f32_32x32x16_bf16: R = Rh + Rl and x = xh + xl
@@ -198,6 +314,7 @@ namespace qsv
         struct BlockMatrix<KQ, true>
         {
             static constexpr int D = 2 << KQ, MT = D / 32, H = D / 2, ST = H / 8;
+            static constexpr int NACC = MT, OUT_H = 2; // as in the FP32 form
             // LDS copy, as in the FP32 form: the hi units, then the lo units
             static constexpr int UNITS = 2 * MT * ST;
             // the products of K step st (K values 8 st .. 8 st + 7) need part st
@@ -306,6 +423,15 @@ namespace qsv
                 for (int st = 0; st < ST; ++st)
                     apply_part<G>(x, d, st);
             }
+            template <class W>
+            static __device__ __forceinline__ void outputs2(const v16f (&d)[NACC], W &&w)
+            {
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        w(16 * mt + 4 * q, v4f{d[mt][4 * q + 0], d[mt][4 * q + 1], d[mt][4 * q + 2], d[mt][4 * q + 3]});
+            }
         };
 
         __device__ __forceinline__ uint64_t expand(uint64_t t, const Subspace &sub)
@@ -322,7 +448,6 @@ namespace qsv
         __global__ __launch_bounds__(kThreads) void k_block(c64 *__restrict__ s, uint64_t ncols, Subspace sub, BlockOp op)
         {
             constexpr int D = 2 << KQ;   // real dimension
-            constexpr int MT = D / 32;   // 32-row output tiles
             constexpr int H = D / 2;     // K values per lane
             constexpr int AMPS = H / 2;  // amplitudes per lane
 
@@ -350,25 +475,20 @@ namespace qsv
                     b[2 * j] = v.x;
                     b[2 * j + 1] = v.y;
                 }
-                v16f d[MT];
+                typedef BlockMatrix<KQ, X3> Mat;
+                v16f d[Mat::NACC];
 #pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                    d[mt] = v16f{0.f};
+                for (int i = 0; i < Mat::NACC; ++i)
+                    d[i] = v16f{0.f};
                 A.apply(b, d);
                 if (!ok)
                     continue;
-                // accumulator register r of tile mt holds row 32*mt + 8*(r/4) + 4*h + r%4
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                    {
-                        const int e0 = 16 * mt + 4 * q; // + 2h
-                        const uint64_t o0 = h ? op.off[e0 + 2] : op.off[e0];
-                        const uint64_t o1 = h ? op.off[e0 + 3] : op.off[e0 + 1];
-                        __builtin_nontemporal_store(v2f{d[mt][4 * q + 0], d[mt][4 * q + 1]}, reinterpret_cast<v2f *>(s + (base | o0)));
-                        __builtin_nontemporal_store(v2f{d[mt][4 * q + 2], d[mt][4 * q + 3]}, reinterpret_cast<v2f *>(s + (base | o1)));
-                    }
+                Mat::outputs2(d, [&](int e, v4f v) {
+                    const uint64_t o0 = h ? op.off[e + Mat::OUT_H] : op.off[e];
+                    const uint64_t o1 = h ? op.off[e + Mat::OUT_H + 1] : op.off[e + 1];
+                    __builtin_nontemporal_store(v2f{v.x, v.y}, reinterpret_cast<v2f *>(s + (base | o0)));
+                    __builtin_nontemporal_store(v2f{v.z, v.w}, reinterpret_cast<v2f *>(s + (base | o1)));
+                });
             }
         }
         // LDS slot of tile index i in the complex64 stage kernels (stage_lds_offset in slots).
@@ -417,7 +537,6 @@ namespace qsv
         __global__ __launch_bounds__(kThreads) void k_block_lds(c64 *__restrict__ s, uint64_t ntiles, Subspace tsub, BlockLdsOp op)
         {
             constexpr int D = 2 << KQ;
-            constexpr int MT = D / 32;
             constexpr int H = D / 2;
             constexpr int AMPS = H / 2;
             constexpr int K = 1 << KQ;
@@ -493,20 +612,14 @@ namespace qsv
                     b_[4 * r + 2] = w.z;
                     b_[4 * r + 3] = w.w;
                 }
-                v16f d[MT];
+                typedef BlockMatrix<KQ, X3> Mat;
+                v16f d[Mat::NACC];
 #pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                    d[mt] = v16f{0.f};
+                for (int i = 0; i < Mat::NACC; ++i)
+                    d[i] = v16f{0.f};
                 A.apply(b_, d);
                 wave_sync();
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                    {
-                        const int e0 = 16 * mt + 4 * q + 2 * h;
-                        *reinterpret_cast<v4f *>(&tl[c * ROW + e0]) = v4f{d[mt][4 * q + 0], d[mt][4 * q + 1], d[mt][4 * q + 2], d[mt][4 * q + 3]};
-                    }
+                Mat::outputs2(d, [&](int e, v4f v) { *reinterpret_cast<v4f *>(&tl[c * ROW + e + Mat::OUT_H * h]) = v; });
                 wave_sync();
 #pragma unroll
                 for (int j = 0; j < NJ; ++j)
@@ -689,7 +802,6 @@ namespace qsv
         __device__ __forceinline__ void stage_block_at(v2f *tile, const BlockMatrix<KQ, X3> &A, LoadPart &&load_part, const BlockOffsets<KQ, M> &O, int wave, int c, int h)
         {
             constexpr int D = 2 << KQ;
-            constexpr int MT = D / 32;
             constexpr int H = D / 2;
             constexpr int AMPS = H / 2;
             constexpr int NT = 1 << M;
@@ -722,7 +834,7 @@ namespace qsv
                 lanep ^= uint32_t(-int((c >> j) & 1)) & cb[j];
             const uint32_t hmask = uint32_t(-h);
             const uint32_t g_lane = lanep ^ (hmask & amp_off(AMPS));
-            const uint32_t s_lane = lanep ^ (hmask & amp_off(2));
+            const uint32_t s_lane = lanep ^ (hmask & amp_off(BlockMatrix<KQ, X3>::OUT_H));
             const char *tbytes = reinterpret_cast<const char *>(tile);
             char *wbytes = reinterpret_cast<char *>(tile);
             // G column groups per pass (independent MFMA chains).  The next pass's gathers are
@@ -770,12 +882,13 @@ namespace qsv
             for (int gr0 = 0; gr0 < PER_WAVE; gr0 += G)
             {
                 const bool more = AHEAD && gr0 + G < PER_WAVE;
-                v16f d[G][MT];
+                constexpr int NACC = BlockMatrix<KQ, X3>::NACC;
+                v16f d[G][NACC];
 #pragma unroll
                 for (int g = 0; g < G; ++g)
 #pragma unroll
-                    for (int mt = 0; mt < MT; ++mt)
-                        d[g][mt] = v16f{0.f};
+                    for (int i = 0; i < NACC; ++i)
+                        d[g][i] = v16f{0.f};
                 if (PIPE && gr0 == 0)
                 {
                     group_base(0, U);
@@ -804,21 +917,16 @@ namespace qsv
                         gather(gr0 + G, xn, Un);
                     A.template apply_n<G>(x, d);
                 }
-                // accumulator register r of tile mt holds real row 32*mt + 8*(r/4) + 4*h + r%4,
-                // i.e. amplitudes 16*mt + 4*(r/4) + 2*h and the next one
+                // amplitudes e and e + 1 (e even, plus OUT_H*h in s_lane) of each column
 #pragma unroll
                 for (int g = 0; g < G; ++g)
                 {
                     const uint32_t sbase = s_lane ^ U[g];
-#pragma unroll
-                    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                        {
-                            const uint32_t o0 = sbase ^ amp_off(16 * mt + 4 * q);
-                            *reinterpret_cast<v2f *>(wbytes + o0) = v2f{d[g][mt][4 * q + 0], d[g][mt][4 * q + 1]};
-                            *reinterpret_cast<v2f *>(wbytes + (o0 ^ tb[0])) = v2f{d[g][mt][4 * q + 2], d[g][mt][4 * q + 3]};
-                        }
+                    BlockMatrix<KQ, X3>::outputs2(d[g], [&](int e, v4f v) {
+                        const uint32_t o0 = sbase ^ amp_off(e);
+                        *reinterpret_cast<v2f *>(wbytes + o0) = v2f{v.x, v.y};
+                        *reinterpret_cast<v2f *>(wbytes + (o0 ^ tb[0])) = v2f{v.z, v.w};
+                    });
                 }
                 if (gr0 + G < PER_WAVE)
                 {

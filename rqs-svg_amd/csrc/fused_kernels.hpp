@@ -7,8 +7,10 @@ namespace qsv
     // One dense block on kq qubits: off[e] is the amplitude offset of local basis state e
     // (bit j of e = the block's j-th qubit); R points to the (2^(kq+1))^2 real matrix on
     // interleaved (re, im) vectors, row-major, in device memory.  x3 = 0: R is FP32 and the
-    // product runs on FP32 MFMA; x3 = 1: R holds the bfloat16 high parts of the D*D entries
-    // followed by their bfloat16 low parts (same byte size), BF16x3 split-precision MFMA.
+    // product runs on FP32 MFMA (kq = 5: R holds Re M, Im M and Re M + Im M instead, each
+    // 32 x 32 row-major, for three real products in place of four); x3 = 1: R holds the
+    // bfloat16 high parts of the D*D entries followed by their bfloat16 low parts (same byte
+    // size), BF16x3 split-precision MFMA.
     struct BlockOp
     {
         int kq;
