@@ -208,8 +208,8 @@ int qcs_ext_batch_run_direct(qcs_ext_batch *batch, double *expz, uint64_t count,
  * default plan's by the model's own typical error: a factor 2^-rms, rms = the root mean
  * squared error (log2 microseconds) of the model's last qcs_ext_mlp_train on its data,
  * within [0.75, 0.97] (0.97 for parameters set by qcs_ext_mlp_set_params).  Without a model
- * the default plan is used.  With the measured check (qcs_ext_planner_set_judge, on by
- * default) the default plan also judges each override by measurement: a flush structure
+ * the default plan is used.  With the measured check (qcs_ext_planner_set_check, on by
+ * default) the default plan also checks each override by measurement: a flush structure
  * keeps the model's choice only if it ran faster than the default plan.  The caller keeps
  * the model alive while it is attached. */
 typedef struct qcs_ext_mlp qcs_ext_mlp;
@@ -251,7 +251,7 @@ int qcs_ext_planner_last_choice(qcs_simulator *sim, int *plan);
  * (qcs_ext_set_stage_bits, qcs_ext_set_stage_stream) forgets the checks; at most 1024
  * structures are checked per simulator (later ones run the model's choice).  Queued gates
  * are flushed first. */
-int qcs_ext_planner_set_judge(qcs_simulator *sim, int on);
+int qcs_ext_planner_set_check(qcs_simulator *sim, int on);
 
 /* Per-kernel-class timing with HIP events around every launch (oracle: host clock). */
 int qcs_ext_profile(qcs_simulator *sim, int enable);

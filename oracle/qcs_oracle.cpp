@@ -1418,12 +1418,12 @@ extern "C"
                 throw std::runtime_error("stage stream setting must be 0 or 1");
         });
     }
-    int qcs_ext_planner_set_judge(qcs_simulator *s, int on)
+    int qcs_ext_planner_set_check(qcs_simulator *s, int on)
     { // the oracle has no fused path; the setting is validated and ignored
         return guard([&] {
             (void)core(s);
             if (on != 0 && on != 1)
-                throw std::runtime_error("planner judge setting must be 0 or 1");
+                throw std::runtime_error("planner check setting must be 0 or 1");
         });
     }
     int qcs_ext_set_plan(qcs_simulator *s, int, int, int, int, int, int)

@@ -423,7 +423,7 @@ extern "C"
     int qcs_ext_permute(qcs_simulator *s, const bit_num_t *p, bit_num_t count) { return guard([&] { core(s).permute(vec(p, count)); }); }
     int qcs_ext_set_stage_bits(qcs_simulator *s, int m) { return guard([&] { core(s).set_stage_bits(m); }); }
     int qcs_ext_set_stage_stream(qcs_simulator *s, int on) { return guard([&] { core(s).set_stage_stream(on); }); }
-    int qcs_ext_planner_set_judge(qcs_simulator *s, int on) { return guard([&] { core(s).set_planner_judge(on); }); }
+    int qcs_ext_planner_set_check(qcs_simulator *s, int on) { return guard([&] { core(s).set_planner_check(on); }); }
     int qcs_ext_set_precision(qcs_simulator *s, int mode, int k) { return guard([&] { core(s).set_precision(mode, k); }); }
     int qcs_ext_flush(qcs_simulator *s) { return guard([&] { core(s).flush(); }); }
     int qcs_ext_set_plan(qcs_simulator *s, int en, int bits, int k, int tiles, int window, int rule)

@@ -256,30 +256,30 @@ namespace qcore
             (void)hipEventDestroy(copy_done_);
         for (hipEvent_t e : ev_pool_)
             (void)hipEventDestroy(e);
-        clear_verdicts();
+        clear_trials();
     }
 
-    void FusedEngine::clear_verdicts()
+    void FusedEngine::clear_trials()
     {
-        for (auto &kv : verdicts_)
+        for (auto &kv : trials_)
             for (hipEvent_t e : kv.second.ev)
                 if (e)
                     (void)hipEventDestroy(e);
-        verdicts_.clear();
+        trials_.clear();
     }
 
     void FusedEngine::set_model(CostModel *m)
     {
         // measured checks belong to the model that made the choices
         if (m != model_)
-            clear_verdicts();
+            clear_trials();
         model_ = m;
     }
 
-    void FusedEngine::set_judge(bool on)
+    void FusedEngine::set_check(bool on)
     {
-        judge_ = on;
-        clear_verdicts();
+        check_ = on;
+        clear_trials();
     }
 
     uint64_t FusedEngine::structure_key(const std::vector<ResolvedGate> &gates)
@@ -522,24 +522,24 @@ namespace qcore
                 best = oi;
         if (!(best != 0 && std::isfinite(cost[0]) && cost[best] < model_->choice_factor() * cost[0]))
             best = 0;
-        // The default plan judges an override by measurement: the first flush of this
+        // The default plan checks an override by measurement: the first flush of this
         // structure runs the model's choice timed, the second the default timed, and the
         // later ones the faster of the two.
         size_t run = best;
         hipEvent_t t0 = nullptr, t1 = nullptr;
-        if (best != 0 && judge_)
+        if (best != 0 && check_)
         {
             const uint64_t key = structure_key(gates);
-            auto it = verdicts_.find(key);
-            if (it == verdicts_.end() && verdicts_.size() < kMaxVerdicts)
+            auto it = trials_.find(key);
+            if (it == trials_.end() && trials_.size() < kMaxTrials)
             {
-                it = verdicts_.emplace(key, Verdict{}).first;
+                it = trials_.emplace(key, PlanTrial{}).first;
                 for (hipEvent_t &e : it->second.ev)
                     qsv::check(hipEventCreate(&e), "hipEventCreate");
             }
-            if (it != verdicts_.end())
+            if (it != trials_.end())
             {
-                Verdict &v = it->second;
+                PlanTrial &v = it->second;
                 if (v.stage == 0 || v.option != best)
                 {
                     v.stage = 1;
@@ -1642,7 +1642,7 @@ namespace qcore
     {
         flush();
         if (on != stage_stream_)
-            clear_verdicts(); // measured with the other kernel
+            clear_trials(); // measured with the other kernel
         stage_stream_ = on;
     }
 
@@ -1652,7 +1652,7 @@ namespace qcore
             throw std::runtime_error("stage tile bits must be 0 (per-block launches), 11, 12, 13 or 14");
         flush();
         if (m != stage_bits_)
-            clear_verdicts(); // the candidate plans (and the default) change
+            clear_trials(); // the candidate plans (and the default) change
         stage_bits_ = m;
     }
 

@@ -229,7 +229,7 @@ namespace qcore
         // model picks a non-default plan runs that plan with events around it, the next flush
         // of the same structure runs the default plan timed, and later ones run whichever of
         // the two was faster.  Off: every flush runs the model's choice.
-        void set_judge(bool on);
+        void set_check(bool on);
         // A fixed plan for every flush without a model: stages on tiles of 2^tile_bits
         // amplitudes with k-qubit blocks and tile choice `tiles` (StageTiles), or, with
         // tile_bits 0, one launch per block planned with k, window and rule (plan_rule).
@@ -293,18 +293,18 @@ namespace qcore
         PlanOption last_choice_{0, 0, 0, 0, kTilesAuto};
         // one flush structure's measured check: stage 1 after the timed run of the model's
         // choice (option), 2 after the timed run of the default plan, 3 once decided
-        struct Verdict
+        struct PlanTrial
         {
             int stage = 0;
             size_t option = 0;
             bool learned = false; // decided: the model's choice was faster
             hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
         };
-        static constexpr size_t kMaxVerdicts = 1024;
+        static constexpr size_t kMaxTrials = 1024;
         static uint64_t structure_key(const std::vector<ResolvedGate> &gates);
-        void clear_verdicts();
-        bool judge_ = true;
-        std::unordered_map<uint64_t, Verdict> verdicts_;
+        void clear_trials();
+        bool check_ = true;
+        std::unordered_map<uint64_t, PlanTrial> trials_;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> *launch_log_ = nullptr;
         std::vector<hipEvent_t> ev_pool_;
         size_t ev_used_ = 0;

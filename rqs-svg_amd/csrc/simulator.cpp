@@ -145,14 +145,14 @@ namespace qcore
             fused_->set_stage_stream(on != 0);
     }
 
-    void Simulator::set_planner_judge(int on)
+    void Simulator::set_planner_check(int on)
     {
         if (on != 0 && on != 1)
-            throw std::runtime_error("planner judge setting must be 0 or 1");
+            throw std::runtime_error("planner check setting must be 0 or 1");
         flush();
-        planner_judge_ = on;
+        planner_check_ = on;
         if (fused_)
-            fused_->set_judge(on != 0);
+            fused_->set_check(on != 0);
     }
 
     void Simulator::reinitialize_mapping()
@@ -203,7 +203,7 @@ namespace qcore
             fused_ = new FusedEngine(*this, block_qubits_, precision_);
             fused_->set_stage_bits(stage_bits_);
             fused_->set_stage_stream(stage_stream_ != 0);
-            fused_->set_judge(planner_judge_ != 0);
+            fused_->set_check(planner_check_ != 0);
         }
         work_len_ = qsv::reduce_workspace(count) + 16;
         qsv::check(hipMalloc((void **)&work_, work_len_ * sizeof(double)), "hipMalloc workspace");

@@ -69,7 +69,7 @@ namespace qcore
         void set_precision(int mode, int k);
         void set_stage_bits(int m);
         void set_stage_stream(int on);
-        void set_planner_judge(int on);
+        void set_planner_check(int on);
         void flush();
         void overlap(Simulator &other, double out[3]);
         void set_planner_model(CostModel *m);
@@ -127,7 +127,7 @@ namespace qcore
         int precision_ = 0;
         int stage_bits_ = 13;
         int stage_stream_ = 1;
-        int planner_judge_ = 1;
+        int planner_check_ = 1;
         bool c64_state() const { return precision_ == 1 || precision_ == 3; }
         int block_qubits_ = 5;
         qsv::c128 *state_ = nullptr; // complex128 state (precision 0)

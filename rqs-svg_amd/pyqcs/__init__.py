@@ -127,7 +127,7 @@ class Library:
         lib.qcs_ext_permute.argtypes = [vp, P(bit_num_t), bit_num_t]
         lib.qcs_ext_set_stage_bits.argtypes = [vp, ctypes.c_int]
         lib.qcs_ext_set_stage_stream.argtypes = [vp, ctypes.c_int]
-        lib.qcs_ext_planner_set_judge.argtypes = [vp, ctypes.c_int]
+        lib.qcs_ext_planner_set_check.argtypes = [vp, ctypes.c_int]
         lib.qcs_ext_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.qcs_ext_flush.argtypes = [vp]
         lib.qcs_ext_set_plan.argtypes = [vp] + [ctypes.c_int] * 6
@@ -546,11 +546,11 @@ def set_planner(sim, model):
     sim._call(sim.lib.qcs_ext_set_planner(sim.h, model.h if model is not None else None), "set_planner")
 
 
-def planner_set_judge(sim, on):
+def planner_set_check(sim, on):
     """Measured check of the model's overrides (default on): per flush structure, the model's
     non-default choice runs timed once, the default plan runs timed once, and later flushes
     of that structure run the faster of the two.  Off: every flush runs the model's choice."""
-    sim._call(sim.lib.qcs_ext_planner_set_judge(sim.h, int(bool(on))), "planner_set_judge")
+    sim._call(sim.lib.qcs_ext_planner_set_check(sim.h, int(bool(on))), "planner_set_check")
 
 
 def planner_stats(sim):

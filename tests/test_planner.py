@@ -190,7 +190,7 @@ def test_learned_planner_measured_gain(gpu):
 
 
 @pytest.mark.gpu
-def test_planner_judge_keeps_the_faster_plan(gpu, oracle):
+def test_planner_check_keeps_the_faster_plan(gpu, oracle):
     # A hand-set model whose prediction falls with the tile bits (feature 17 = m / 16, through
     # one hidden unit of each layer: log2 us = b3 - 32 * m / 16; block launches, with feature
     # 17 = 0, are priced at b3) rates a stage launch on 2^14 tiles 4x cheaper than one on
@@ -216,10 +216,10 @@ def test_planner_judge_keeps_the_faster_plan(gpu, oracle):
         o.run(circ)
         want = o.statevector()
     try:
-        for judge in (True, False):
+        for check in (True, False):
             with pyqcs.Simulator(gpu, n, seed=1, precision=3, block_qubits=5) as sim:
                 pyqcs.set_planner(sim, model)
-                pyqcs.planner_set_judge(sim, judge)
+                pyqcs.planner_set_check(sim, check)
                 choices = []
                 for r in range(4):
                     sim.call("set_zero_state")
@@ -228,9 +228,9 @@ def test_planner_judge_keeps_the_faster_plan(gpu, oracle):
                     choices.append(tuple(pyqcs.planner_last_choice(sim)))
                 st = pyqcs.planner_stats(sim)
                 pyqcs.set_planner(sim, None)
-            print(f"judge {judge}: choices {choices}, stats {st}", flush=True)
+            print(f"check {check}: choices {choices}, stats {st}", flush=True)
             assert st["flushes"] == 4
-            if judge:
+            if check:
                 learned, default = choices[0], choices[1]
                 assert learned[0] == 14 and default[:3] == (13, 5, 2)
                 assert choices[2] == choices[3] and choices[2] in (learned, default)
