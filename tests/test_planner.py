@@ -240,3 +240,48 @@ def test_planner_check_keeps_the_faster_plan(gpu, oracle):
                 assert st["learned_choices"] == 4
     finally:
         model.close()
+
+
+@pytest.mark.gpu
+def test_planner_check_per_structure_and_reset(gpu):
+    # The measured check keeps one record per flush structure (gate names, kinds and
+    # qubits; parameters ignored): two circuits of different structure run in alternation
+    # each go through model plan -> default plan -> decision on their own; a circuit that
+    # differs from one of them only in its angles takes that structure's decision at once;
+    # and a change of the stage tile bits forgets the records (the next run is the model's).
+    n = 20
+    F = pyqcs.planner_num_features(gpu)
+    model = pyqcs.Mlp(gpu, F, seed=1)
+    p = np.zeros(model.P, dtype=np.float32)
+    p[17] = 1.0
+    p[H * F + H] = 1.0
+    p[H * F + H + H * H + H] = -32.0
+    p[-1] = np.log2(100.0) + 28.0
+    model.set_params(p)
+    circs = [circuits.brick(n, 6, seed=11), circuits.brick(n, 5, seed=12)]
+    same_structure = circuits.brick(n, 6, seed=13)  # circs[0] with other angles
+
+    def run(sim, circ):
+        sim.call("set_zero_state")
+        sim.run(circ)
+        sim.flush()
+        return tuple(pyqcs.planner_last_choice(sim))
+
+    try:
+        with pyqcs.Simulator(gpu, n, seed=1, precision=3, block_qubits=5) as sim:
+            pyqcs.set_planner(sim, model)
+            seq = {0: [], 1: []}
+            for r in range(3):
+                for i, circ in enumerate(circs):
+                    seq[i].append(run(sim, circ))
+            print(f"alternating: {seq}", flush=True)
+            for i in (0, 1):
+                learned, default, decided = seq[i]
+                assert learned != default and default[:3] == (13, 5, 2) and decided in (learned, default)
+            assert run(sim, same_structure) == seq[0][2]
+            sim.set_stage_bits(12)
+            sim.set_stage_bits(13)
+            assert run(sim, circs[0]) == seq[0][0]  # checked afresh
+            pyqcs.set_planner(sim, None)
+    finally:
+        model.close()
