@@ -15,7 +15,9 @@
 // from HBM -- and keeps its rows of R in registers for the whole launch.  The D/32
 // accumulators hold, per lane, pairs of consecutive amplitudes of column c, written back
 // in place.  Accumulation is FP32.  The BF16x3 form (BlockMatrix<KQ, true>) runs the same
-// K split in steps of 8 values per lane on the bf16 MFMA.
+// K split in steps of 8 values per lane on the bf16 MFMA.  FP32 5-qubit blocks
+// (BlockMatrix<5, false>) take M as Re M, Im M and Re M + Im M instead and form y with
+// three real 32 x 32 products (see there).
 #include "fused_kernels.hpp"
 
 #include <algorithm>
