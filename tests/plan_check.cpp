@@ -9,8 +9,13 @@
 // controls: the phase only where every control bit is 1); and the trailing tables of
 // stages of dense gates followed by diagonal ones (the greedy blocks after the last dense
 // block, applied as a table at the store): the LDS block count, the table's alignment and
-// every entry against the trailing gates' phases.  Prints one line per case and "ok" at
-// the end; exits 1 at the first violation.
+// every entry against the trailing gates' phases.  Also the complex64 LDS tile layout
+// (qsv::stage_lds_offset: a bijection onto 8-B slots that keeps amplitude pairs 2j, 2j + 1
+// in one aligned 16-B slot and is XOR-linear over disjoint index bits), the swizzled
+// offsets the complex64 stage descriptors carry, and the FP32 block matrices' layouts (the
+// real form's [[Re, -Im], [Im, Re]] pattern for 4-qubit blocks; Re, Im and Re + Im for
+// 5-qubit blocks).  Prints one line per case and "ok" at the end; exits 1 at the first
+// violation.
 // Usage: plan_check <cases> <seed>
 #include "fused.hpp"
 
@@ -56,10 +61,73 @@ namespace
     }
 } // namespace
 
+// the complex64 stage kernels' LDS slot map on tiles of 2^m amplitudes
+static void check_lds_offsets(uint64_t seed)
+{
+    for (int m = 11; m <= 14; ++m)
+    {
+        const uint32_t nt = 1u << m;
+        std::vector<char> seen(nt, 0);
+        for (uint32_t i = 0; i < nt; ++i)
+        {
+            const uint32_t off = qsv::stage_lds_offset(i);
+            if (off % 8 != 0 || off / 8 >= nt || seen[off / 8]++)
+                fail("stage_lds_offset is not a bijection onto the tile's 8-B slots", m);
+            if ((i & 1) == 0 && (off % 16 != 0 || qsv::stage_lds_offset(i | 1) != off + 8))
+                fail("stage_lds_offset splits an amplitude pair", m);
+        }
+        for (int r = 0; r < 1000; ++r)
+        {
+            const uint32_t a = uint32_t(mix(seed)) & (nt - 1), mask = uint32_t(mix(seed)) & (nt - 1);
+            const uint32_t x = a & mask, y = a & ~mask;
+            if (qsv::stage_lds_offset(x | y) != (qsv::stage_lds_offset(x) ^ qsv::stage_lds_offset(y)))
+                fail("stage_lds_offset is not XOR-linear over disjoint bits", m);
+        }
+    }
+    std::printf("lds offsets: m 11..14 ok\n");
+}
+
+// complex64 stage descriptors and FP32 block matrices of a prepared dense stage
+static void check_c64_stage(const PreparedStage &p, const std::vector<float> &mats, int kb, int m, int c)
+{
+    const auto *desc = reinterpret_cast<const qsv::StageBlock *>(mats.data() + p.desc_offset);
+    for (size_t b = 0; b < p.nblocks; ++b)
+    {
+        const qsv::StageBlock &sb = desc[b];
+        for (int j = 0; j < m; ++j)
+            if (sb.lds[j] != qsv::stage_lds_offset(uint32_t(sb.w[j])))
+                fail("descriptor LDS offset differs from the swizzled weight", c);
+        const float *R = mats.data() + sb.matrix;
+        if (kb == 5)
+        { // Re M, Im M, Re M + Im M (the sum formed in FP64, then rounded)
+            double norm = 0;
+            for (int i = 0; i < 1024; ++i)
+            {
+                const double re = R[i], im = R[1024 + i], sum = R[2048 + i];
+                if (std::abs(sum - (re + im)) > 2.5e-7 * (std::abs(re) + std::abs(im)) + 1e-30)
+                    fail("5-qubit FP32 matrix: third part is not Re + Im", c);
+                norm += re * re + im * im;
+            }
+            if (std::abs(norm - 32.0) > 1e-4) // a unitary 32 x 32 matrix: sum of |entries|^2 = 32
+                fail("5-qubit FP32 matrix: Re and Im parts are not those of a unitary", c);
+        }
+        else
+        { // real form R[2r][2c] = R[2r+1][2c+1] = Re, R[2r+1][2c] = -R[2r][2c+1] = Im
+            const int D = 2 << kb;
+            for (int r = 0; r < D / 2; ++r)
+                for (int q = 0; q < D / 2; ++q)
+                    if (R[(2 * r) * D + 2 * q] != R[(2 * r + 1) * D + 2 * q + 1] ||
+                        R[(2 * r + 1) * D + 2 * q] != -R[(2 * r) * D + 2 * q + 1])
+                        fail("FP32 real-form block matrix pattern broken", c);
+        }
+    }
+}
+
 int main(int argc, char **argv)
 {
     const int cases = argc > 1 ? std::atoi(argv[1]) : 50;
     uint64_t seed = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 1;
+    check_lds_offsets(seed);
     // gate names by target count; p, rz, rzz, z, s, t are diagonal, the rest not
     const char *one[] = {"h", "x", "z", "s", "t", "p", "rz", "rx", "u3", "sx"};
     const char *two[] = {"swap", "rzz", "rxx", "ecr", "iswap"};
@@ -337,6 +405,7 @@ int main(int argc, char **argv)
                         fail("a stage with dense gates ran as a table", c);
                     if (toff != SIZE_MAX && toff % 4 != 0)
                         fail("complex64 trailing table not 4-float aligned", c);
+                    check_c64_stage(p, mats, kb, m, c);
                 }
                 if ((int)nblocks != lds)
                     fail("LDS block count differs from the blocks up to the last dense one", c);
